@@ -1,0 +1,180 @@
+/*
+ * tse_hip.h -- C-ABI of the MI355X (gfx950) TRANSPORTSE SOR / transport core.
+ *
+ * Plain C: device pointers, sizes, POD structs and int status codes; no
+ * torch or HIP types cross this boundary (streams are passed as void*).
+ *
+ * Device field layout ("checkerboard split"): a field of the nx*ny*nz grid
+ * is 2*S doubles, S = nxh*ny*nz, nxh = (nx+1)/2.  Cell (i,j,k) has colour
+ * c = (i+j+k) & 1 (0 = red, swept first, tse.py:170-176) and lives at
+ *     c*S + (k*ny + j)*nxh + (i >> 1).
+ * Each colour half is x-contiguous; slots with 2*(i>>1)+(c^((j+k)&1)) >= nx
+ * (odd nx only) are padding and hold 0.  tse_to_split / tse_from_split
+ * convert from / to any strided (i,j,k) layout, e.g. numpy C order
+ * (nx,ny,nz) with strides (ny*nz, nz, 1) or a species view C[:,:,:,n] with
+ * strides (ny*nz*S, nz*S, S).
+ *
+ * Return codes: 0 = success, < 0 = error (tse_last_error() has the text).
+ * Solver non-convergence is not an error: it is reported in tse_result,
+ * as the reference returns converged=False (tse.py:742, 916, 1081).
+ * One context = one stream; calls on one context must not overlap.
+ */
+#ifndef TSE_HIP_H
+#define TSE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSE_ABI_VERSION 1
+
+enum tse_status {
+    TSE_OK = 0,
+    TSE_ERR_ARG = -1,     /* invalid argument / shape           */
+    TSE_ERR_HIP = -2,     /* HIP runtime error                  */
+    TSE_ERR_STATE = -3,   /* grid not set / context misuse      */
+    TSE_ERR_NOMEM = -4    /* device allocation failed           */
+};
+
+/* Solver forms for tse_compute_p. */
+enum tse_p_form {
+    TSE_FORM_FAITHFUL64 = 0, /* reference arithmetic, bit-identical      */
+    TSE_FORM_PLAIN64 = 1,    /* precomputed symmetric operator, FP64     */
+    TSE_FORM_MIXED = 2       /* FP32 sweeps in FP64 defect correction    */
+};
+
+typedef struct tse_ctx tse_ctx;
+
+/* Scalar results of one solver call: the non-array part of
+ * (t_P, dt_P, P, Pn, converged, num_it) of compute_P (tse.py:742) and
+ * (t, dt, X, converged, num_it) of compute_C / compute_T (tse.py:916,1081). */
+typedef struct {
+    double t;          /* accepted time t_P / t_C / t_T                 */
+    double dt;         /* next step dt_P / dt_C / dt_T                  */
+    int32_t converged; /* 1 converged, 0 not (dt fell to min_dt)        */
+    int32_t num_it;    /* iterations, including halved attempts         */
+    int32_t num_halvings;
+    int32_t status;    /* internal final state (1 conv, 3 dt<=min_dt)   */
+    double last_diff;  /* last evaluated change norm / residual         */
+} tse_result;
+
+/* SOR control parameters: the keyword arguments of compute_P / compute_C /
+ * compute_T (tse.py:649-651, 785-787, 921-923). omega is the OMEGA time
+ * weight (transport only). */
+typedef struct {
+    double dt;
+    double sor;
+    double omega;
+    double inc_it;
+    double max_dt;
+    double min_dt;
+    double conv;
+    int32_t max_it;
+    int32_t form;      /* enum tse_p_form (compute_P only)               */
+} tse_sor_params;
+
+/* Inputs of compute_P (tse.py:649). P is updated in place; Pn receives P at
+ * entry.  c_r is the frozen module global (tse.py:688).  qx/qy/qz are not
+ * used by the reference kernel and are not taken. */
+typedef struct {
+    double *P, *Pn;
+    const double *c_f, *nue, *rho_f, *phi, *Kx, *Ky, *Kz, *BP, *WP, *c_r;
+} tse_p_fields;
+
+/* Inputs of compute_C (tse.py:785) for one species; lf_l / lf_t are the
+ * frozen dispersion lengths, imblflo is config.C_IMBLFLO (tse.py:865). */
+typedef struct {
+    double *C, *Cn;
+    const double *D_fx, *D_fy, *D_fz, *phi, *rho_f, *qx, *qy, *qz, *BC, *WC,
+        *RC, *e, *lf_l, *lf_t;
+    int32_t imblflo;
+} tse_c_fields;
+
+/* Inputs of compute_T (tse.py:921); l_r, cp_r, rho_r, lt_l, lt_t are the
+ * frozen module globals, imblflo is config.T_IMBLFLO (tse.py:1033). */
+typedef struct {
+    double *T, *Tn;
+    const double *l_f, *cp_f, *rho_f, *phi, *qx, *qy, *qz, *BT, *WT, *e,
+        *l_r, *cp_r, *rho_r, *lt_l, *lt_t;
+    int32_t imblflo;
+} tse_t_fields;
+
+/* ---- library / context ------------------------------------------------ */
+int tse_abi_version(void);
+const char *tse_last_error(void);
+
+/* Create a context on HIP device `device` issuing work on `stream`
+ * (a hipStream_t; NULL = the default stream). */
+int tse_ctx_create(tse_ctx **out, int device, void *stream);
+int tse_ctx_destroy(tse_ctx *ctx);
+int tse_ctx_set_stream(tse_ctx *ctx, void *stream);
+/* Blocks until all work issued on the context's stream is done. */
+int tse_ctx_synchronize(tse_ctx *ctx);
+
+/* Grid geometry (tse.py:82-94): host arrays dx[nx], dy[ny], dz[nz];
+ * sin_beta / cos_beta of the slope angle in radians. */
+int tse_set_grid(tse_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                 const double *dx, const double *dy, const double *dz,
+                 double g, double sin_beta, double cos_beta);
+/* Doubles per device field (2*S). */
+int64_t tse_field_elems(const tse_ctx *ctx);
+
+/* ---- layout conversion ------------------------------------------------ */
+/* dst_split[cell] = src[i*si + j*sj + k*sk] (element strides). */
+int tse_to_split(tse_ctx *ctx, double *dst_split, const double *src,
+                 int64_t si, int64_t sj, int64_t sk);
+/* dst[i*si + j*sj + k*sk] = src_split[cell]. */
+int tse_from_split(tse_ctx *ctx, double *dst, int64_t si, int64_t sj,
+                   int64_t sk, const double *src_split);
+
+/* ---- hot path ------------------------------------------------------------ */
+/* compute_P (tse.py:647-742): red-black SOR with the reference's norm
+ * schedule and accept / grow / halve / reset control. */
+int tse_compute_p(tse_ctx *ctx, const tse_p_fields *f,
+                  const tse_sor_params *prm, tse_result *out);
+
+/* compute_q (tse.py:744-780): writes qx, qy, qz in place; the last index in
+ * each direction keeps its previous value, as in the reference. */
+int tse_compute_q(tse_ctx *ctx, const double *P, const double *BP,
+                  const double *Kx, const double *Ky, const double *Kz,
+                  double *qx, double *qy, double *qz, const double *nue,
+                  const double *rho_f);
+
+/* flow_error (tse.py:409-446): writes e for every cell. */
+int tse_flow_error(tse_ctx *ctx, const double *P, const double *Pn,
+                   const double *qx, const double *qy, const double *qz,
+                   const double *WP, const double *phi, const double *c_f,
+                   const double *c_r, const double *rho_f, double dt_P,
+                   double *e);
+
+/* compute_C (tse.py:783-916) for one species slice. */
+int tse_compute_c(tse_ctx *ctx, const tse_c_fields *f,
+                  const tse_sor_params *prm, tse_result *out);
+
+/* compute_T (tse.py:919-1081). */
+int tse_compute_t(tse_ctx *ctx, const tse_t_fields *f,
+                  const tse_sor_params *prm, tse_result *out);
+
+/* l2_norm (tse.py:273-281) of two device fields: a parallel FP64
+ * reduction (rounding differs from Numba's sequential sum by O(1e-16)). */
+int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
+                double *out);
+
+/* l2_norm on host arrays of n doubles with Numba's sequential C-order sums
+ * (used by the SIA density test, tse.py:1426). Returns the norm, or 1.0 and
+ * *is_nan = 1 on NaN. */
+double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
+                            int32_t *is_nan);
+
+/* ---- measurement hooks -------------------------------------------------- */
+/* Runs n_iter full red+black iterations of the faithful compute_P sweep at
+ * fixed dt (no norm, no control), for timing and sweep-level parity. */
+int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
+                 int32_t n_iter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,453 @@
+// tse_core.hip -- context, layout conversion and the C-ABI entry points.
+//
+// The SOR solvers run as a queue of (red sweep, black sweep, control) launch
+// triples on the context's stream.  The control kernel keeps the solver state
+// (iteration counters, t, dt, status) in device memory and the sweeps return
+// immediately once the solve has stopped, so the host enqueues iterations in
+// growing chunks and synchronises once per chunk, not once per iteration.
+// Only a dt halving needs the host: it restores the field, halves dt,
+// re-assembles the coefficient records (the storage term depends on dt) and
+// resumes.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "tse_internal.h"
+
+namespace tse {
+
+static thread_local std::string g_err;
+
+void set_error(const std::string &msg) { g_err = msg; }
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+int check_ctx(tse_ctx *ctx) {
+    if (!ctx) return fail(TSE_ERR_ARG, "null context");
+    if (!ctx->grid_set) return fail(TSE_ERR_STATE, "grid not set");
+    TSE_HIP_TRY(hipSetDevice(ctx->device));
+    return 0;
+}
+
+int ensure_rec(tse_ctx *ctx, size_t elems) {
+    if (ctx->rec_elems >= elems) return 0;
+    if (ctx->rec) {
+        TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        TSE_HIP_TRY(hipFree(ctx->rec));
+        ctx->rec = nullptr;
+        ctx->rec_elems = 0;
+    }
+    if (hipMalloc((void **)&ctx->rec, elems * sizeof(double)) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "hipMalloc of coefficient records failed");
+    ctx->rec_elems = elems;
+    return 0;
+}
+
+int ensure_mask(tse_ctx *ctx, size_t elems) {
+    if (ctx->mask_elems >= elems) return 0;
+    if (ctx->mask) {
+        TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        TSE_HIP_TRY(hipFree(ctx->mask));
+        ctx->mask = nullptr;
+        ctx->mask_elems = 0;
+    }
+    if (hipMalloc((void **)&ctx->mask, elems) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "hipMalloc of cell mask failed");
+    ctx->mask_elems = elems;
+    return 0;
+}
+
+// ---- layout conversion ---------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock)
+to_split_kernel(Geo G, double *__restrict__ dst, const double *__restrict__ src,
+                long si, long sj, long sk) {
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        dst[t] = q.valid ? src[q.i * si + q.j * sj + q.k * sk] : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+from_split_kernel(Geo G, double *__restrict__ dst, long si, long sj, long sk,
+                  const double *__restrict__ src) {
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        if (q.valid) dst[q.i * si + q.j * sj + q.k * sk] = src[t];
+    }
+}
+
+static inline int conv_grid(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min(b, 16384L));
+}
+
+// ---- SOR driver loop shared by compute_P / compute_C / compute_T --------------------------
+
+struct SolveHooks {
+    // re-evaluate the coefficient records for time step dt
+    int (*assemble)(tse_ctx *, const void *, double, double);
+    const void *fields;
+    double *X;   // unknown (in/out)
+    double *Xn;  // copy at entry
+    bool transport;
+};
+
+static int p_assemble_hook(tse_ctx *ctx, const void *f, double dt, double) {
+    return launch_p_assemble(ctx, (const tse_p_fields *)f, dt);
+}
+static int c_assemble_hook(tse_ctx *ctx, const void *f, double dt,
+                           double omega) {
+    return launch_c_assemble(ctx, (const tse_c_fields *)f, dt, omega);
+}
+static int t_assemble_hook(tse_ctx *ctx, const void *f, double dt,
+                           double omega) {
+    return launch_t_assemble(ctx, (const tse_t_fields *)f, dt, omega);
+}
+
+static int write_ctl(tse_ctx *ctx, const Ctl &h) {
+    *ctx->ctl_host = h;
+    TSE_HIP_TRY(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return 0;
+}
+
+static int sor_solve(tse_ctx *ctx, const SolveHooks &hk,
+                     const tse_sor_params *prm, tse_result *out) {
+    const Geo &G = ctx->geo;
+    const size_t bytes = 2ull * G.S * sizeof(double);
+    if (prm->max_it < 1) return fail(TSE_ERR_ARG, "max_it must be >= 1");
+    if (!(prm->dt > 0.0)) return fail(TSE_ERR_ARG, "dt must be > 0");
+    TSE_HIP_TRY(hipMemcpyAsync(hk.Xn, hk.X, bytes, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    Ctl h{};
+    h.num_it = 0;
+    h.num_sub_it = 0;
+    h.status = RUNNING;
+    h.check_mode = hk.transport ? CHECK_EVERY : CHECK_P_SCHEDULE;
+    h.max_it = prm->max_it;
+    h.omega_zero = (hk.transport && prm->omega == 0.0) ? 1 : 0;
+    h.t = 0.0;
+    h.dt = prm->dt;
+    h.diff = 1e100;
+    h.conv = prm->conv;
+    h.inc_it = prm->inc_it;
+    h.max_dt = prm->max_dt;
+    h.min_dt = prm->min_dt;
+    int rc = write_ctl(ctx, h);
+    if (rc) return rc;
+    rc = hk.assemble(ctx, hk.fields, h.dt, prm->omega);
+    if (rc) return rc;
+    const int nb = sweep_blocks(G.S);
+    int halvings = 0;
+    int chunk = 4;
+    for (;;) {
+        for (int n = 0; n < chunk; ++n) {
+            for (int colour = 0; colour < 2; ++colour) {
+                rc = hk.transport
+                         ? launch_tr_sweep(ctx, hk.X, colour, prm->sor, prm->omega)
+                         : launch_p_sweep(ctx, hk.X, colour, prm->sor, true);
+                if (rc) return rc;
+            }
+            rc = launch_control(ctx, nb);
+            if (rc) return rc;
+        }
+        TSE_HIP_TRY(hipMemcpyAsync(ctx->ctl_host, ctx->ctl, sizeof(Ctl),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+        TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        h = *ctx->ctl_host;
+        if (h.status == RUNNING) {
+            chunk = std::min(chunk * 2, 256);
+            continue;
+        }
+        if (h.status == NEED_HALVE) {
+            // tse.py:733-740: dt /= 2, num_sub_it = 0, X = Xn, stop if dt <= min_dt
+            h.dt = h.dt / 2;
+            h.num_sub_it = 0;
+            halvings += 1;
+            TSE_HIP_TRY(hipMemcpyAsync(hk.X, hk.Xn, bytes,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+            if (h.dt <= h.min_dt) {
+                h.status = FAILED;
+                break;
+            }
+            h.status = RUNNING;
+            rc = write_ctl(ctx, h);
+            if (rc) return rc;
+            rc = hk.assemble(ctx, hk.fields, h.dt, prm->omega);
+            if (rc) return rc;
+            chunk = 4;
+            continue;
+        }
+        break;  // CONVERGED or FAILED
+    }
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out->t = h.t;
+    out->dt = h.dt;
+    out->converged = h.status == CONVERGED ? 1 : 0;
+    out->num_it = h.num_it;
+    out->num_halvings = halvings;
+    out->status = h.status;
+    out->last_diff = h.diff;
+    return 0;
+}
+
+}  // namespace tse
+
+using namespace tse;
+
+extern "C" {
+
+int tse_abi_version(void) { return TSE_ABI_VERSION; }
+
+const char *tse_last_error(void) { return tse::g_err.c_str(); }
+
+int tse_ctx_create(tse_ctx **out, int device, void *stream) {
+    if (!out) return fail(TSE_ERR_ARG, "null out pointer");
+    *out = nullptr;
+    int ndev = 0;
+    TSE_HIP_TRY(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev)
+        return fail(TSE_ERR_ARG, "device index out of range");
+    TSE_HIP_TRY(hipSetDevice(device));
+    tse_ctx *ctx = new tse_ctx();
+    ctx->device = device;
+    ctx->stream = (hipStream_t)stream;
+    if (hipMalloc((void **)&ctx->ctl, sizeof(Ctl)) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->ctl_host, sizeof(Ctl)) != hipSuccess ||
+        hipMalloc((void **)&ctx->partials,
+                  sizeof(double) * 2 * 2 * kMaxSweepBlocks) != hipSuccess ||
+        hipMalloc((void **)&ctx->red_out, 2 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->red_host, 2 * sizeof(double)) !=
+            hipSuccess) {
+        tse_ctx_destroy(ctx);
+        return fail(TSE_ERR_NOMEM, "context allocation failed");
+    }
+    *out = ctx;
+    return 0;
+}
+
+int tse_ctx_destroy(tse_ctx *ctx) {
+    if (!ctx) return 0;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->d_dxyz) (void)hipFree(ctx->d_dxyz);
+    if (ctx->rec) (void)hipFree(ctx->rec);
+    if (ctx->mask) (void)hipFree(ctx->mask);
+    if (ctx->partials) (void)hipFree(ctx->partials);
+    if (ctx->ctl) (void)hipFree(ctx->ctl);
+    if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
+    if (ctx->red_out) (void)hipFree(ctx->red_out);
+    if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+    delete ctx;
+    return 0;
+}
+
+int tse_ctx_set_stream(tse_ctx *ctx, void *stream) {
+    if (!ctx) return fail(TSE_ERR_ARG, "null context");
+    ctx->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int tse_ctx_synchronize(tse_ctx *ctx) {
+    if (!ctx) return fail(TSE_ERR_ARG, "null context");
+    TSE_HIP_TRY(hipSetDevice(ctx->device));
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int tse_set_grid(tse_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
+                 const double *dx, const double *dy, const double *dz,
+                 double g, double sin_beta, double cos_beta) {
+    if (!ctx) return fail(TSE_ERR_ARG, "null context");
+    if (nx < 1 || ny < 1 || nz < 1 || !dx || !dy || !dz)
+        return fail(TSE_ERR_ARG, "invalid grid");
+    const long nxh = (nx + 1) / 2;
+    const long S = nxh * (long)ny * (long)nz;
+    if (2 * S >= (1L << 31))
+        return fail(TSE_ERR_ARG, "grid too large for 32-bit slot indices");
+    TSE_HIP_TRY(hipSetDevice(ctx->device));
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->d_dxyz) {
+        TSE_HIP_TRY(hipFree(ctx->d_dxyz));
+        ctx->d_dxyz = nullptr;
+    }
+    const size_t n = (size_t)nx + ny + nz;
+    if (hipMalloc((void **)&ctx->d_dxyz, n * sizeof(double)) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "hipMalloc of spacing arrays failed");
+    TSE_HIP_TRY(hipMemcpy(ctx->d_dxyz, dx, nx * sizeof(double),
+                          hipMemcpyHostToDevice));
+    TSE_HIP_TRY(hipMemcpy(ctx->d_dxyz + nx, dy, ny * sizeof(double),
+                          hipMemcpyHostToDevice));
+    TSE_HIP_TRY(hipMemcpy(ctx->d_dxyz + nx + ny, dz, nz * sizeof(double),
+                          hipMemcpyHostToDevice));
+    Geo &G = ctx->geo;
+    G.nx = nx;
+    G.ny = ny;
+    G.nz = nz;
+    G.nxh = (int)nxh;
+    G.S = (int)S;
+    G.div_nxh = make_fastdiv((uint32_t)nxh);
+    G.div_ny = make_fastdiv((uint32_t)ny);
+    G.dx = ctx->d_dxyz;
+    G.dy = ctx->d_dxyz + nx;
+    G.dz = ctx->d_dxyz + nx + ny;
+    G.g = g;
+    G.sinb = sin_beta;
+    G.cosb = cos_beta;
+    ctx->grid_set = true;
+    return 0;
+}
+
+int64_t tse_field_elems(const tse_ctx *ctx) {
+    if (!ctx || !ctx->grid_set) return -1;
+    return 2LL * ctx->geo.S;
+}
+
+int tse_to_split(tse_ctx *ctx, double *dst_split, const double *src,
+                 int64_t si, int64_t sj, int64_t sk) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dst_split || !src) return fail(TSE_ERR_ARG, "null pointer");
+    const long F = 2L * ctx->geo.S;
+    hipLaunchKernelGGL(to_split_kernel, dim3(conv_grid(F)), dim3(kBlock), 0,
+                       ctx->stream, ctx->geo, dst_split, src, (long)si,
+                       (long)sj, (long)sk);
+    TSE_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int tse_from_split(tse_ctx *ctx, double *dst, int64_t si, int64_t sj,
+                   int64_t sk, const double *src_split) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!dst || !src_split) return fail(TSE_ERR_ARG, "null pointer");
+    const long F = 2L * ctx->geo.S;
+    hipLaunchKernelGGL(from_split_kernel, dim3(conv_grid(F)), dim3(kBlock), 0,
+                       ctx->stream, ctx->geo, dst, (long)si, (long)sj,
+                       (long)sk, src_split);
+    TSE_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int tse_compute_p(tse_ctx *ctx, const tse_p_fields *f,
+                  const tse_sor_params *prm, tse_result *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !prm || !out) return fail(TSE_ERR_ARG, "null argument");
+    if (!f->P || !f->Pn || !f->c_f || !f->nue || !f->rho_f || !f->phi ||
+        !f->Kx || !f->Ky || !f->Kz || !f->BP || !f->WP || !f->c_r)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    if (prm->form != TSE_FORM_FAITHFUL64)
+        return fail(TSE_ERR_ARG, "tse_compute_p: unsupported form");
+    SolveHooks hk{p_assemble_hook, f, f->P, f->Pn, false};
+    return sor_solve(ctx, hk, prm, out);
+}
+
+int tse_compute_c(tse_ctx *ctx, const tse_c_fields *f,
+                  const tse_sor_params *prm, tse_result *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !prm || !out) return fail(TSE_ERR_ARG, "null argument");
+    if (!f->C || !f->Cn || !f->D_fx || !f->D_fy || !f->D_fz || !f->phi ||
+        !f->rho_f || !f->qx || !f->qy || !f->qz || !f->BC || !f->WC ||
+        !f->RC || !f->e || !f->lf_l || !f->lf_t)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    SolveHooks hk{c_assemble_hook, f, f->C, f->Cn, true};
+    return sor_solve(ctx, hk, prm, out);
+}
+
+int tse_compute_t(tse_ctx *ctx, const tse_t_fields *f,
+                  const tse_sor_params *prm, tse_result *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !prm || !out) return fail(TSE_ERR_ARG, "null argument");
+    if (!f->T || !f->Tn || !f->l_f || !f->cp_f || !f->rho_f || !f->phi ||
+        !f->qx || !f->qy || !f->qz || !f->BT || !f->WT || !f->e || !f->l_r ||
+        !f->cp_r || !f->rho_r || !f->lt_l || !f->lt_t)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    SolveHooks hk{t_assemble_hook, f, f->T, f->Tn, true};
+    return sor_solve(ctx, hk, prm, out);
+}
+
+int tse_compute_q(tse_ctx *ctx, const double *P, const double *BP,
+                  const double *Kx, const double *Ky, const double *Kz,
+                  double *qx, double *qy, double *qz, const double *nue,
+                  const double *rho_f) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!P || !BP || !Kx || !Ky || !Kz || !qx || !qy || !qz || !nue || !rho_f)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    return launch_q(ctx, P, BP, Kx, Ky, Kz, qx, qy, qz, nue, rho_f);
+}
+
+int tse_flow_error(tse_ctx *ctx, const double *P, const double *Pn,
+                   const double *qx, const double *qy, const double *qz,
+                   const double *WP, const double *phi, const double *c_f,
+                   const double *c_r, const double *rho_f, double dt_P,
+                   double *e) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!P || !Pn || !qx || !qy || !qz || !WP || !phi || !c_f || !c_r ||
+        !rho_f || !e)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    return launch_flow_error(ctx, P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f,
+                             dt_P, e);
+}
+
+int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
+                double *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!New || !Old || !out) return fail(TSE_ERR_ARG, "null pointer");
+    rc = launch_l2(ctx, New, Old);
+    if (rc) return rc;
+    TSE_HIP_TRY(hipMemcpyAsync(ctx->red_host, ctx->red_out, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *out = ctx->red_host[0];
+    return 0;
+}
+
+double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
+                            int32_t *is_nan) {
+    double s1 = 0.0, s2 = 0.0;
+    for (int64_t p = 0; p < n; ++p) {
+        const double d = New[p] - Old[p];
+        s1 += d * d;
+    }
+    for (int64_t p = 0; p < n; ++p) s2 += Old[p] * Old[p];
+    const double l2 = std::sqrt(s1) / std::sqrt(s2 + 1e-200);
+    if (std::isnan(l2)) {
+        if (is_nan) *is_nan = 1;
+        return 1.0;
+    }
+    if (is_nan) *is_nan = 0;
+    return l2;
+}
+
+int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
+                 int32_t n_iter) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || n_iter < 0) return fail(TSE_ERR_ARG, "invalid argument");
+    rc = launch_p_assemble(ctx, f, dt);
+    if (rc) return rc;
+    for (int n = 0; n < n_iter; ++n)
+        for (int colour = 0; colour < 2; ++colour) {
+            rc = launch_p_sweep(ctx, f->P, colour, sor, false);
+            if (rc) return rc;
+        }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+"""Device context: one HIP context, one grid, torch tensors as field buffers.
+
+Fields live on the GPU in the checkerboard-split layout of
+include/tse_hip.h (2*S float64 per field). Host numpy arrays in the reference
+layout (C order (nx, ny, nz), or strided species views such as
+``C[:, :, :, n]``) are copied into a flat staging tensor and permuted by the
+``tse_to_split`` kernel; results go back through ``tse_from_split``.
+PyTorch supplies device memory and the stream only.
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _lib
+
+
+def _torch():
+    import torch
+    return torch
+
+
+class HipContext:
+    """Owns a ``tse_ctx`` bound to one grid on one device."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, device: int = 0):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError("transportse_amd needs a ROCm GPU (torch.cuda is"
+                               " not available); there is no CPU fallback")
+        self.lib = _lib.load()
+        self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
+        self.shape = (self.nx, self.ny, self.nz)
+        self.dx = np.ascontiguousarray(dx, dtype=np.float64)
+        self.dy = np.ascontiguousarray(dy, dtype=np.float64)
+        self.dz = np.ascontiguousarray(dz, dtype=np.float64)
+        if self.dx.shape != (self.nx,) or self.dy.shape != (self.ny,) \
+                or self.dz.shape != (self.nz,):
+            raise ValueError("dx/dy/dz must have lengths nx/ny/nz")
+        self.device_index = int(device)
+        self.device = torch.device("cuda", self.device_index)
+        beta = math.radians(float(beta_deg))
+        self.g = float(g)
+        self.sin_beta, self.cos_beta = math.sin(beta), math.cos(beta)
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.device):
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+        _lib.check(self.lib.tse_ctx_create(ctypes.byref(handle), self.device_index,
+                                           ctypes.c_void_p(stream)), "tse_ctx_create")
+        self.ctx = handle
+        dp = ctypes.POINTER(ctypes.c_double)
+        _lib.check(self.lib.tse_set_grid(
+            self.ctx, self.nx, self.ny, self.nz,
+            self.dx.ctypes.data_as(dp), self.dy.ctypes.data_as(dp),
+            self.dz.ctypes.data_as(dp), self.g, self.sin_beta, self.cos_beta),
+            "tse_set_grid")
+        self.F = int(self.lib.tse_field_elems(self.ctx))
+        self.nxh = (self.nx + 1) // 2
+        self._staging = None
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.tse_ctx_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # buffers -----------------------------------------------------------------
+    def empty(self):
+        torch = _torch()
+        return torch.empty(self.F, dtype=torch.float64, device=self.device)
+
+    def zeros(self):
+        torch = _torch()
+        return torch.zeros(self.F, dtype=torch.float64, device=self.device)
+
+    def sync_stream(self):
+        torch = _torch()
+        stream = torch.cuda.current_stream(self.device).cuda_stream
+        self.lib.tse_ctx_set_stream(self.ctx, ctypes.c_void_p(stream))
+
+    # host <-> device -------------------------------------------------------------
+    def upload(self, arr: np.ndarray, out=None):
+        """Copy a host (nx,ny,nz) float64 array (any strides) into a split field."""
+        torch = _torch()
+        a = np.asarray(arr)
+        if a.shape != self.shape:
+            raise ValueError(f"field shape {a.shape} != grid {self.shape}")
+        if a.dtype != np.float64:
+            a = a.astype(np.float64)
+        # keep the element strides of the host view when its memory is one
+        # contiguous block (e.g. a species slice of a C-order 4-D array)
+        base, strides = _block_and_strides(a)
+        src = torch.from_numpy(base).to(self.device)
+        if out is None:
+            out = self.empty()
+        self.sync_stream()
+        _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), src.data_ptr(),
+                                          *strides), "tse_to_split")
+        return out
+
+    def download(self, field, out: np.ndarray | None = None) -> np.ndarray:
+        """Copy a split field into a host (nx,ny,nz) array (new, or `out`)."""
+        torch = _torch()
+        n = self.nx * self.ny * self.nz
+        if self._staging is None or self._staging.numel() < n:
+            self._staging = torch.empty(n, dtype=torch.float64, device=self.device)
+        st = self._staging[:n]
+        self.sync_stream()
+        _lib.check(self.lib.tse_from_split(self.ctx, st.data_ptr(),
+                                            self.ny * self.nz, self.nz, 1,
+                                            field.data_ptr()), "tse_from_split")
+        host = st.cpu().numpy().reshape(self.shape)
+        if out is None:
+            return host
+        out[...] = host
+        return out
+
+    def synchronize(self):
+        _lib.check(self.lib.tse_ctx_synchronize(self.ctx), "tse_ctx_synchronize")
+
+
+def _block_and_strides(a: np.ndarray):
+    """Return (contiguous 1-D buffer, element strides (si, sj, sk)).
+
+    For a C-contiguous array this is the array itself. For a strided view
+    whose elements lie in one C-contiguous parent block (a species slice),
+    the smallest covering slice of that parent is used so the device gather
+    reads the strided layout directly; anything else is copied contiguous.
+    """
+    if a.flags["C_CONTIGUOUS"]:
+        return a.reshape(-1), (a.shape[1] * a.shape[2], a.shape[2], 1)
+    base = a.base
+    while base is not None and base.base is not None:
+        base = base.base
+    if (isinstance(base, np.ndarray) and base.flags["C_CONTIGUOUS"]
+            and base.dtype == np.float64 and all(s > 0 for s in a.strides)
+            and all(s % 8 == 0 for s in a.strides)):
+        start = (a.__array_interface__["data"][0]
+                 - base.__array_interface__["data"][0])
+        if start >= 0 and start % 8 == 0:
+            flat = base.reshape(-1)
+            off = start // 8
+            si, sj, sk = (s // 8 for s in a.strides)
+            last = off + (a.shape[0] - 1) * si + (a.shape[1] - 1) * sj \
+                + (a.shape[2] - 1) * sk
+            if last < flat.size:
+                return flat[off:last + 1], (si, sj, sk)
+    c = np.ascontiguousarray(a)
+    return c.reshape(-1), (a.shape[1] * a.shape[2], a.shape[2], 1)

@@ -1,0 +1,518 @@
+"""Time-stepping / SIA Picard driver around the hot-path solvers.
+
+A restatement of the TRANSPORTSE main loop (tse.py:1118-1494) that calls the
+solvers through the reference's own signatures, so that config files, the
+EOS callback, time-step control, chemistry callbacks, dynamic boundary
+conditions, mass balance and the 34-tuple pickle output run unchanged. The
+order of operations and the state handling follow the reference exactly:
+
+  P (tse.py:1214) -> q (1228) -> e (1236) -> CFL (1241-1249) -> Peclet
+  (1255-1260) -> dt_max (1268-1271) -> CFL clamp (1289-1292) -> species
+  (1296-1336) -> heat (1338-1381) -> chemistry (1383-1418) -> EOS (1422)
+  -> SIA test with rhof_prev starting at zeros (1179, 1425-1442)
+  -> dynamic_bc (1449-1458) -> mass_balance (1461-1463) -> t_sim += dt
+  -> next dt = min(dt_P, dt_T, dt_C, t - t_sim) (1486-1492)
+  -> plotresults dt clamp / dump (312-329).
+
+The solver backend is any object with compute_P, compute_q, flow_error,
+compute_C, compute_T and l2_norm; ``make_hip_backend`` builds the GPU one.
+Not covered yet (raise NotImplementedError): SMOLAR_C / SMOLAR_T
+(tse.py:1318-1336, 1368-1381) and NLTE two-phase heat (1343-1354).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+import types
+
+import numpy
+
+from .config_loader import load_config
+
+# config attributes the reference binds at start-up (tse.py:81-151)
+_STATE_ARRAYS = ("phi", "Kx", "Ky", "Kz", "c_r", "cp_r", "l_r", "rho_r",
+                 "lf_l", "lf_t", "lt_l", "lt_t", "L", "R", "D_fx", "D_fy",
+                 "D_fz", "c_f", "cp_f", "l_f", "rho_f", "nue", "P", "C", "T",
+                 "BP", "BC", "BT", "WP", "WT", "WC", "qx", "qy", "qz", "mass")
+
+_FP_NAMES = ("C", "T", "P", "nue", "rho_f", "c_f", "cp_f", "l_f", "cp_r",
+             "l_r", "phi", "Kx", "Ky", "Kz", "D_fx", "D_fy", "D_fz", "rho_r")
+
+
+class ConvergenceError(SystemExit):
+    """The reference calls sys.exit on solver non-convergence (tse.py:1220)."""
+
+
+class Simulation:
+    """One TRANSPORTSE run of a config module with an injected backend."""
+
+    def __init__(self, config, backend_factory, out_dir=None, verbose=True,
+                 record_fields=False, max_steps=None, eos_mode="auto"):
+        if isinstance(config, str):
+            self.cfile = os.path.splitext(os.path.basename(config))[0]
+            self.cdir = os.path.dirname(os.path.abspath(config))
+            config = load_config(config)
+        else:
+            self.cfile = getattr(config, "__name__", "config").split(".")[-1]
+            self.cdir = os.path.dirname(os.path.abspath(getattr(config, "__file__", ".")))
+        self.config = config
+        self.out_dir = out_dir if out_dir is not None else self.cdir
+        self.verbose = verbose
+        self.record_fields = record_fields
+        self.max_steps = max_steps
+        self.eos_mode = eos_mode
+        c = config
+        # tse.py:81-167 ------------------------------------------------------
+        st = types.SimpleNamespace()
+        st.lx, st.ly, st.lz = c.lx, c.ly, c.lz
+        st.nx, st.ny, st.nz = int(c.nx), int(c.ny), int(c.nz)
+        st.dx, st.dy, st.dz = c.dx, c.dy, c.dz
+        st.g = c.g
+        st.beta_deg = c.beta
+        st.t = c.t
+        st.dt_max = c.dt_max
+        for name in _STATE_ARRAYS:
+            setattr(st, name, getattr(c, name))
+        shp = (st.nx, st.ny, st.nz)
+        st.vx = numpy.zeros(shp)
+        st.vy = numpy.zeros(shp)
+        st.vz = numpy.zeros(shp)
+        st.PeC = numpy.zeros(shp + (3,))
+        st.PeT = numpy.zeros(shp + (3,))
+        st.e = numpy.zeros(shp)
+        st.ZEROS = numpy.zeros(shp)
+        st.ONES = numpy.ones(shp)
+        self.st = st
+        # tse.py:231-245
+        self.MAXIT_P = c.MAXIT_P if getattr(c, "MAXIT_P", 0) else 20000
+        self.MAXIT_C = c.MAXIT_C if getattr(c, "MAXIT_C", 0) else 100
+        self.MAXIT_T = c.MAXIT_T if getattr(c, "MAXIT_T", 0) else 100
+        if getattr(c, "SMOLAR_C", False) or getattr(c, "SMOLAR_T", False):
+            raise NotImplementedError("Smolarkiewicz correction pass")
+        if getattr(c, "NLTE_SOLV", False):
+            raise NotImplementedError("NLTE two-phase heat transport")
+        self._eos_vectorized = None
+        if c.FLUIDPROP:
+            self.update_fluid_properties()
+        # the backend snapshots frozen globals at first call from `st`
+        self.solvers = backend_factory(self)
+        self.trajectory = []
+
+    # ---- EOS (tse.py:178-229) -------------------------------------------------
+    def _eos_cell(self, i, j, k):
+        st = self.st
+        args = [getattr(st, n)[i, j, k] for n in _FP_NAMES]
+        return self.config.fluid_properties(*args)
+
+    def _eos_loop(self):
+        st = self.st
+        arrs = [getattr(st, n) for n in _FP_NAMES]
+        fp = self.config.fluid_properties
+        for k in range(st.nz):
+            for j in range(st.ny):
+                for i in range(st.nx):
+                    out = fp(*[a[i, j, k] for a in arrs])
+                    for a, v in zip(arrs, out):
+                        a[i, j, k] = v
+
+    def _eos_vector(self):
+        """Call fluid_properties once on whole fields (species axis first)."""
+        st = self.st
+        args = []
+        for n in _FP_NAMES:
+            a = getattr(st, n)
+            args.append(numpy.moveaxis(a, -1, 0) if a.ndim == 4 else a)
+        out = self.config.fluid_properties(*args)
+        res = []
+        for n, v in zip(_FP_NAMES, out):
+            a = getattr(st, n)
+            v = numpy.asarray(v, dtype=numpy.float64)
+            if a.ndim == 4 and v.ndim == 4:
+                v = numpy.moveaxis(v, 0, -1)  # back to species-last
+            res.append(numpy.broadcast_to(v, a.shape))
+        return res
+
+    def update_fluid_properties(self):
+        st = self.st
+        if st.T.ndim != 3:
+            raise NotImplementedError("4-D (NLTE) temperature arrays")
+        if self.eos_mode == "loop" or self._eos_vectorized is False:
+            self._eos_loop()
+            return
+        try:
+            res = self._eos_vector()
+            ok = self._eos_vectorized or self._validate_eos(res)
+        except Exception:
+            ok = False
+        if not ok:
+            self._eos_vectorized = False
+            self._eos_loop()
+            return
+        self._eos_vectorized = True
+        for n, v in zip(_FP_NAMES, res):
+            getattr(st, n)[...] = v
+
+    def _validate_eos(self, res) -> bool:
+        """Check the whole-field call against the per-cell contract on a sample
+        of cells (bit-for-bit); any difference selects the per-cell loop."""
+        st = self.st
+        rng = numpy.random.default_rng(0)
+        cells = {(0, 0, 0), (st.nx - 1, st.ny - 1, st.nz - 1)}
+        for _ in range(min(64, st.nx * st.ny * st.nz)):
+            cells.add((int(rng.integers(st.nx)), int(rng.integers(st.ny)),
+                       int(rng.integers(st.nz))))
+        for (i, j, k) in cells:
+            ref = self._eos_cell(i, j, k)
+            for n, v, r in zip(_FP_NAMES, res, ref):
+                got = v[i, j, k]
+                if not numpy.array_equal(numpy.asarray(got, dtype=numpy.float64),
+                                         numpy.asarray(r, dtype=numpy.float64)):
+                    return False
+        return True
+
+    # ---- diagnostics (tse.py:331-396) ---------------------------------------------
+    def peclet_C(self):
+        st = self.st
+        act = st.BC == 0
+        out = []
+        for a, D, d in ((0, st.D_fx, st.dx[:, None, None]),
+                        (1, st.D_fy, st.dy[None, :, None]),
+                        (2, st.D_fz, st.dz[None, None, :])):
+            q = (st.qx, st.qy, st.qz)[a]
+            mD = numpy.max(D, axis=-1)
+            with numpy.errstate(divide="ignore", invalid="ignore"):
+                val = numpy.where(mD != 0, numpy.abs(q) / mD * d, numpy.nan)
+            st.PeC[..., a] = numpy.where(act, val, st.PeC[..., a])
+            out.append(numpy.max(st.PeC[..., a]))
+        return tuple(out)
+
+    def peclet_T(self):
+        st = self.st
+        act = st.BT == 0
+        out = []
+        den = (1 - st.phi) * st.l_r + st.phi * st.l_f
+        for a, d in ((0, st.dx[:, None, None]), (1, st.dy[None, :, None]),
+                     (2, st.dz[None, None, :])):
+            q = (st.qx, st.qy, st.qz)[a]
+            with numpy.errstate(divide="ignore", invalid="ignore"):
+                val = numpy.abs(q) * st.rho_f * st.cp_f / den * d
+            st.PeT[..., a] = numpy.where(act, val, st.PeT[..., a])
+            out.append(numpy.max(st.PeT[..., a]))
+        return tuple(out)
+
+    def explicit_C(self):
+        st = self.st
+        inv = (1 / st.dx[:, None, None] ** 2 + 1 / st.dy[None, :, None] ** 2) \
+            + 1 / st.dz[None, None, :] ** 2
+        return float(numpy.min(0.45 * numpy.max(st.D_fx, axis=-1) / st.phi / inv))
+
+    # ---- output (tse.py:312-329) ----------------------------------------------------
+    def plotresults(self):
+        c, st = self.config, self.st
+        if not c.PLOTSAVE:
+            return
+        if self.PLOTINDEX <= numpy.shape(c.PLOTTIME)[0] - 1:
+            t_plot = c.PLOTTIME[self.PLOTINDEX]
+            if self.t_sim < t_plot:
+                self.dt_prev = self.dt
+                self.dt = min(self.dt, t_plot - self.t_sim)
+            elif self.t_sim == t_plot:
+                from joblib import dump
+                path = os.path.join(self.out_dir, "{:s}-{:05d}.p".format(
+                    self.cfile, self.PLOTINDEX))
+                with open(path, "wb") as pfile:
+                    dump((st.lx, st.ly, st.lz, st.nx, st.ny, st.nz, st.dx,
+                          st.dy, st.dz, self.t_sim, self.dt, st.P, st.C, st.T,
+                          st.qx, st.qy, st.qz, st.phi, st.Kx, st.Ky, st.Kz,
+                          st.rho_f, st.nue, st.cp_f, st.l_f, st.BP, st.BC,
+                          st.BT, st.L, st.mass, st.WP, st.WT, st.WC, st.e),
+                         pfile, compress=3)
+                self.dt = self.dt_prev
+                self.PLOTINDEX += 1
+
+    def _host_mutated(self):
+        inv = getattr(self.solvers, "invalidate", None)
+        if inv is not None:
+            inv()
+
+    def _print(self, *a, **k):
+        if self.verbose:
+            print(*a, **k)
+
+    # ---- main loop (tse.py:1118-1494) --------------------------------------------------
+    def run(self):
+        c, st, sv = self.config, self.st, self.solvers
+        self.dt = c.dt
+        self.dt_prev = self.dt
+        self.t_sim = getattr(c, "t_sim", 0)
+        self.PLOTINDEX = 0
+        n_tsteps = 0
+        numitP = numitC = numitT = num_CFL = 0
+        max_DT = 0
+        Pex_T = Pey_T = Pez_T = Pex_C = Pey_C = Pez_C = 0
+        bc_inc = chem_inc = dt_chem = 0
+        dt_max_abs = st.dt_max
+        dt_P = dt_C = dt_T = None
+        Pn = Tn = Cn = None
+        time_start = time.time()
+        self.plotresults()
+        rhof_prev = numpy.zeros_like(st.rho_f)
+        solve_time = 0.0
+
+        while self.t_sim < st.t:
+            if self.max_steps is not None and n_tsteps >= self.max_steps:
+                break
+            SIA_filled = False
+            SIA_iter = 0
+            SIA_diff = 0
+            while not SIA_filled and SIA_iter < c.SIA_maxit:
+                Co_Filled = False
+                CFL = 0
+                if c.OMEGA_C == 0:
+                    dt_NeC = self.explicit_C()
+                num_it_P = 0
+                if c.FLUIDFLOW:
+                    while not Co_Filled:
+                        t0 = time.perf_counter()
+                        self.dt, dt_P, st.P, Pn, converged_P, num_it_P = \
+                            sv.compute_P(st.P, st.c_f, st.nue, st.rho_f,
+                                         st.phi, st.Kx, st.Ky, st.Kz, st.qx,
+                                         st.qy, st.qz, self.dt, st.BP, st.WP,
+                                         max_dt=st.dt_max, inc_it=1.25,
+                                         SOR=c.SOR_P, max_it=self.MAXIT_P,
+                                         P_conv=c.CONV_P)
+                        if not converged_P:
+                            raise ConvergenceError(
+                                '* P CONVERGENCE ERROR - check dt and CONV_P in compute_P')
+                        st.qx, st.qy, st.qz = sv.compute_q(
+                            st.P, st.BP, st.Kx, st.Ky, st.Kz, st.qx, st.qy,
+                            st.qz, st.nue, st.rho_f)
+                        if c.HEATTRANS or c.SPECTRANS:
+                            st.e = sv.flow_error(st.P, Pn, st.qx, st.qy, st.qz,
+                                                 st.WP, st.phi, st.c_f, st.c_r,
+                                                 st.rho_f, self.dt)
+                        solve_time += time.perf_counter() - t0
+                        if self.t_sim > 0:
+                            dt_CFL = 0.9 / (
+                                numpy.max(numpy.reshape(numpy.abs(st.qx) / st.phi,
+                                                        (st.ny, st.nz, st.nx)) / st.dx)
+                                + numpy.max(numpy.reshape(numpy.abs(st.qy) / st.phi,
+                                                          (st.nx, st.nz, st.ny)) / st.dy)
+                                + numpy.max(numpy.abs(st.qz) / st.phi / st.dz))
+                        else:
+                            dt_CFL = 1e10
+                        dt_max_C = dt_max_T = dt_max_P = dt_max_abs
+                        if c.HEATTRANS:
+                            Pex_T, Pey_T, Pez_T = self.peclet_T()
+                        if c.SPECTRANS:
+                            Pex_C, Pey_C, Pez_C = self.peclet_C()
+                        if c.OMEGA_C == 0:
+                            dt_max_C = dt_NeC
+                        if c.SIA:
+                            st.dt_max = min(dt_max_C, dt_max_T, dt_max_P)
+                        else:
+                            st.dt_max = min(dt_CFL, dt_max_C, dt_max_T, dt_max_P)
+                        if n_tsteps % c.PRINTOUT == 0 and self.verbose:
+                            if c.OMEGA_C + c.OMEGA_T < 2:
+                                print(' [dt_NeC] %8.2e [dt_NeT] %8.2e' % (dt_max_C, dt_max_T))
+                            print(' [CFL] %6.2f%% | Pe_T %.1e/%.1e/%.1e | Pe_C %.1e/%.1e/%.1e'
+                                  % (self.dt / dt_CFL * 100, Pex_T, Pey_T, Pez_T,
+                                     Pex_C, Pey_C, Pez_C))
+                            print('[P+] %6.2e [P-] %6.2e [T+] %6.4f [T-] %6.4f'
+                                  % (numpy.max(st.P), numpy.min(st.P),
+                                     numpy.max(st.T), numpy.min(st.T))
+                                  + ' [C+] %6.4f [C-] %6.4f' % (numpy.max(st.C),
+                                                                numpy.min(st.C)),
+                                  end='')
+                        Co_Filled = True
+                        if self.dt > st.dt_max * 1.0:
+                            num_CFL += num_it_P
+                            CFL += 1
+                            self.dt = st.dt_max
+
+                num_it_C = 0
+                if c.SPECTRANS:
+                    Cn = st.C.copy()
+                    for n in c.SPTRANS:
+                        if numpy.sum(st.C[:, :, :, n]) != 0.0 \
+                                or numpy.sum(st.WC[:, :, :, n]) != 0.0:
+                            t0 = time.perf_counter()
+                            self.dt, dt_C, st.C[:, :, :, n], converged_C, num_itC = \
+                                sv.compute_C(st.C[:, :, :, n], st.D_fx[:, :, :, n],
+                                             st.D_fy[:, :, :, n], st.D_fz[:, :, :, n],
+                                             st.phi, st.rho_f, st.qx, st.qy, st.qz,
+                                             self.dt, st.BC, st.WC[:, :, :, n], st.R,
+                                             st.e, omega=c.OMEGA_C, SOR=c.SOR_C,
+                                             max_it=self.MAXIT_C, C_conv=c.CONV_C)
+                            solve_time += time.perf_counter() - t0
+                            num_it_C += num_itC
+                            if not converged_C:
+                                raise ConvergenceError(
+                                    '* C CONVERGENCE ERROR- check dt and C_conv in compute_C')
+
+                num_it_T = 0
+                if c.HEATTRANS:
+                    Tn = st.T.copy()
+                    t0 = time.perf_counter()
+                    self.dt, dt_T, st.T, converged_T, num_itT = \
+                        sv.compute_T(st.T, st.l_f, st.cp_f, st.rho_f, st.phi,
+                                     st.qx, st.qy, st.qz, self.dt, st.BT, st.WT,
+                                     st.e, omega=c.OMEGA_T, max_it=self.MAXIT_T,
+                                     SOR=c.SOR_T, T_conv=c.CONV_T)
+                    solve_time += time.perf_counter() - t0
+                    num_it_T += num_itT
+                    if not converged_T:
+                        raise ConvergenceError(
+                            '* T CONVERGENCE ERROR - check dt and T_conv in compute_T')
+
+                if c.CHEMREACT:
+                    if self.t_sim > 0:
+                        chem_inc += self.dt
+                    else:
+                        chem_inc = 0
+                        dt_chem = self.dt
+                    if chem_inc >= c.DTCHEM:
+                        dt_chem = chem_inc
+                        chem_inc = 0
+                    chem_list = [(i, j, k) for k in range(st.nz)
+                                 for j in range(st.ny) for i in range(st.nx)
+                                 if st.BC[i, j, k] != 1]
+                    if chem_list != [] and chem_inc == 0:
+                        self._chemistry(chem_list, dt_chem)
+                        self._host_mutated()
+
+                if c.FLUIDPROP:
+                    self.update_fluid_properties()
+                    self._host_mutated()
+
+                if c.SIA:
+                    SIA_diff = sv.l2_norm(st.rho_f, rhof_prev, 'rhof')
+                    if n_tsteps % c.PRINTOUT == 0:
+                        self._print(' [SIA] %4d [SIA] %6.2e' % (SIA_iter, SIA_diff))
+                    if SIA_diff < c.SIA_conv or SIA_iter == c.SIA_maxit - 1:
+                        SIA_filled = True
+                    else:
+                        rhof_prev = st.rho_f.copy()
+                        st.P = Pn.copy()
+                        if c.HEATTRANS:
+                            st.T = Tn.copy()
+                        if c.SPECTRANS:
+                            st.C = Cn.copy()
+                        SIA_iter += 1
+                        self._host_mutated()
+                else:
+                    SIA_filled = True
+                    if n_tsteps % c.PRINTOUT == 0:
+                        self._print('')
+                numitP += num_it_P
+                numitC += num_it_C
+                numitT += num_it_T
+
+            if c.DYNAMICBC:
+                bc_inc_ = bc_inc
+                bc_inc, st.P, st.T, st.C, st.BP, st.BT, st.BC, st.WP, st.WT, \
+                    st.WC, st.Kx, st.Ky, st.Kz = c.dynamic_bc(
+                        self.t_sim + self.dt, bc_inc, st.P, st.T, st.C, st.BP,
+                        st.BT, st.BC, st.WP, st.WT, st.WC, st.Kx, st.Ky, st.Kz,
+                        st.rho_f, st.nue, st.c_f, st.cp_f, st.qx, st.phi,
+                        st.l_f, st.l_r)
+                self._host_mutated()
+                if bc_inc_ != bc_inc and c.FLUIDPROP:
+                    self.update_fluid_properties()
+
+            if c.MASSBALAN:
+                st.mass = c.mass_balance(st.C, st.qx, st.qy, st.qz, st.WP,
+                                         st.WC, st.WT, st.D_fx, st.D_fy,
+                                         st.D_fz, self.dt, st.mass)
+
+            step_dt = self.dt
+            self.t_sim += self.dt
+            n_tsteps += 1
+            max_DT = max(max_DT, self.dt)
+            rec = dict(step=n_tsteps, t_sim=self.t_sim, dt=step_dt,
+                       numitP=numitP, numitC=numitC, numitT=numitT,
+                       dt_P=dt_P, dt_C=dt_C, dt_T=dt_T,
+                       mass=numpy.array(st.mass, dtype=float).copy())
+            if self.record_fields:
+                rec.update(P=st.P.copy(), C=st.C.copy(), T=st.T.copy(),
+                           qx=st.qx.copy(), qy=st.qy.copy(), qz=st.qz.copy(),
+                           e=st.e.copy(), rho_f=st.rho_f.copy())
+            self.trajectory.append(rec)
+
+            if n_tsteps % c.PRINTOUT == 0 and self.verbose:
+                print('[t] %8.2e/%8.2e [dt] %8.2e [S] %4d [P] %3d'
+                      % (self.t_sim, st.t, self.dt, n_tsteps, num_it_P)
+                      + ' [C] %3d [T] %3d [CFL] %d' % (num_it_C, num_it_T, CFL))
+
+            self.dt = 1e99
+            if c.FLUIDFLOW:
+                self.dt = min(dt_P, st.t - self.t_sim)
+            if c.HEATTRANS:
+                self.dt = min(dt_T, st.t - self.t_sim, self.dt)
+            if c.SPECTRANS:
+                if dt_C is None:
+                    raise NameError("name 'dt_C' is not defined")
+                self.dt = min(dt_C, st.t - self.t_sim, self.dt)
+            self.plotresults()
+
+        self.wall_time = time.time() - time_start
+        self.solve_time = solve_time
+        self.summary = dict(t_sim=self.t_sim, n_tsteps=n_tsteps, numitP=numitP,
+                            numitC=numitC, numitT=numitT, num_CFL=num_CFL,
+                            max_DT=max_DT, wall_time=self.wall_time)
+        self._print('[ SIMULATION SUMMARY ]')
+        self._print('[t/s] %8.2e/%8.2e [S] %5d [P] %5d [C] %5d [T] %5d [CFL] %1d'
+                    % (self.t_sim, st.t, n_tsteps, numitP, numitC, numitT, num_CFL))
+        self._print('[Run time]', round(self.wall_time, 2), '(s)')
+        return self.trajectory
+
+    def _chemistry(self, chem_list, dt_chem):
+        """tse.py:1384-1418: vectorised or per-cell chemistry callbacks."""
+        c, st = self.config, self.st
+        if hasattr(c, 'chemical_reactions_all'):
+            st.C, st.P, st.T, st.rho_f, st.phi, st.Kx, st.Ky, st.Kz = \
+                c.chemical_reactions_all(st.C, st.P, st.T, st.rho_f, st.phi,
+                                         st.Kx, st.Ky, st.Kz, dt_chem)
+            return
+        from joblib import Parallel, delayed
+        results = Parallel(n_jobs=c.parallel_chemistry)(
+            delayed(c.chemical_reactions)(
+                st.C[i, j, k], st.P[i, j, k], st.T[i, j, k], st.rho_f[i, j, k],
+                st.phi[i, j, k], st.Kx[i, j, k], st.Ky[i, j, k], st.Kz[i, j, k],
+                dt_chem, i, j, k) for i, j, k in chem_list)
+        for num, l in enumerate(chem_list):
+            r = results[num]
+            st.C[l], st.P[l], st.T[l], st.rho_f[l], st.phi[l], st.Kx[l], \
+                st.Ky[l], st.Kz[l] = r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]
+
+
+def make_hip_backend(sim: Simulation, p_form: str = "faithful64", device: int = 0):
+    """GPU backend for a Simulation (frozen globals read from its state)."""
+    from .solvers import HipSolvers
+    st, c = sim.st, sim.config
+    return HipSolvers(st.nx, st.ny, st.nz, st.dx, st.dy, st.dz, st.g,
+                      st.beta_deg, frozen_source=st, maxit_p=sim.MAXIT_P,
+                      maxit_c=sim.MAXIT_C, maxit_t=sim.MAXIT_T,
+                      dt_max=c.dt_max, c_imblflo=c.C_IMBLFLO,
+                      t_imblflo=c.T_IMBLFLO, device=device,
+                      auto_invalidate=False, verbose=sim.verbose, p_form=p_form)
+
+
+def main(argv=None):
+    """``python -m transportse_amd.driver config.py`` (tse.py's CLI)."""
+    argv = sys.argv if argv is None else argv
+    if len(argv) < 2:
+        print("usage: python -m transportse_amd.driver <config.py>")
+        return 2
+    sim = Simulation(argv[1], make_hip_backend)
+    try:
+        sim.run()
+    except ConvergenceError as e:
+        print(e)
+        return 1
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,257 @@
+"""Drop-in solvers with the reference's call signatures, run on the GPU.
+
+``HipSolvers`` exposes ``compute_P``, ``compute_q``, ``flow_error``,
+``compute_C``, ``compute_T`` and ``l2_norm`` with the argument lists, default
+values and return tuples of tse.py:
+
+* compute_P  (tse.py:647-651 -> 742): ``(t_P, dt_P, P, Pn, converged, num_it)``
+* compute_q  (tse.py:746 -> 780):     ``(qx, qy, qz)`` written in place
+* flow_error (tse.py:410 -> 446):     a fresh ``e``
+* compute_C  (tse.py:785-787 -> 916): ``(t_C, dt_C, C, converged, num_it)``
+* compute_T  (tse.py:921-923 -> 1081): ``(t_T, dt_T, T, converged, num_it)``
+
+Arrays are numpy float64 in the reference layout; species arguments may be
+strided views. Results are written into the passed arrays (P, C, T, q) and
+also returned, so a caller that rebinds (``P = compute_P(...)[2]``) and one
+that keeps its own reference both see the solution.
+
+Numba freezes module globals into a kernel at its first call (tse.py
+reads ``c_r`` in compute_P, ``lf_l``/``lf_t`` in compute_C and ``l_r``,
+``cp_r``, ``rho_r``, ``lt_l``, ``lt_t`` in compute_T as globals). The
+solvers reproduce that: they read those arrays from ``frozen_source`` at the
+first call of the function that uses them and keep the device copy.
+
+Device residency: inputs are cached on the device per argument role. A
+cached copy is reused while the same host array (same buffer, strides and
+shape) is passed and no host mutation was signalled; ``invalidate()``
+signals one. With ``auto_invalidate=True`` (the default, for stand-alone
+use) every call re-uploads its inputs; the SIA driver turns it off and
+invalidates after every host callback.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .device import HipContext
+
+_FORMS = {"faithful64": _lib.FORM_FAITHFUL64, "plain64": _lib.FORM_PLAIN64,
+          "mixed": _lib.FORM_MIXED}
+
+
+def _key(a: np.ndarray):
+    ai = a.__array_interface__
+    return (ai["data"][0], a.strides, a.shape, a.dtype.str)
+
+
+class HipSolvers:
+    """GPU implementation of the reference's solver calls for one grid."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, frozen_source=None,
+                 maxit_p=20000, maxit_c=100, maxit_t=100, dt_max=1e99,
+                 c_imblflo=False, t_imblflo=False, device=0,
+                 auto_invalidate=True, verbose=True, p_form="faithful64"):
+        self.hip = HipContext(nx, ny, nz, dx, dy, dz, g, beta_deg, device)
+        self.lib = self.hip.lib
+        self.shape = self.hip.shape
+        self.frozen_source = frozen_source
+        self.maxit_p, self.maxit_c, self.maxit_t = maxit_p, maxit_c, maxit_t
+        self.dt_max = dt_max
+        self.c_imblflo, self.t_imblflo = bool(c_imblflo), bool(t_imblflo)
+        self.auto_invalidate = auto_invalidate
+        self.verbose = verbose
+        self.p_form = p_form
+        self._gen = 0
+        self._cache = {}   # role -> [key, gen, tensor, host_ref]
+        self._frozen = {}  # name -> tensor
+        self.last = None
+        self.stats = {"uploads": 0, "downloads": 0}
+
+    # residency -------------------------------------------------------------------
+    def invalidate(self):
+        """Signal that host code may have changed any input array."""
+        self._gen += 1
+
+    def _dev(self, role: str, arr):
+        a = np.asarray(arr)
+        k = _key(a)
+        ent = self._cache.get(role)
+        if ent is not None and ent[0] == k and ent[1] == self._gen:
+            return ent[2]
+        out = ent[2] if ent is not None else None
+        t = self.hip.upload(a, out=out)
+        self.stats["uploads"] += 1
+        self._cache[role] = [k, self._gen, t, a]
+        return t
+
+    def _mark(self, role: str, arr, tensor):
+        """Record that `tensor` now equals host `arr` (after a download)."""
+        a = np.asarray(arr)
+        self._cache[role] = [_key(a), self._gen, tensor, a]
+
+    def _scratch(self, role: str):
+        ent = self._cache.get(role)
+        if ent is None:
+            t = self.hip.empty()
+            self._cache[role] = [None, -1, t, None]
+            return t
+        return ent[2]
+
+    def _frozen_field(self, name: str):
+        if name not in self._frozen:
+            src = getattr(self.frozen_source, name)
+            self._frozen[name] = self.hip.upload(np.asarray(src, dtype=np.float64))
+        return self._frozen[name]
+
+    def set_frozen(self, **arrays):
+        for k, v in arrays.items():
+            self._frozen[k] = self.hip.upload(np.asarray(v, dtype=np.float64))
+
+    def _begin(self):
+        if self.auto_invalidate:
+            self.invalidate()
+        self.hip.sync_stream()
+
+    # solvers -----------------------------------------------------------------------
+    def compute_P(self, P, c_f, nue, rho_f, phi, Kx, Ky, Kz, qx, qy, qz, dt_P,
+                  BP, WP, SOR=1.0, max_it=None, inc_it=1.05, max_dt=None,
+                  min_dt=1e-8, P_conv=1e-10):
+        self._begin()
+        max_it = self.maxit_p if max_it is None else max_it
+        max_dt = self.dt_max if max_dt is None else max_dt
+        d = {n: self._dev(n, a) for n, a in (
+            ("P", P), ("c_f", c_f), ("nue", nue), ("rho_f", rho_f),
+            ("phi", phi), ("Kx", Kx), ("Ky", Ky), ("Kz", Kz), ("BP", BP),
+            ("WP", WP))}
+        d["Pn"] = self._scratch("Pn")
+        d["c_r"] = self._frozen_field("c_r")
+        f = _lib.PFields(**{n: d[n].data_ptr() for n in _lib.P_FIELDS})
+        prm = _lib.SorParams(dt=float(dt_P), sor=float(SOR), omega=1.0,
+                             inc_it=float(inc_it), max_dt=float(max_dt),
+                             min_dt=float(min_dt), conv=float(P_conv),
+                             max_it=int(max_it), form=_FORMS[self.p_form])
+        res = _lib.Result()
+        _lib.check(self.lib.tse_compute_p(self.hip.ctx, ctypes.byref(f),
+                                          ctypes.byref(prm), ctypes.byref(res)),
+                   "tse_compute_p")
+        self._report_halvings("P", res, dt_P)
+        self.hip.download(d["P"], out=P)
+        Pn = self.hip.download(d["Pn"])
+        self.stats["downloads"] += 2
+        self._mark("P", P, d["P"])
+        self._mark("Pn", Pn, d["Pn"])
+        self.last = res
+        return res.t, res.dt, P, Pn, bool(res.converged), int(res.num_it)
+
+    def compute_q(self, P, BP, Kx, Ky, Kz, qx, qy, qz, nue, rho_f):
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (
+            ("P", P), ("BP", BP), ("Kx", Kx), ("Ky", Ky), ("Kz", Kz),
+            ("qx", qx), ("qy", qy), ("qz", qz), ("nue", nue),
+            ("rho_f", rho_f))}
+        _lib.check(self.lib.tse_compute_q(
+            self.hip.ctx, *[d[n].data_ptr() for n in
+                            ("P", "BP", "Kx", "Ky", "Kz", "qx", "qy", "qz",
+                             "nue", "rho_f")]), "tse_compute_q")
+        for n, a in (("qx", qx), ("qy", qy), ("qz", qz)):
+            self.hip.download(d[n], out=a)
+            self._mark(n, a, d[n])
+        self.stats["downloads"] += 3
+        return qx, qy, qz
+
+    def flow_error(self, P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f, dt_P):
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (
+            ("P", P), ("Pn", Pn), ("qx", qx), ("qy", qy), ("qz", qz),
+            ("WP", WP), ("phi", phi), ("c_f", c_f), ("c_r_live", c_r),
+            ("rho_f", rho_f))}
+        e_dev = self._scratch("e")
+        _lib.check(self.lib.tse_flow_error(
+            self.hip.ctx, *[d[n].data_ptr() for n in
+                            ("P", "Pn", "qx", "qy", "qz", "WP", "phi", "c_f",
+                             "c_r_live", "rho_f")], float(dt_P),
+            e_dev.data_ptr()), "tse_flow_error")
+        e = self.hip.download(e_dev)
+        self.stats["downloads"] += 1
+        self._mark("e", e, e_dev)
+        return e
+
+    def compute_C(self, C, D_fx, D_fy, D_fz, phi, rho_f, qx, qy, qz, dt_C, BC,
+                  WC, RC, e, omega=1, SOR=1.0, max_it=None, inc_it=1.05,
+                  max_dt=None, min_dt=1e-8, C_conv=1e-10):
+        self._begin()
+        max_it = self.maxit_c if max_it is None else max_it
+        max_dt = self.dt_max if max_dt is None else max_dt
+        d = {n: self._dev(n, a) for n, a in (
+            ("C", C), ("D_fx", D_fx), ("D_fy", D_fy), ("D_fz", D_fz),
+            ("phi", phi), ("rho_f", rho_f), ("qx", qx), ("qy", qy),
+            ("qz", qz), ("BC", BC), ("WC", WC), ("RC", RC), ("e", e))}
+        d["Cn"] = self._scratch("Cn")
+        d["lf_l"] = self._frozen_field("lf_l")
+        d["lf_t"] = self._frozen_field("lf_t")
+        f = _lib.CFields(**{n: d[n].data_ptr() for n in _lib.C_FIELDS},
+                         imblflo=int(self.c_imblflo))
+        prm = _lib.SorParams(dt=float(dt_C), sor=float(SOR), omega=float(omega),
+                             inc_it=float(inc_it), max_dt=float(max_dt),
+                             min_dt=float(min_dt), conv=float(C_conv),
+                             max_it=int(max_it), form=0)
+        res = _lib.Result()
+        _lib.check(self.lib.tse_compute_c(self.hip.ctx, ctypes.byref(f),
+                                          ctypes.byref(prm), ctypes.byref(res)),
+                   "tse_compute_c")
+        self._report_halvings("C", res, dt_C)
+        self.hip.download(d["C"], out=C)
+        self.stats["downloads"] += 1
+        self._mark("C", C, d["C"])
+        self.last = res
+        return res.t, res.dt, C, bool(res.converged), int(res.num_it)
+
+    def compute_T(self, T, l_f, cp_f, rho_f, phi, qx, qy, qz, dt_T, BT, WT, e,
+                  omega=1, SOR=1.0, max_it=None, inc_it=1.05, max_dt=None,
+                  min_dt=1e-8, T_conv=1e-10):
+        self._begin()
+        max_it = self.maxit_t if max_it is None else max_it
+        max_dt = self.dt_max if max_dt is None else max_dt
+        d = {n: self._dev(n, a) for n, a in (
+            ("T", T), ("l_f", l_f), ("cp_f", cp_f), ("rho_f", rho_f),
+            ("phi", phi), ("qx", qx), ("qy", qy), ("qz", qz), ("BT", BT),
+            ("WT", WT), ("e", e))}
+        d["Tn"] = self._scratch("Tn")
+        for n in ("l_r", "cp_r", "rho_r", "lt_l", "lt_t"):
+            d[n] = self._frozen_field(n)
+        f = _lib.TFields(**{n: d[n].data_ptr() for n in _lib.T_FIELDS},
+                         imblflo=int(self.t_imblflo))
+        prm = _lib.SorParams(dt=float(dt_T), sor=float(SOR), omega=float(omega),
+                             inc_it=float(inc_it), max_dt=float(max_dt),
+                             min_dt=float(min_dt), conv=float(T_conv),
+                             max_it=int(max_it), form=0)
+        res = _lib.Result()
+        _lib.check(self.lib.tse_compute_t(self.hip.ctx, ctypes.byref(f),
+                                          ctypes.byref(prm), ctypes.byref(res)),
+                   "tse_compute_t")
+        self._report_halvings("T", res, dt_T)
+        self.hip.download(d["T"], out=T)
+        self.stats["downloads"] += 1
+        self._mark("T", T, d["T"])
+        self.last = res
+        return res.t, res.dt, T, bool(res.converged), int(res.num_it)
+
+    def l2_norm(self, New, Old, Process=""):
+        """l2_norm (tse.py:273-281) on host arrays, Numba's sequential sums."""
+        a = np.ascontiguousarray(New, dtype=np.float64)
+        b = np.ascontiguousarray(Old, dtype=np.float64)
+        flag = ctypes.c_int32(0)
+        dp = ctypes.POINTER(ctypes.c_double)
+        v = self.lib.tse_host_l2_norm_seq(a.ctypes.data_as(dp), b.ctypes.data_as(dp),
+                                          a.size, ctypes.byref(flag))
+        if flag.value:
+            print("ISNAN:", Process)
+            return 1
+        return v
+
+    def _report_halvings(self, what, res, dt0):
+        if self.verbose and res.num_halvings:
+            print(f" [{what}] {res.num_halvings} dt halving(s): {dt0} -> {res.dt}"
+                  f" (s) after {res.num_it} iterations")
