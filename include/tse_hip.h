@@ -170,9 +170,12 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
 
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
- * fixed dt (no norm, no control), for timing and sweep-level parity. */
+ * fixed dt (no norm, no control), for timing and sweep-level parity.
+ * assemble != 0 first evaluates the coefficient records from f (as
+ * tse_compute_p does on entry); assemble == 0 reuses the records of the
+ * previous call on this context (same fields and dt). */
 int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
-                 int32_t n_iter);
+                 int32_t n_iter, int32_t assemble);
 
 #ifdef __cplusplus
 }

@@ -436,12 +436,17 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
 }
 
 int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
-                 int32_t n_iter) {
+                 int32_t n_iter, int32_t assemble) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if (!f || n_iter < 0) return fail(TSE_ERR_ARG, "invalid argument");
-    rc = launch_p_assemble(ctx, f, dt);
-    if (rc) return rc;
+    if (!f || !f->P || n_iter < 0) return fail(TSE_ERR_ARG, "invalid argument");
+    if (assemble) {
+        rc = launch_p_assemble(ctx, f, dt);
+        if (rc) return rc;
+    } else if (ctx->rec_elems < 13ull * 2ull * ctx->geo.S ||
+               ctx->mask_elems < 2ull * ctx->geo.S) {
+        return fail(TSE_ERR_STATE, "tse_p_sweeps: no records assembled");
+    }
     for (int n = 0; n < n_iter; ++n)
         for (int colour = 0; colour < 2; ++colour) {
             rc = launch_p_sweep(ctx, f->P, colour, sor, false);
