@@ -53,12 +53,25 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
-def lib():
+_libs = {}
+
+
+def lib(path=None):
+    """The oracle library (default: libtse_oracle.so next to this file)."""
     global _lib
+    if path is not None and path != _LIB_PATH:
+        if path not in _libs:
+            _libs[path] = _bind(ctypes.CDLL(path))
+        return _libs[path]
     if _lib is None:
         if not os.path.exists(_LIB_PATH):
             build()
-        L = ctypes.CDLL(_LIB_PATH)
+        _lib = _bind(ctypes.CDLL(_LIB_PATH))
+    return _lib
+
+
+def _bind(L):
+    if True:
         D, I, G, R = ctypes.c_double, ctypes.c_int, ctypes.POINTER(_Grid), ctypes.POINTER(_Result)
         L.tse_or_harm.restype = D
         L.tse_or_harm.argtypes = [G, _dp, I, I, I, I, I, I, I]
@@ -86,8 +99,7 @@ def lib():
         L.tse_or_compute_t.argtypes = [G] + [_dp] * 17 + [I, D, D, D, I, D, D, D, D, R]
         L.tse_or_set_threads.argtypes = [I]
         L.tse_or_get_threads.restype = I
-        _lib = L
-    return _lib
+    return L
 
 
 def _p(a: np.ndarray):
@@ -113,7 +125,9 @@ class Oracle:
 
     def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, frozen_source=None,
                  maxit_p=20000, maxit_c=100, maxit_t=100, dt_max=1e99,
-                 c_imblflo=False, t_imblflo=False, verbose=False):
+                 c_imblflo=False, t_imblflo=False, verbose=False,
+                 lib_path=None):
+        self._L = lib(lib_path)
         self.nx, self.ny, self.nz = int(nx), int(ny), int(nz)
         self.dx = _c(dx)
         self.dy = _c(dy)
@@ -151,28 +165,27 @@ class Oracle:
     # helpers -------------------------------------------------------------
     def harm(self, i1, j1, k1, i2, j2, k2, H, n):
         H = _c(H)
-        return lib().tse_or_harm(ctypes.byref(self.grid), _p(H), i1, j1, k1, i2, j2, k2, n)
+        return self._L.tse_or_harm(ctypes.byref(self.grid), _p(H), i1, j1, k1, i2, j2, k2, n)
 
     def amean(self, i1, j1, k1, i2, j2, k2, X, n):
         X = _c(X)
-        return lib().tse_or_amean(ctypes.byref(self.grid), _p(X), i1, j1, k1, i2, j2, k2, n)
+        return self._L.tse_or_amean(ctypes.byref(self.grid), _p(X), i1, j1, k1, i2, j2, k2, n)
 
     def alpha(self, i1, j1, k1, i2, j2, k2, nue, rho_f, K, d1, d2, n):
         nue, rho_f, K = _c(nue), _c(rho_f), _c(K)
-        return lib().tse_or_alpha(ctypes.byref(self.grid), i1, j1, k1, i2, j2, k2,
+        return self._L.tse_or_alpha(ctypes.byref(self.grid), i1, j1, k1, i2, j2, k2,
                                   _p(nue), _p(rho_f), _p(K), d1, d2, n)
 
     def beta_g(self, i1, j1, k1, i2, j2, k2, rho_f, d):
         rho_f = _c(rho_f)
-        return lib().tse_or_beta_g(ctypes.byref(self.grid), i1, j1, k1, i2, j2, k2,
+        return self._L.tse_or_beta_g(ctypes.byref(self.grid), i1, j1, k1, i2, j2, k2,
                                    _p(rho_f), d)
 
-    @staticmethod
-    def l2_norm(New, Old, Process=""):
+    def l2_norm(self, New, Old, Process=""):
         """l2_norm (tse.py:273-281); prints ISNAN and returns 1 on NaN."""
         New, Old = _c(New), _c(Old)
         flag = ctypes.c_int(0)
-        v = lib().tse_or_l2_norm(_p(New), _p(Old), New.size, ctypes.byref(flag))
+        v = self._L.tse_or_l2_norm(_p(New), _p(Old), New.size, ctypes.byref(flag))
         if flag.value:
             print("ISNAN:", Process)
             return 1
@@ -181,7 +194,7 @@ class Oracle:
     def dispersivity(self, qx, qy, qz, l_l, l_t, B, i, j, k):
         out = np.zeros(6)
         arrs = [_c(a) for a in (qx, qy, qz, l_l, l_t, B)]
-        lib().tse_or_dispersivity(ctypes.byref(self.grid), *[_p(a) for a in arrs],
+        self._L.tse_or_dispersivity(ctypes.byref(self.grid), *[_p(a) for a in arrs],
                                   i, j, k, _p(out))
         return tuple(float(v) for v in out)
 
@@ -196,7 +209,7 @@ class Oracle:
         Pn = np.empty_like(Pw)
         ins = [_c(a) for a in (c_f, nue, rho_f, phi, Kx, Ky, Kz, BP, WP)]
         res = _Result()
-        rc = lib().tse_or_compute_p(ctypes.byref(self.grid), _p(Pw), _p(Pn),
+        rc = self._L.tse_or_compute_p(ctypes.byref(self.grid), _p(Pw), _p(Pn),
                                     *[_p(a) for a in ins], _p(c_r), float(dt_P),
                                     float(SOR), int(max_it), float(inc_it),
                                     float(max_dt), float(min_dt), float(P_conv),
@@ -212,7 +225,7 @@ class Oracle:
         (c_r,) = self._freeze("c_r")
         Pw = _c(P)
         ins = [_c(a) for a in (Pn, c_f, nue, rho_f, phi, Kx, Ky, Kz, BP, WP)]
-        lib().tse_or_p_half_sweep(ctypes.byref(self.grid), _p(Pw),
+        self._L.tse_or_p_half_sweep(ctypes.byref(self.grid), _p(Pw),
                                   *[_p(a) for a in ins], _p(c_r), float(dt_P),
                                   float(SOR), int(colour))
         P[...] = Pw
@@ -222,7 +235,7 @@ class Oracle:
         outs = [_c(a) for a in (qx, qy, qz)]
         ins1 = [_c(a) for a in (P, BP, Kx, Ky, Kz)]
         ins2 = [_c(a) for a in (nue, rho_f)]
-        lib().tse_or_compute_q(ctypes.byref(self.grid), *[_p(a) for a in ins1],
+        self._L.tse_or_compute_q(ctypes.byref(self.grid), *[_p(a) for a in ins1],
                                *[_p(a) for a in outs], *[_p(a) for a in ins2])
         qx[...], qy[...], qz[...] = outs
         return qx, qy, qz
@@ -230,7 +243,7 @@ class Oracle:
     def flow_error(self, P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f, dt_P):
         ins = [_c(a) for a in (P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f)]
         e = np.empty(self.shape)
-        lib().tse_or_flow_error(ctypes.byref(self.grid), *[_p(a) for a in ins],
+        self._L.tse_or_flow_error(ctypes.byref(self.grid), *[_p(a) for a in ins],
                                 float(dt_P), _p(e))
         return e
 
@@ -244,7 +257,7 @@ class Oracle:
         Cn = np.empty_like(Cw)
         ins = [_c(a) for a in (D_fx, D_fy, D_fz, phi, rho_f, qx, qy, qz, BC, WC, RC, e)]
         res = _Result()
-        rc = lib().tse_or_compute_c(ctypes.byref(self.grid), _p(Cw), _p(Cn),
+        rc = self._L.tse_or_compute_c(ctypes.byref(self.grid), _p(Cw), _p(Cn),
                                     *[_p(a) for a in ins], _p(lf_l), _p(lf_t),
                                     int(self.c_imblflo), float(dt_C), float(omega),
                                     float(SOR), int(max_it), float(inc_it),
@@ -266,7 +279,7 @@ class Oracle:
         Tn = np.empty_like(Tw)
         ins = [_c(a) for a in (l_f, cp_f, rho_f, phi, qx, qy, qz, BT, WT, e)]
         res = _Result()
-        rc = lib().tse_or_compute_t(ctypes.byref(self.grid), _p(Tw), _p(Tn),
+        rc = self._L.tse_or_compute_t(ctypes.byref(self.grid), _p(Tw), _p(Tn),
                                     *[_p(a) for a in ins], _p(l_r), _p(cp_r),
                                     _p(rho_r), _p(lt_l), _p(lt_t),
                                     int(self.t_imblflo), float(dt_T), float(omega),

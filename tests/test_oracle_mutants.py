@@ -1,0 +1,99 @@
+"""Planted mistakes in the oracle must be caught (CPU only).
+
+Each case copies oracle/tse_oracle.c, plants one plausible mistake (a wrong
+sign, a dropped term, a wrong index or neighbour, a transposed operand, a
+changed schedule), compiles the copy with gcc into a temporary library and
+runs the golden-trajectory comparison with it. The comparison must report a
+difference for every mutant; otherwise the checks would not pin that part of
+the restatement.
+"""
+import os
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import CASES, GOLDEN, ROOT
+
+SRC = os.path.join(ROOT, "oracle", "tse_oracle.c")
+HDR = os.path.join(ROOT, "oracle", "tse_oracle.h")
+
+MUTANTS = [
+    ("P gravity product sign", "Anew + A_ * B_ + Cnew", "Anew - A_ * B_ + Cnew"),
+    ("P source WP dropped", "K_ * L_ + R * Pn[c] + WP[c])", "K_ * L_ + R * Pn[c])"),
+    ("alpha centre spacing", "hk / hn / (d[d2] * (d[d1] + d[d2])",
+     "hk / hn / (d[d1] * (d[d1] + d[d2])"),
+    ("C upwind side", "N_ = py_min0(qx[c]) / dd;", "N_ = py_max0(qx[c]) / dd;"),
+    ("dispersivity longitudinal index", "qxl = Q(qx, i - 1, j, k);",
+     "qxl = Q(qx, i, j, k);"),
+    ("P norm schedule", "else if (num_it % 10 == 0)", "else if (num_it % 5 == 0)"),
+    ("flow_error storage sign", "(P[c] - Pn[c]) * rho_f[c] / dt_P);",
+     "(Pn[c] - P[c]) * rho_f[c] / dt_P);"),
+    ("T imbalance term", "ET = cp_f[c] * Tn[c] * e[c];", "ET = cp_f[c] * e[c];"),
+    ("q gravity component", "g->g * g->sin_beta));", "g->g * g->cos_beta));"),
+    ("T conduction mixing transposed",
+     "*A = ((1 - por) * lr + por * lf + Ddisp * rhocpf) / dd2;",
+     "*A = ((1 - por) * lf + por * lr + Ddisp * rhocpf) / dd2;"),
+    ("P reset after halving missing",
+     "                halvings += 1;\n                memcpy(P, Pn, N * sizeof(double));",
+     "                halvings += 1;"),
+    ("C retardation storage", "R = RC[c] * phi[c] / dt_C;", "R = RC[c] / dt_C;"),
+    ("l2 norm squares", "s1 += d * d;", "s1 += fabs(d);"),
+]
+
+
+def _build_mutant(tmp, old, new):
+    src = open(SRC).read()
+    assert src.count(old) == 1, f"mutation anchor not unique: {old!r}"
+    with open(os.path.join(tmp, "tse_oracle.c"), "w") as f:
+        f.write(src.replace(old, new))
+    shutil.copy(HDR, tmp)
+    so = os.path.join(tmp, "libmut.so")
+    subprocess.run(["gcc", "-O1", "-fPIC", "-ffp-contract=off", "-fopenmp",
+                    "-shared", "-o", so, os.path.join(tmp, "tse_oracle.c"),
+                    "-lm"], check=True)
+    return so
+
+
+def _golden_differs(lib_path, case, tmp):
+    from oracle import Oracle
+    from transportse_amd.driver import ConvergenceError, Simulation
+
+    def backend(sim):
+        st, c = sim.st, sim.config
+        return Oracle(st.nx, st.ny, st.nz, st.dx, st.dy, st.dz, st.g,
+                      st.beta_deg, frozen_source=st, maxit_p=sim.MAXIT_P,
+                      maxit_c=sim.MAXIT_C, maxit_t=sim.MAXIT_T,
+                      dt_max=c.dt_max, c_imblflo=c.C_IMBLFLO,
+                      t_imblflo=c.T_IMBLFLO, lib_path=lib_path)
+
+    g = np.load(f"{GOLDEN}/{case}.npz")
+    sim = Simulation(f"{GOLDEN}/configs/{case}.py", backend, verbose=False,
+                     record_fields=True, out_dir=tmp, max_steps=g["P"].shape[0])
+    try:
+        traj = sim.run()
+    except (ConvergenceError, FloatingPointError, ValueError):
+        return True
+    if len(traj) != g["P"].shape[0]:
+        return True
+    for s, r in enumerate(traj):
+        for k in ("P", "C", "T", "qx", "qy", "qz", "e", "rho_f"):
+            if not np.array_equal(r[k], g[k][s]):
+                return True
+        for k in ("numitP", "numitC", "numitT"):
+            if r[k] != int(g[k][s]):
+                return True
+    return False
+
+
+@pytest.mark.parametrize("name,old,new", MUTANTS, ids=[m[0] for m in MUTANTS])
+def test_planted_mistake_is_caught(name, old, new, tmp_path, oracle_mod):
+    so = _build_mutant(str(tmp_path), old, new)
+    assert any(_golden_differs(so, c, str(tmp_path)) for c in CASES), name
+
+
+def test_unmutated_copy_matches(tmp_path, oracle_mod):
+    """Control for the harness: the same build of the unchanged source agrees."""
+    so = _build_mutant(str(tmp_path), "s1 += d * d;", "s1 += d * d;")
+    assert not any(_golden_differs(so, c, str(tmp_path)) for c in CASES)
