@@ -168,6 +168,50 @@ int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
 double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
                             int32_t *is_nan);
 
+/* ---- performance forms of the pressure solve ------------------------------ */
+/* The compute_P system (tse.py:657-697 at its fixed point) scaled by the cell
+ * volume is symmetric: each cell stores its three "+" face transmissibilities,
+ * 1/diagonal and the right-hand side (natural x-contiguous layout, owned by
+ * the context).  Sweeps are fused red+black z-marches (NS sweeps per launch);
+ * convergence is the relative residual ||b - A p|| / ||b|| in FP64.
+ *   TSE_FORM_PLAIN64: FP64 sweeps on p.
+ *   TSE_FORM_MIXED:   FP32 sweeps on the correction A e = r, r = b - A p in
+ *                     FP64, p += e in FP64 (defect correction). */
+typedef struct {
+    int32_t form;       /* TSE_FORM_PLAIN64 or TSE_FORM_MIXED              */
+    int32_t ns;         /* sweeps fused per launch: 1, 2 (3: MIXED only)    */
+    int32_t by;         /* tile rows: 8 or 16 (ns > 1: 16)                  */
+    int32_t zchunk;     /* z planes per block (<= 0: whole column)          */
+    int32_t inner;      /* sweeps per residual check / correction cycle     */
+    int32_t max_sweeps; /* stop after this many sweeps                      */
+    double sor;         /* over-relaxation factor                           */
+    double tol;         /* relative residual target                         */
+} tse_fast_params;
+
+typedef struct {
+    int32_t sweeps;     /* sweeps performed                                 */
+    int32_t cycles;     /* residual checks / correction cycles              */
+    int32_t converged;  /* 1 if rel_residual <= tol                          */
+    int32_t pad;
+    double initial_rel; /* relative residual of the initial iterate         */
+    double rel_residual;
+} tse_fast_result;
+
+/* Assembles the operator from compute_P's inputs at time step dt (Pn = the
+ * storage-term reference pressure; NULL uses P) and loads P as the iterate. */
+int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt);
+/* Solves to prm->tol (or prm->max_sweeps) from the current iterate. */
+int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
+                     tse_fast_result *out);
+/* Relative residual of the current iterate. */
+int tse_fast_p_residual(tse_ctx *ctx, double *rel);
+/* Writes the current iterate into a split-layout field. */
+int tse_fast_p_result(tse_ctx *ctx, double *P);
+/* Timing hook: n_sweeps fused sweeps (FP64 on p, or fp32 != 0: FP32 on the
+ * correction with the last residual), no residual checks. */
+int tse_fast_p_sweeps(tse_ctx *ctx, int32_t n_sweeps, double sor, int32_t ns,
+                      int32_t by, int32_t zchunk, int32_t fp32);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
  * fixed dt (no norm, no control), for timing and sweep-level parity.

@@ -1,0 +1,95 @@
+"""GPU: the performance forms of the pressure solve (plain FP64, mixed FP32).
+
+1. The fused red/black z-march (NS sweeps per launch, z-chunks with halos,
+   ping-pong buffers) must reproduce the same number of plain red/black
+   sweeps of the oracle (reference arithmetic, tse.py:705-716). The fast
+   form uses the volume-scaled symmetric operator, so rounding differs:
+   tolerance 1e-10 of the change the sweeps make.
+2. Solving to a relative residual target must land on the oracle's fully
+   converged compute_P fixed point: plain FP64 to 1e-12 and mixed
+   FP32-in-FP64 to 1e-10, each within the error that residual bounds.
+"""
+import ctypes
+import types
+
+import numpy as np
+import pytest
+
+from helpers import Fields
+
+pytestmark = pytest.mark.gpu
+
+PLANS = [(1, 8, 0), (1, 16, 5), (2, 16, 0), (2, 16, 6)]
+
+
+def _setup(f, dt):
+    from transportse_amd import _lib
+    from transportse_amd.device import HipContext
+    hip = HipContext(*f.grid_args())
+    d = {n: hip.upload(getattr(f, n)) for n in
+         ("c_f", "nue", "rho_f", "phi", "Kx", "Ky", "Kz", "BP", "WP", "c_r")}
+    d["P"] = hip.upload(f.P)
+    d["Pn"] = hip.upload(f.P)
+    pf = _lib.PFields(**{n: d[n].data_ptr() for n in _lib.P_FIELDS})
+    _lib.check(hip.lib.tse_fast_p_setup(hip.ctx, ctypes.byref(pf), dt),
+               "tse_fast_p_setup")
+    return hip, d
+
+
+def _result(hip, d):
+    from transportse_amd import _lib
+    _lib.check(hip.lib.tse_fast_p_result(hip.ctx, d["P"].data_ptr()),
+               "tse_fast_p_result")
+    return hip.download(d["P"])
+
+
+@pytest.mark.parametrize("shape", [(70, 23, 19), (33, 20, 17), (9, 40, 12)])
+@pytest.mark.parametrize("ns,by,zchunk", PLANS)
+def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, zchunk):
+    from oracle import Oracle
+    from transportse_amd import _lib
+    f = Fields(*shape, seed=21)
+    dt, sor, n = 3600.0, 1.7, 4
+    o = Oracle(*f.grid_args(), frozen_source=types.SimpleNamespace(c_r=f.c_r))
+    Po = f.P.copy()
+    for _ in range(n):
+        for colour in (0, 1):
+            o.p_half_sweep(Po, f.P, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky,
+                           f.Kz, f.BP, f.WP, dt, sor, colour)
+    hip, d = _setup(f, dt)
+    _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, n, sor, ns, by, zchunk, 0),
+               "tse_fast_p_sweeps")
+    Ph = _result(hip, d)
+    change = np.max(np.abs(Po - f.P))
+    err = np.max(np.abs(Ph - Po))
+    print("fused", shape, ns, by, zchunk, "change", change, "err", err, flush=True)
+    assert change > 0
+    assert err <= 1e-10 * change
+
+
+@pytest.mark.parametrize("form,tol,bound", [(1, 1e-12, 1e-8), (2, 1e-10, 1e-6)])
+def test_solve_reaches_oracle_fixed_point(form, tol, bound):
+    from oracle import Oracle
+    from transportse_amd import _lib
+    f = Fields(40, 30, 24, seed=22)
+    dt = 3600.0
+    o = Oracle(*f.grid_args(), frozen_source=types.SimpleNamespace(c_r=f.c_r))
+    Po = f.P.copy()
+    r = o.compute_P(Po, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky, f.Kz, f.qx,
+                    f.qy, f.qz, dt, f.BP, f.WP, SOR=1.9, max_it=200000,
+                    inc_it=1.0, max_dt=dt, P_conv=1e-15)
+    assert r[4]
+    hip, d = _setup(f, dt)
+    prm = _lib.FastParams(form=form, ns=2 if form == 2 else 1, by=16, zchunk=0,
+                          inner=20, max_sweeps=200000, sor=1.9, tol=tol)
+    res = _lib.FastResult()
+    _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                        ctypes.byref(res)), "tse_fast_p_solve")
+    assert res.converged, (res.sweeps, res.rel_residual)
+    Ph = _result(hip, d)
+    change = np.max(np.abs(Po - f.P))
+    err = np.max(np.abs(Ph - Po))
+    print("solve form", form, "sweeps", res.sweeps, "rel", res.rel_residual,
+          "err/change", err / change, "err/|P|", err / np.max(np.abs(Po)),
+          flush=True)
+    assert err <= bound * change

@@ -43,6 +43,20 @@ T_FIELDS = ("T", "Tn", "l_f", "cp_f", "rho_f", "phi", "qx", "qy", "qz", "BT",
             "WT", "e", "l_r", "cp_r", "rho_r", "lt_l", "lt_t")
 
 
+class FastParams(ctypes.Structure):
+    _fields_ = [("form", ctypes.c_int32), ("ns", ctypes.c_int32),
+                ("by", ctypes.c_int32), ("zchunk", ctypes.c_int32),
+                ("inner", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
+                ("sor", ctypes.c_double), ("tol", ctypes.c_double)]
+
+
+class FastResult(ctypes.Structure):
+    _fields_ = [("sweeps", ctypes.c_int32), ("cycles", ctypes.c_int32),
+                ("converged", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("initial_rel", ctypes.c_double),
+                ("rel_residual", ctypes.c_double)]
+
+
 class PFields(ctypes.Structure):
     _fields_ = [(n, _dp) for n in P_FIELDS]
 
@@ -82,6 +96,13 @@ SIGNATURES = {
     "tse_host_l2_norm_seq": (_D, [_DPH, _DPH, _L, ctypes.POINTER(ctypes.c_int32)]),
     "tse_p_sweeps": (_I, [_V, ctypes.POINTER(PFields), _D, _D, ctypes.c_int32,
                            ctypes.c_int32]),
+    "tse_fast_p_setup": (_I, [_V, ctypes.POINTER(PFields), _D]),
+    "tse_fast_p_solve": (_I, [_V, ctypes.POINTER(FastParams),
+                              ctypes.POINTER(FastResult)]),
+    "tse_fast_p_residual": (_I, [_V, _DPH]),
+    "tse_fast_p_result": (_I, [_V, _dp]),
+    "tse_fast_p_sweeps": (_I, [_V, ctypes.c_int32, _D, ctypes.c_int32,
+                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
 }
 
 _lib = None

@@ -31,6 +31,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 UNITS = {
     "tse_core.hip": ["-ffp-contract=off"],
     "tse_faithful.hip": ["-ffp-contract=off"],
+    "tse_fast.hip": ["-ffp-contract=fast"],
 }
 
 

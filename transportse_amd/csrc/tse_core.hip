@@ -248,6 +248,7 @@ int tse_ctx_destroy(tse_ctx *ctx) {
     if (ctx->ctl_host) (void)hipHostFree(ctx->ctl_host);
     if (ctx->red_out) (void)hipFree(ctx->red_out);
     if (ctx->red_host) (void)hipHostFree(ctx->red_host);
+    tse::fast_free(ctx);
     delete ctx;
     return 0;
 }

@@ -1,0 +1,683 @@
+// tse_fast.hip -- performance forms of the pressure solve (plain FP64 and
+// FP32-in-FP64 defect correction).
+//
+// Operator.  At its fixed point the compute_P update (tse.py:657-697) solves
+//   (R + M) P_c - sum_f a_f P_f = gravity_c + R Pn_c + WP_c.
+// Scaled by the cell volume V_c the face coefficients become symmetric:
+//   W_f = harm(K) / harm(nue) * harm(rho) / ((d1 + d2)/2) * (face area),
+// so each cell stores its three "+" faces Wx, Wy, Wz (toward i+1, j+1, k+1),
+// the inverse diagonal invD = 1 / (V R + sum of its six W) (0 marks a
+// Dirichlet cell) and the right-hand side
+//   b = V (R Pn + WP) + Wx+ Gx+ - Wx- Gx- + Wz+ Gz+ - Wz- Gz-
+// with G the gravity head of a face (beta_g, tse.py:263-271).  Arrays use the
+// natural x-contiguous layout idx = i + nx*(j + ny*k).
+//
+// Sweep kernel (rb_zmarch).  A block owns a tile of 64 x BY columns and marches
+// up a z-chunk, streaming one z-plane per step through LDS (P, Wx, Wy of the
+// plane; coalesced 64-wide row loads).  NS full red+black sweeps are fused per
+// launch as a phase pipeline: phase h (colour h & 1, red first) updates plane
+// k-1-h at step k, after phase h-1 has updated planes k-h .. k-2-h.  A phase
+// is valid on the columns at least h+1 cells inside the tile edge, so the
+// tile's outer ring of 2*NS columns is halo (recomputed by the neighbouring
+// tile) and the inner (64-4NS) x (BY-4NS) columns are written.  A z-chunk
+// loads 2*NS extra planes below and above for the same reason.  Input and
+// output arrays are distinct (ping-pong), so tiles and chunks never see each
+// other's writes within a launch.  Each launch reads every operand once and
+// writes the unknown once for NS sweeps.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "tse_internal.h"
+
+namespace tse {
+
+struct ZGeo {
+    int nx, ny, nz;
+    int tiles_x, tiles_y, zchunk, nzc;
+    int ntiles, per_xcd;
+    FastDiv div_tx, div_ty;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs; map block b to tile
+// (b % 8) * per_xcd + b / 8 so each XCD (own L2) works on a contiguous band of
+// tiles and shares their halos.
+__device__ inline int xcd_tile(const ZGeo &g) {
+    const int b = blockIdx.x;
+    return (b & 7) * g.per_xcd + (b >> 3);
+}
+
+template <typename T, int NS, int BY>
+__global__ void __launch_bounds__(64 * BY)
+rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                 const T *__restrict__ Wx, const T *__restrict__ Wy,
+                 const T *__restrict__ Wz, const T *__restrict__ invD,
+                 const T *__restrict__ rhs, T omega, int zero_init) {
+    constexpr int R = 2 * NS;          // halo ring width / number of phases
+    constexpr int BX = 64;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int NSLOT = R + 1;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[NSLOT][BY][BX];
+    __shared__ T sWx[NSLOT][BY][BX];
+    __shared__ T sWy[NSLOT][BY][BX];
+
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;  // uniform per block
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);   // tile / tiles_x
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);    // / tiles_y
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int i = tx * TX - R + lx;
+    const int j = ty * TY - R + ly;
+    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const long plane = (long)g.nx * g.ny;
+    const long col = in_dom ? (long)i + (long)g.nx * j : 0;
+
+    T p[R + 2], rinv[R + 1], rb[R + 1], rwz[R + 2];
+#pragma unroll
+    for (int q = 0; q < R + 2; ++q) {
+        p[q] = T(0);
+        rwz[q] = T(0);
+    }
+#pragma unroll
+    for (int q = 0; q < R + 1; ++q) {
+        rinv[q] = T(0);
+        rb[q] = T(0);
+    }
+
+    for (int k = kl0; k <= k1 + R - 1; ++k) {
+        T p0 = T(0), i0 = T(0), b0 = T(0), wz0 = T(0), wx0 = T(0), wy0 = T(0);
+        if (in_dom && k <= kl1) {
+            const long o = col + plane * k;
+            p0 = zero_init ? T(0) : Pin[o];
+            i0 = invD[o];
+            b0 = rhs[o];
+            wz0 = Wz[o];
+            wx0 = Wx[o];
+            wy0 = Wy[o];
+        }
+#pragma unroll
+        for (int q = R + 1; q > 0; --q) {
+            p[q] = p[q - 1];
+            rwz[q] = rwz[q - 1];
+        }
+#pragma unroll
+        for (int q = R; q > 0; --q) {
+            rinv[q] = rinv[q - 1];
+            rb[q] = rb[q - 1];
+        }
+        p[0] = p0;
+        rwz[0] = wz0;
+        rinv[0] = i0;
+        rb[0] = b0;
+        const int s0 = k % NSLOT;
+        __syncthreads();  // slot s0 held plane k-NSLOT, last read at step k-1
+        sP[s0][ly][lx] = p0;
+        sWx[s0][ly][lx] = wx0;
+        sWy[s0][ly][lx] = wy0;
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+            __syncthreads();
+            const int m = k - 1 - h;
+            if (m >= kl0 && m <= kl1 && d >= h + 1 && in_dom &&
+                ((i + j + m) & 1) == (h & 1) && rinv[1 + h] > T(0)) {
+                const int s = m % NSLOT;
+                T acc = rb[1 + h];
+                acc += sWx[s][ly][lx] * sP[s][ly][lx + 1];
+                acc += sWx[s][ly][lx - 1] * sP[s][ly][lx - 1];
+                acc += sWy[s][ly][lx] * sP[s][ly + 1][lx];
+                acc += sWy[s][ly - 1][lx] * sP[s][ly - 1][lx];
+                acc += rwz[1 + h] * p[h];
+                acc += rwz[2 + h] * p[h + 2];
+                const T pc = p[1 + h];
+                const T pn = pc + omega * (acc * rinv[1 + h] - pc);
+                p[1 + h] = pn;
+                sP[s][ly][lx] = pn;
+            }
+        }
+        const int mo = k - R;
+        if (mo >= k0 && mo < k1 && d >= R && in_dom) Pout[col + plane * mo] = p[R];
+    }
+}
+
+// ---- operator assembly (from split-layout fields) ---------------------------------------
+
+__device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
+    return n == 0 ? G.dx[i] : (n == 1 ? G.dy[j] : G.dz[k]);
+}
+
+__device__ inline double f_harm(const Geo &G, const double *H, int i1, int j1,
+                                int k1, int i2, int j2, int k2, int n) {
+    const double h1 = H[slot(G, i1, j1, k1)], h2 = H[slot(G, i2, j2, k2)];
+    if (h1 != 0 && h2 != 0) {
+        const double d1 = f_dsp(G, n, i1, j1, k1), d2 = f_dsp(G, n, i2, j2, k2);
+        return (d1 + d2) / (d1 / h1 + d2 / h2);
+    }
+    return 0.0;
+}
+
+__device__ inline double f_amean(const Geo &G, const double *X, int i1, int j1,
+                                 int k1, int i2, int j2, int k2, int n) {
+    const double x1 = X[slot(G, i1, j1, k1)], x2 = X[slot(G, i2, j2, k2)];
+    const double d1 = f_dsp(G, n, i1, j1, k1), d2 = f_dsp(G, n, i2, j2, k2);
+    return (x1 * d1 + x2 * d2) / (d1 + d2);
+}
+
+// Symmetric face transmissibility between (i,j,k) and its + neighbour on
+// axis n (caller guarantees the neighbour exists).
+__device__ inline double face_W(const Geo &G, const double *K,
+                                const double *nue, const double *rho, int i,
+                                int j, int k, int n) {
+    const int i2 = i + (n == 0), j2 = j + (n == 1), k2 = k + (n == 2);
+    const double hk = f_harm(G, K, i2, j2, k2, i, j, k, n);
+    const double hn = f_harm(G, nue, i2, j2, k2, i, j, k, n);
+    const double hr = f_harm(G, rho, i2, j2, k2, i, j, k, n);
+    const double d1 = f_dsp(G, n, i2, j2, k2), d2 = f_dsp(G, n, i, j, k);
+    const double area = n == 0 ? G.dy[j] * G.dz[k]
+                                : (n == 1 ? G.dx[i] * G.dz[k] : G.dx[i] * G.dy[j]);
+    if (hn == 0.0) return 0.0;
+    return hk / hn * hr / ((d1 + d2) * 0.5) * area;
+}
+
+__device__ inline double face_G(const Geo &G, const double *rho, int i, int j,
+                                int k, int n) {
+    if (n == 0)
+        return f_amean(G, rho, i + 1, j, k, i, j, k, 0) * G.g * 0.5 *
+               (G.dx[i + 1] + G.dx[i]) * G.sinb;
+    return f_amean(G, rho, i, j, k + 1, i, j, k, 2) * G.g * 0.5 *
+           (G.dz[k + 1] + G.dz[k]) * G.cosb;
+}
+
+struct FastOp {  // natural-layout operator arrays
+    double *Wx, *Wy, *Wz, *D, *invD, *b;
+    float *Wx32, *Wy32, *Wz32, *invD32;
+};
+
+__global__ void __launch_bounds__(kBlock)
+fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny,
+                     const double *__restrict__ Pn, const double *__restrict__ c_f,
+                     const double *__restrict__ nue, const double *__restrict__ rho,
+                     const double *__restrict__ phi, const double *__restrict__ Kx,
+                     const double *__restrict__ Ky, const double *__restrict__ Kz,
+                     const double *__restrict__ BP, const double *__restrict__ WP,
+                     const double *__restrict__ c_r, double dt, FastOp op) {
+    const long N = (long)G.nx * G.ny * G.nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * G.nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * G.ny;
+        const int c = slot(G, i, j, k);
+        double W[6] = {0, 0, 0, 0, 0, 0};  // x+, x-, y+, y-, z+, z-
+        double Gx = 0, Gxm = 0, Gz = 0, Gzm = 0;
+        if (i < G.nx - 1) {
+            W[0] = face_W(G, Kx, nue, rho, i, j, k, 0);
+            Gx = face_G(G, rho, i, j, k, 0);
+        }
+        if (i > 0) {
+            W[1] = face_W(G, Kx, nue, rho, i - 1, j, k, 0);
+            Gxm = face_G(G, rho, i - 1, j, k, 0);
+        }
+        if (j < G.ny - 1) W[2] = face_W(G, Ky, nue, rho, i, j, k, 1);
+        if (j > 0) W[3] = face_W(G, Ky, nue, rho, i, j - 1, k, 1);
+        if (k < G.nz - 1) {
+            W[4] = face_W(G, Kz, nue, rho, i, j, k, 2);
+            Gz = face_G(G, rho, i, j, k, 2);
+        }
+        if (k > 0) {
+            W[5] = face_W(G, Kz, nue, rho, i, j, k - 1, 2);
+            Gzm = face_G(G, rho, i, j, k - 1, 2);
+        }
+        const double V = G.dx[i] * G.dy[j] * G.dz[k];
+        const double por = phi[c];
+        const double R = rho[c] * ((1 - por) * c_r[c] + por * c_f[c]) / dt;
+        const double Dg = V * R + (W[0] + W[1]) + (W[2] + W[3]) + (W[4] + W[5]);
+        const double bb = V * (R * Pn[c] + WP[c]) + W[0] * Gx - W[1] * Gxm +
+                          W[4] * Gz - W[5] * Gzm;
+        const bool active = !(BP[c] != 0) && Dg > 0.0;
+        const double inv = active ? 1.0 / Dg : 0.0;
+        op.Wx[t] = W[0];
+        op.Wy[t] = W[2];
+        op.Wz[t] = W[4];
+        op.D[t] = Dg;
+        op.invD[t] = inv;
+        op.b[t] = bb;
+        op.Wx32[t] = (float)W[0];
+        op.Wy32[t] = (float)W[2];
+        op.Wz32[t] = (float)W[4];
+        op.invD32[t] = (float)inv;
+    }
+}
+
+// ---- layout conversion split <-> natural ------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock)
+split_to_nat_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ src,
+                    double *__restrict__ dst) {
+    const long N = (long)G.nx * G.ny * G.nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * G.nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * G.ny;
+        dst[t] = src[slot(G, i, j, k)];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+nat_to_split_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ src,
+                    double *__restrict__ dst) {
+    const long N = (long)G.nx * G.ny * G.nz;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * G.nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * G.ny;
+        dst[slot(G, i, j, k)] = src[t];
+    }
+}
+
+// ---- FP64 residual r = b - A p (and p <- p + e) ------------------------------------------------
+
+__device__ inline double wsum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// mode 0: check only (||r||, ||b||) on p_in.
+// mode 1: defect correction: p_out = p_in + e; r = b - A p_out written as FP32.
+// mode 2: r = b - A p_in written as FP32 (first correction cycle).
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
+                const double *__restrict__ pin, const float *__restrict__ e,
+                double *__restrict__ pout, FastOp op, float *__restrict__ r32,
+                double *__restrict__ partials) {
+    const long N = (long)nx * ny * nz;
+    const long plane = (long)nx * ny;
+    double sr = 0.0, sb = 0.0;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * ny;
+        auto P = [&](long o) -> double {
+            return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
+        };
+        const double pc = P(t);
+        if (MODE == 1) pout[t] = pc;
+        double r = 0.0;
+        if (op.invD[t] > 0.0) {
+            double acc = op.b[t];
+            if (i < nx - 1) acc += op.Wx[t] * P(t + 1);
+            if (i > 0) acc += op.Wx[t - 1] * P(t - 1);
+            if (j < ny - 1) acc += op.Wy[t] * P(t + nx);
+            if (j > 0) acc += op.Wy[t - nx] * P(t - nx);
+            if (k < nz - 1) acc += op.Wz[t] * P(t + plane);
+            if (k > 0) acc += op.Wz[t - plane] * P(t - plane);
+            r = acc - op.D[t] * pc;
+            sr += r * r;
+            sb += op.b[t] * op.b[t];
+        }
+        if (MODE != 0) r32[t] = (float)r;
+    }
+    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
+    sr = wsum(sr);
+    sb = wsum(sb);
+    if ((threadIdx.x & 63) == 0) {
+        s1[threadIdx.x >> 6] = sr;
+        s2[threadIdx.x >> 6] = sb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            a += s1[q];
+            c += s2[q];
+        }
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = c;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+reduce2_kernel(const double *__restrict__ partials, int n, double *__restrict__ out) {
+    double a = 0, c = 0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) {
+        a += partials[2 * q];
+        c += partials[2 * q + 1];
+    }
+    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
+    a = wsum(a);
+    c = wsum(c);
+    if ((threadIdx.x & 63) == 0) {
+        s1[threadIdx.x >> 6] = a;
+        s2[threadIdx.x >> 6] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double x = 0, y = 0;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            x += s1[q];
+            y += s2[q];
+        }
+        out[0] = x;
+        out[1] = y;
+    }
+}
+
+// ---- host side ---------------------------------------------------------------------------------------
+
+// Fast-form buffers owned by the context.
+struct FastState {
+    long N = 0;
+    double *buf64 = nullptr;  // Wx Wy Wz D invD b | p[0] p[1]   (8 N)
+    float *buf32 = nullptr;   // Wx Wy Wz invD r | e[0] e[1]     (7 N)
+    double *partials = nullptr;
+    double *red = nullptr;
+    double *red_host = nullptr;
+    int cur = 0;              // which of p[0], p[1] holds the iterate
+    bool ready = false;
+};
+
+static FastState *state(tse_ctx *ctx) {
+    if (!ctx->fast) ctx->fast = new FastState();
+    return (FastState *)ctx->fast;
+}
+
+void fast_free(tse_ctx *ctx) {
+    FastState *s = (FastState *)ctx->fast;
+    if (!s) return;
+    if (s->buf64) (void)hipFree(s->buf64);
+    if (s->buf32) (void)hipFree(s->buf32);
+    if (s->partials) (void)hipFree(s->partials);
+    if (s->red) (void)hipFree(s->red);
+    if (s->red_host) (void)hipHostFree(s->red_host);
+    delete s;
+    ctx->fast = nullptr;
+}
+
+static int ensure(tse_ctx *ctx, FastState *s) {
+    const Geo &G = ctx->geo;
+    const long N = (long)G.nx * G.ny * G.nz;
+    if (!s->partials) {
+        if (hipMalloc((void **)&s->partials, sizeof(double) * 2 * 4096) != hipSuccess ||
+            hipMalloc((void **)&s->red, sizeof(double) * 2) != hipSuccess ||
+            hipHostMalloc((void **)&s->red_host, sizeof(double) * 2) != hipSuccess)
+            return fail(TSE_ERR_NOMEM, "fast-form scratch allocation failed");
+    }
+    if (s->N == N && s->buf64) return 0;
+    if (s->buf64) {
+        TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+        TSE_HIP_TRY(hipFree(s->buf64));
+        TSE_HIP_TRY(hipFree(s->buf32));
+        s->buf64 = nullptr;
+        s->buf32 = nullptr;
+    }
+    if (hipMalloc((void **)&s->buf64, sizeof(double) * 8 * N) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "fast-form FP64 operator allocation failed");
+    if (hipMalloc((void **)&s->buf32, sizeof(float) * 7 * N) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "fast-form FP32 operator allocation failed");
+    s->N = N;
+    s->ready = false;
+    return 0;
+}
+
+static FastOp fast_op(FastState *s) {
+    const long N = s->N;
+    FastOp op;
+    op.Wx = s->buf64;
+    op.Wy = s->buf64 + N;
+    op.Wz = s->buf64 + 2 * N;
+    op.D = s->buf64 + 3 * N;
+    op.invD = s->buf64 + 4 * N;
+    op.b = s->buf64 + 5 * N;
+    op.Wx32 = s->buf32;
+    op.Wy32 = s->buf32 + N;
+    op.Wz32 = s->buf32 + 2 * N;
+    op.invD32 = s->buf32 + 3 * N;
+    return op;
+}
+
+static inline double *pbuf(FastState *s, int q) { return s->buf64 + (6 + q) * s->N; }
+static inline float *r32buf(FastState *s) { return s->buf32 + 4 * s->N; }
+static inline float *ebuf(FastState *s, int q) { return s->buf32 + (5 + q) * s->N; }
+
+static int grid_n(long n) {
+    long b = (n + kBlock - 1) / kBlock;
+    return (int)std::max(1L, std::min(b, 4096L));
+}
+
+// Launches a residual pass and returns ||r||, ||b|| (synchronises).
+template <int MODE>
+static int residual(tse_ctx *ctx, FastState *s, const double *pin, const float *e,
+                    double *pout, double *rn, double *bn) {
+    const Geo &G = ctx->geo;
+    const int nb = grid_n(s->N);
+    hipLaunchKernelGGL((residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
+                       ctx->stream, G.nx, G.ny, G.nz, make_fastdiv(G.nx),
+                       make_fastdiv(G.ny), pin, e, pout, fast_op(s), r32buf(s),
+                       s->partials);
+    TSE_HIP_TRY(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
+                       s->partials, nb, s->red);
+    TSE_HIP_TRY(hipGetLastError());
+    TSE_HIP_TRY(hipMemcpyAsync(s->red_host, s->red, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *rn = std::sqrt(s->red_host[0]);
+    *bn = std::sqrt(s->red_host[1]);
+    return 0;
+}
+
+template <typename T>
+static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
+                         const T *Pin, T *Pout, const T *Wx, const T *Wy,
+                         const T *Wz, const T *invD, const T *rhs, T omega,
+                         int zero_init) {
+    const dim3 grid(zg.per_xcd * 8);
+#define TSE_ZM(NS_, BY_)                                                        \
+    if (ns == NS_ && by == BY_) {                                               \
+        hipLaunchKernelGGL((rb_zmarch_kernel<T, NS_, BY_>), grid, dim3(64, BY_), \
+                           0, st, zg, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,  \
+                           zero_init);                                          \
+        TSE_HIP_TRY(hipGetLastError());                                         \
+        return 0;                                                               \
+    }
+    TSE_ZM(1, 8)
+    TSE_ZM(1, 16)
+    TSE_ZM(2, 16)
+    if constexpr (sizeof(T) == 4) {
+        TSE_ZM(3, 16)
+    }
+#undef TSE_ZM
+    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by)");
+}
+
+bool fast_plan_ok(int ns, int by, int fp32) {
+    return (ns == 1 && (by == 8 || by == 16)) || (ns == 2 && by == 16) ||
+           (fp32 && ns == 3 && by == 16);
+}
+
+ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
+    ZGeo z{};
+    z.nx = G.nx;
+    z.ny = G.ny;
+    z.nz = G.nz;
+    const int R = 2 * ns, TX = 64 - 2 * R, TY = by - 2 * R;
+    z.tiles_x = (G.nx + TX - 1) / TX;
+    z.tiles_y = (G.ny + TY - 1) / TY;
+    z.zchunk = std::max(1, std::min(zchunk > 0 ? zchunk : G.nz, G.nz));
+    z.nzc = (G.nz + z.zchunk - 1) / z.zchunk;
+    z.ntiles = z.tiles_x * z.tiles_y * z.nzc;
+    z.per_xcd = (z.ntiles + 7) / 8;
+    z.div_tx = make_fastdiv((uint32_t)z.tiles_x);
+    z.div_ty = make_fastdiv((uint32_t)z.tiles_y);
+    return z;
+}
+
+// n sweeps on the iterate (plain) or on the correction e (mixed, e starts
+// at 0 when zero_first); keeps the ping-pong index.
+static int plain_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg, int ns,
+                        int by, int n, double sor) {
+    FastOp op = fast_op(s);
+    for (int q = 0; q < n / ns; ++q) {
+        int rc = launch_zmarch<double>(ctx->stream, zg, ns, by, pbuf(s, s->cur),
+                                       pbuf(s, 1 - s->cur), op.Wx, op.Wy, op.Wz,
+                                       op.invD, op.b, sor, 0);
+        if (rc) return rc;
+        s->cur = 1 - s->cur;
+    }
+    return 0;
+}
+
+static int correction_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg, int ns,
+                             int by, int n, float sor, int *ecur) {
+    FastOp op = fast_op(s);
+    int e = 0;
+    for (int q = 0; q < n / ns; ++q) {
+        int rc = launch_zmarch<float>(ctx->stream, zg, ns, by, ebuf(s, e),
+                                      ebuf(s, 1 - e), op.Wx32, op.Wy32, op.Wz32,
+                                      op.invD32, r32buf(s), sor, q == 0);
+        if (rc) return rc;
+        e = 1 - e;
+    }
+    *ecur = e;
+    return 0;
+}
+
+}  // namespace tse
+
+using namespace tse;
+
+extern "C" {
+
+int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !f->P || !f->c_f || !f->nue || !f->rho_f || !f->phi ||
+        !f->Kx || !f->Ky || !f->Kz || !f->BP || !f->WP || !f->c_r)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    if (!(dt > 0)) return fail(TSE_ERR_ARG, "dt must be > 0");
+    FastState *s = state(ctx);
+    rc = ensure(ctx, s);
+    if (rc) return rc;
+    const Geo &G = ctx->geo;
+    const FastDiv dnx = make_fastdiv(G.nx), dny = make_fastdiv(G.ny);
+    const double *Pn = f->Pn ? f->Pn : f->P;
+    hipLaunchKernelGGL(fast_assemble_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, dnx, dny, Pn, f->c_f, f->nue, f->rho_f,
+                       f->phi, f->Kx, f->Ky, f->Kz, f->BP, f->WP, f->c_r, dt,
+                       fast_op(s));
+    TSE_HIP_TRY(hipGetLastError());
+    s->cur = 0;
+    hipLaunchKernelGGL(split_to_nat_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, dnx, dny, f->P, pbuf(s, 0));
+    TSE_HIP_TRY(hipGetLastError());
+    s->ready = true;
+    return 0;
+}
+
+int tse_fast_p_result(tse_ctx *ctx, double *P) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    FastState *s = state(ctx);
+    if (!s->ready || !P) return fail(TSE_ERR_STATE, "fast form not set up");
+    const Geo &G = ctx->geo;
+    hipLaunchKernelGGL(nat_to_split_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       pbuf(s, s->cur), P);
+    TSE_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int tse_fast_p_residual(tse_ctx *ctx, double *rel) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    FastState *s = state(ctx);
+    if (!s->ready || !rel) return fail(TSE_ERR_STATE, "fast form not set up");
+    double rn, bn;
+    rc = residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
+    if (rc) return rc;
+    *rel = rn / (bn > 0 ? bn : 1.0);
+    return 0;
+}
+
+int tse_fast_p_sweeps(tse_ctx *ctx, int32_t n_sweeps, double sor, int32_t ns,
+                      int32_t by, int32_t zchunk, int32_t fp32) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    FastState *s = state(ctx);
+    if (!s->ready) return fail(TSE_ERR_STATE, "fast form not set up");
+    if (!fast_plan_ok(ns, by, fp32) || n_sweeps < 0 || n_sweeps % ns)
+        return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
+    const ZGeo zg = make_zgeo(ctx->geo, ns, by, zchunk);
+    if (!fp32) return plain_sweeps(ctx, s, zg, ns, by, n_sweeps, sor);
+    // FP32 timing path: sweeps on the correction equation with the last r
+    int e;
+    return correction_sweeps(ctx, s, zg, ns, by, n_sweeps, (float)sor, &e);
+}
+
+int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
+                     tse_fast_result *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    FastState *s = state(ctx);
+    if (!s->ready || !prm || !out) return fail(TSE_ERR_STATE, "fast form not set up");
+    const int fp32 = prm->form == TSE_FORM_MIXED;
+    if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
+        return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
+    const int ns = prm->ns, by = prm->by;
+    const int chunk = prm->inner;
+    if (!fast_plan_ok(ns, by, fp32) || chunk < ns || chunk % ns || prm->max_sweeps < 0)
+        return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    const ZGeo zg = make_zgeo(ctx->geo, ns, by, prm->zchunk);
+    double rn, bn;
+    rc = fp32 ? residual<2>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn)
+              : residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
+    if (rc) return rc;
+    const double bs = bn > 0 ? bn : 1.0;
+    out->initial_rel = rn / bs;
+    double rel = rn / bs;
+    int sweeps = 0, cycles = 0;
+    while (rel > prm->tol && sweeps < prm->max_sweeps) {
+        if (fp32) {
+            int e;
+            rc = correction_sweeps(ctx, s, zg, ns, by, chunk, (float)prm->sor, &e);
+            if (rc) return rc;
+            rc = residual<1>(ctx, s, pbuf(s, s->cur), ebuf(s, e),
+                             pbuf(s, 1 - s->cur), &rn, &bn);
+            if (rc) return rc;
+            s->cur = 1 - s->cur;
+        } else {
+            rc = plain_sweeps(ctx, s, zg, ns, by, chunk, prm->sor);
+            if (rc) return rc;
+            rc = residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
+            if (rc) return rc;
+        }
+        sweeps += chunk;
+        cycles += 1;
+        rel = rn / bs;
+        if (!(rel == rel)) break;  // NaN
+    }
+    out->sweeps = sweeps;
+    out->cycles = cycles;
+    out->rel_residual = rel;
+    out->converged = rel <= prm->tol ? 1 : 0;
+    return 0;
+}
+
+}  // extern "C"

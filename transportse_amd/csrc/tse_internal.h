@@ -132,6 +132,7 @@ struct tse_ctx {
     tse::Ctl *ctl_host = nullptr; // pinned host mirror
     double *red_out = nullptr;    // device scratch for tse_l2_norm
     double *red_host = nullptr;   // pinned
+    void *fast = nullptr;         // performance-form buffers (tse_fast.hip)
 };
 
 namespace tse {
@@ -160,4 +161,6 @@ int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
 int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
                     double omega);
 int launch_l2(tse_ctx *ctx, const double *New, const double *Old);
+// implemented in tse_fast.hip
+void fast_free(tse_ctx *ctx);
 }  // namespace tse
