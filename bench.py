@@ -1,23 +1,31 @@
 """Benchmark of the TRANSPORTSE SOR hot path on one MI355X.
 
-Workload (default): the 512x512x256 pressure problem (5 m cells, lognormal
-permeability sigma_lnK = 2.5 with exponential correlation length 16 cells,
-seed 3, hydrostatic Dirichlet x faces, salt patch on top).  One step = one
-compute_P call of `--sweeps` red/black iterations of the faithful FP64 form
-at fixed dt (coefficient assembly + sweeps), with inputs resident in HBM.
+Workload: the 512x512x256 pressure problem -- 5 m cells, lognormal
+permeability (sigma_lnK 2.5, exponential correlation length 16 cells,
+seed 3), porosity 0.2, linear EOS rho = 1000 + 700 C with a salt patch on
+top, hydrostatic Dirichlet x faces, dt = 1 day.  One step = one pressure
+solve (compute_P's system) from the hydrostatic initial state to relative
+residual ||b - A p|| / ||b|| <= 1e-10 in the mixed form: FP32 red/black
+sweeps on the correction inside FP64 defect correction (operator assembly,
+sweeps, FP64 residual passes and write-back included; inputs resident in HBM).
 
-Reported: SOR cell-updates/s (whole job), the roofline of the dominant
-kernel (the colour half-sweep) from HIP events around its launches on the
-library's stream, and a CPU baseline: the oracle's OpenMP sweep on the host
-cores over the same grid.
+Reported (one JSON line on stdout):
+  value        SOR cell-updates/s over the whole timed steps
+  roofline     dominant kernel (FP32 fused red/black sweep), algorithmic
+               28 B per cell update per sweep, timed with HIP events on the
+               library's stream
+  cpu_baseline the oracle's OpenMP red/black sweeps on the host cores
+  extra keys   time to solution, solves/s, sweeps, the plain FP64 form's time
+               to solution on the same problem and the speedup over it, and
+               the mixed solution's deviation from the FP64 solution.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--sweeps S]
-                       [--no-cpu-baseline]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+       python bench.py --profile-kernel N   (N launches of the sweep kernel
+                                              and a 1 GiB copy; for rocprof)
 """
 from __future__ import annotations
 
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -27,86 +35,124 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per cell update of the faithful FP64 record sweep:
-# 13 record words + own unknown read + write + neighbour-colour read
-FAITHFUL_BYTES_PER_UPDATE = 16 * 8
+BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
+BYTES_FP64_SWEEP = 56
 
 
 def log(*a):
-    print(*a, file=sys.stderr, flush=True)
+    print("[bench]", *a, file=sys.stderr, flush=True)
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--sweeps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--grid", type=int, nargs=3, default=[512, 512, 256])
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--omega", type=float, default=1.996)
+    ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sweeps", type=int, default=1)
+    ap.add_argument("--cpu-sweeps", type=int, default=8)
+    ap.add_argument("--profile-kernel", type=int, default=0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
-              "supported (the hot path shards by slab, but no multi-GPU run "
-              "is part of this build)", file=sys.stderr)
+              "supported (the solve shards by z slabs with halo exchange, but "
+              "no multi-GPU run is part of this build)", file=sys.stderr)
         return 2
 
     import numpy as np
     import torch
 
-    from transportse_amd import _lib
-    from transportse_amd.device import HipContext
+    from transportse_amd.pressure import FastPressure
     from transportse_amd.workloads import PressureProblem
 
     nx, ny, nz = args.grid
+    N = nx * ny * nz
     t0 = time.time()
     prob = PressureProblem(nx, ny, nz, spacing=5.0, sigma=2.5, corr_len=16,
                            seed=3)
-    log(f"[bench] inputs generated in {time.time() - t0:.1f}s")
-    N = nx * ny * nz
-    hip = HipContext(*prob.grid_args())
-    lib = hip.lib
-    d = {n: hip.upload(a) for n, a in prob.p_inputs().items()}
-    d["P"] = hip.upload(prob.P)
-    d["Pn"] = hip.upload(prob.P)
-    pf = _lib.PFields(**{n: d[n].data_ptr() for n in _lib.P_FIELDS})
-    dt, sor = 86400.0, 1.8
+    log(f"inputs generated in {time.time() - t0:.1f}s")
+    dt = 86400.0
+    fp = FastPressure(*prob.grid_args())
+    fp.load(prob.P, dt, **prob.p_inputs())
     torch.cuda.synchronize()
-    log(f"[bench] uploaded, field {hip.F} doubles")
+    log("uploaded and assembled")
 
-    def step():
-        _lib.check(lib.tse_p_sweeps(hip.ctx, ctypes.byref(pf), dt, sor,
-                                    args.sweeps, 1), "tse_p_sweeps")
+    plan32 = fp.plan("mixed")
+    log(f"autotuned FP32 plan (ns, by, ry, zchunk) = {plan32}")
+
+    if args.profile_kernel:
+        fp.solve("mixed", max_sweeps=0, plan=plan32)
+        n = args.profile_kernel * plan32[0]
+        fp.sweeps(n, plan32, sor=args.omega, fp32=True)
+        src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)      # calibration: 1 GiB read + 1 GiB write
+        torch.cuda.synchronize()
+        print(json.dumps({"profile_kernel_launches": args.profile_kernel,
+                          "plan": plan32, "copy_bytes": 2 * src.numel() * 8}))
+        return 0
+
+    def step(form):
+        fp.assemble()
+        st = fp.solve(form, tol=args.tol, sor=args.omega)
+        fp.result_device()  # solution back into a split-layout device field
+        return st
 
     for _ in range(args.warmup):
-        step()
+        st = step("mixed")
+        log(f"warmup solve: {st.sweeps} sweeps, rel {st.rel_residual:.2e}, "
+            f"{st.seconds:.2f}s")
     torch.cuda.synchronize()
-    log("[bench] warmup done")
+    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step()
+        stats.append(step("mixed"))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    log(f"[bench] timed {args.steps} steps in {elapsed:.3f}s")
-    updates = float(N) * args.sweeps * args.steps
-    value = updates / elapsed
+    sweeps = sum(s.sweeps for s in stats)
+    conv = all(s.converged for s in stats)
+    value = float(N) * sweeps / elapsed
+    tts = elapsed / args.steps
+    log(f"timed {args.steps} mixed solves: {tts:.3f}s each, {sweeps // args.steps}"
+        f" sweeps, converged={conv}, {value:.3e} cell-updates/s")
+    P_mixed = fp.result()
 
-    # dominant kernel: half-sweeps only, HIP events on the library's stream
+    # dominant kernel: the FP32 fused sweep, HIP events on the library stream
     stream = torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
-    n_it = max(args.sweeps, 10)
+    ns = plan32[0]
+    launches = 40
+    fp.sweeps(ns, plan32, sor=args.omega, fp32=True)
     ev0.record(stream)
-    _lib.check(lib.tse_p_sweeps(hip.ctx, ctypes.byref(pf), dt, sor, n_it, 0),
-               "tse_p_sweeps")
+    fp.sweeps(launches * ns, plan32, sor=args.omega, fp32=True)
     ev1.record(stream)
     torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    per_launch_s = ms * 1e-3 / (2 * n_it)
-    bytes_per_launch = FAITHFUL_BYTES_PER_UPDATE * N / 2
-    achieved = bytes_per_launch / per_launch_s / 1e9
-    log(f"[bench] half-sweep {per_launch_s * 1e6:.1f} us, {achieved:.0f} GB/s")
+    launch_s = ev0.elapsed_time(ev1) * 1e-3 / launches
+    bytes_launch = BYTES_FP32_SWEEP * N * ns
+    achieved = bytes_launch / launch_s / 1e9
+    log(f"sweep kernel: {launch_s * 1e6:.1f} us/launch ({ns} sweeps), "
+        f"{achieved:.0f} GB/s algorithmic")
+
+    plain = None
+    if not args.no_compare:
+        st64 = step("plain64")
+        P_plain = fp.result()
+        change = float(np.max(np.abs(P_plain - prob.P)))
+        dev = float(np.max(np.abs(P_mixed - P_plain)))
+        plan64 = fp.plan("plain64")
+        plain = {"time_to_solution_s": st64.seconds, "sweeps": st64.sweeps,
+                 "converged": st64.converged, "rel_residual": st64.rel_residual,
+                 "plan": list(plan64),
+                 "speedup_mixed_vs_plain64": st64.seconds / tts,
+                 "max_abs_dev_mixed_vs_plain64_Pa": dev,
+                 "dev_rel_to_change": dev / change if change else None,
+                 "dev_rel_to_P": dev / float(np.max(np.abs(P_plain)))}
+        log(f"plain64 solve: {st64.seconds:.3f}s, {st64.sweeps} sweeps; speedup "
+            f"{st64.seconds / tts:.2f}x; max |P_mixed - P_plain| = {dev:.3e} Pa")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -122,14 +168,15 @@ def main(argv=None):
             for colour in (0, 1):
                 o.p_half_sweep(P, prob.P, prob.c_f, prob.nue, prob.rho_f,
                                prob.phi, prob.Kx, prob.Ky, prob.Kz, prob.BP,
-                               prob.WP, dt, sor, colour)
+                               prob.WP, dt, args.omega, colour)
         cpu_s = time.perf_counter() - c0
         cpu = {"value": N * args.cpu_sweeps / cpu_s, "unit": "cell-updates/s",
                "cores": threads, "kind": "port",
-               "sample": f"{args.cpu_sweeps} red+black sweep(s) of the full "
-                         f"{nx}x{ny}x{nz} grid, oracle C restatement "
-                         f"(OpenMP, {threads} threads), {cpu_s:.1f}s"}
-        log(f"[bench] cpu baseline {cpu['value']:.3e} cell-updates/s")
+               "sample": f"{args.cpu_sweeps} red+black SOR sweeps of the full "
+                         f"{nx}x{ny}x{nz} grid by the oracle (C restatement of "
+                         f"compute_P's sweep, OpenMP {threads} threads, FP64), "
+                         f"{cpu_s:.1f}s"}
+        log(f"cpu baseline {cpu['value']:.3e} cell-updates/s ({threads} threads)")
 
     line = {
         "metric": "SOR cell-updates/s",
@@ -138,23 +185,28 @@ def main(argv=None):
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": elapsed / args.steps * 1e3,
+        "ms_per_step": tts * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32 sweeps in f64 defect correction",
         "data": "synthetic",
-        "config": {"workload": f"pressure SOR {nx}x{ny}x{nz}, faithful FP64 "
-                               f"red-black sweeps ({args.sweeps} per step, "
-                               "omega 1.8, sigma_lnK 2.5)",
-                   "grid": [nx, ny, nz], "sweeps_per_step": args.sweeps,
-                   "form": "faithful64"},
+        "config": {"workload": f"pressure solve {nx}x{ny}x{nz}, 5 m cells, "
+                               "sigma_lnK 2.5, corr 16 cells, mixed precision "
+                               f"to rel. residual {args.tol:g}",
+                   "grid": [nx, ny, nz], "form": "mixed", "omega": args.omega,
+                   "plan_ns_by_ry_zchunk": list(plan32)},
+        "time_to_solution_s": tts,
+        "solves_per_s": 1.0 / tts,
+        "sweeps_per_solve": sweeps // args.steps,
+        "rel_residual": stats[-1].rel_residual,
+        "converged": conv,
+        "plain64": plain,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "kernel": "p_sweep_kernel",
-                     "bytes_per_update": FAITHFUL_BYTES_PER_UPDATE,
-                     "launch_us": per_launch_s * 1e6},
+                     "traffic": None, "kernel": "rb_zmarch_kernel<float>",
+                     "bytes_per_update": BYTES_FP32_SWEEP,
+                     "sweeps_per_launch": ns, "launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -177,11 +177,19 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *   TSE_FORM_PLAIN64: FP64 sweeps on p.
  *   TSE_FORM_MIXED:   FP32 sweeps on the correction A e = r, r = b - A p in
  *                     FP64, p += e in FP64 (defect correction). */
+/* Launch plan of the fused sweep kernel: ns sweeps per launch, tile of
+ * 64 x by columns with ry rows per thread (ry = 1: one row per thread, all
+ * neighbours through LDS; ry > 1: x neighbours by cross-lane shuffles),
+ * zchunk z planes per block (<= 0: whole column).  Supported (ns, by, ry):
+ * (1,8,1) (1,16,1) (2,16,1) (1,32,2) (2,16,2), and for the FP32 sweeps
+ * also (3,16,1) (2,32,2) (2,32,4) (1,48,3). */
+typedef struct {
+    int32_t ns, by, ry, zchunk;
+} tse_sweep_plan;
+
 typedef struct {
     int32_t form;       /* TSE_FORM_PLAIN64 or TSE_FORM_MIXED              */
-    int32_t ns;         /* sweeps fused per launch: 1, 2 (3: MIXED only)    */
-    int32_t by;         /* tile rows: 8 or 16 (ns > 1: 16)                  */
-    int32_t zchunk;     /* z planes per block (<= 0: whole column)          */
+    tse_sweep_plan plan;
     int32_t inner;      /* sweeps per residual check / correction cycle     */
     int32_t max_sweeps; /* stop after this many sweeps                      */
     double sor;         /* over-relaxation factor                           */
@@ -209,8 +217,8 @@ int tse_fast_p_residual(tse_ctx *ctx, double *rel);
 int tse_fast_p_result(tse_ctx *ctx, double *P);
 /* Timing hook: n_sweeps fused sweeps (FP64 on p, or fp32 != 0: FP32 on the
  * correction with the last residual), no residual checks. */
-int tse_fast_p_sweeps(tse_ctx *ctx, int32_t n_sweeps, double sor, int32_t ns,
-                      int32_t by, int32_t zchunk, int32_t fp32);
+int tse_fast_p_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
+                      int32_t n_sweeps, double sor, int32_t fp32);
 
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at

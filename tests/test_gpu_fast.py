@@ -19,7 +19,10 @@ from helpers import Fields
 
 pytestmark = pytest.mark.gpu
 
-PLANS = [(1, 8, 0), (1, 16, 5), (2, 16, 0), (2, 16, 6)]
+# (ns, by, ry, zchunk): every kernel variant, with and without z-chunks
+PLANS = [(1, 8, 1, 0), (1, 16, 1, 5), (2, 16, 1, 0), (2, 16, 1, 6),
+         (1, 32, 2, 0), (2, 16, 2, 7)]
+PLANS32 = [(2, 32, 2, 6), (2, 32, 4, 0), (1, 48, 3, 5), (3, 16, 1, 9)]
 
 
 def _setup(f, dt):
@@ -44,8 +47,8 @@ def _result(hip, d):
 
 
 @pytest.mark.parametrize("shape", [(70, 23, 19), (33, 20, 17), (9, 40, 12)])
-@pytest.mark.parametrize("ns,by,zchunk", PLANS)
-def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, zchunk):
+@pytest.mark.parametrize("ns,by,ry,zchunk", PLANS)
+def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, ry, zchunk):
     from oracle import Oracle
     from transportse_amd import _lib
     f = Fields(*shape, seed=21)
@@ -57,12 +60,14 @@ def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, zchunk):
             o.p_half_sweep(Po, f.P, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky,
                            f.Kz, f.BP, f.WP, dt, sor, colour)
     hip, d = _setup(f, dt)
-    _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, n, sor, ns, by, zchunk, 0),
+    plan = _lib.SweepPlan(ns=ns, by=by, ry=ry, zchunk=zchunk)
+    _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n, sor, 0),
                "tse_fast_p_sweeps")
     Ph = _result(hip, d)
     change = np.max(np.abs(Po - f.P))
     err = np.max(np.abs(Ph - Po))
-    print("fused", shape, ns, by, zchunk, "change", change, "err", err, flush=True)
+    print("fused", shape, ns, by, ry, zchunk, "change", change, "err", err,
+          flush=True)
     assert change > 0
     assert err <= 1e-10 * change
 
@@ -80,8 +85,9 @@ def test_solve_reaches_oracle_fixed_point(form, tol, bound):
                     inc_it=1.0, max_dt=dt, P_conv=1e-15)
     assert r[4]
     hip, d = _setup(f, dt)
-    prm = _lib.FastParams(form=form, ns=2 if form == 2 else 1, by=16, zchunk=0,
-                          inner=20, max_sweeps=200000, sor=1.9, tol=tol)
+    plan = _lib.SweepPlan(ns=2, by=16, ry=1, zchunk=0)
+    prm = _lib.FastParams(form=form, plan=plan, inner=20, max_sweeps=200000,
+                          sor=1.9, tol=tol)
     res = _lib.FastResult()
     _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
                                         ctypes.byref(res)), "tse_fast_p_solve")
@@ -93,3 +99,27 @@ def test_solve_reaches_oracle_fixed_point(form, tol, bound):
           "err/change", err / change, "err/|P|", err / np.max(np.abs(Po)),
           flush=True)
     assert err <= bound * change
+
+
+@pytest.mark.parametrize("ns,by,ry,zchunk", PLANS32 + PLANS)
+def test_fp32_plans_agree(ns, by, ry, zchunk):
+    """Every FP32 sweep plan gives the same mixed-precision solve (same
+    sweep count to the same residual, fields within FP32-cycle noise)."""
+    from transportse_amd import _lib
+    f = Fields(70, 37, 21, seed=23)
+    outs = []
+    for pl in ((1, 16, 1, 0), (ns, by, ry, zchunk)):
+        hip, d = _setup(f, 3600.0)
+        prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(*pl), inner=12,
+                              max_sweeps=12000, sor=1.9, tol=1e-10)
+        res = _lib.FastResult()
+        _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                            ctypes.byref(res)), "solve")
+        assert res.converged
+        outs.append((_result(hip, d), res.sweeps))
+    (a, na), (b, nb) = outs
+    change = np.max(np.abs(a - f.P))
+    print("fp32 plan", (ns, by, ry, zchunk), "sweeps", na, nb,
+          "diff/change", np.max(np.abs(a - b)) / change, flush=True)
+    assert abs(na - nb) <= 24
+    assert np.max(np.abs(a - b)) <= 1e-6 * change

@@ -43,9 +43,13 @@ T_FIELDS = ("T", "Tn", "l_f", "cp_f", "rho_f", "phi", "qx", "qy", "qz", "BT",
             "WT", "e", "l_r", "cp_r", "rho_r", "lt_l", "lt_t")
 
 
+class SweepPlan(ctypes.Structure):
+    _fields_ = [("ns", ctypes.c_int32), ("by", ctypes.c_int32),
+                ("ry", ctypes.c_int32), ("zchunk", ctypes.c_int32)]
+
+
 class FastParams(ctypes.Structure):
-    _fields_ = [("form", ctypes.c_int32), ("ns", ctypes.c_int32),
-                ("by", ctypes.c_int32), ("zchunk", ctypes.c_int32),
+    _fields_ = [("form", ctypes.c_int32), ("plan", SweepPlan),
                 ("inner", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
                 ("sor", ctypes.c_double), ("tol", ctypes.c_double)]
 
@@ -101,8 +105,8 @@ SIGNATURES = {
                               ctypes.POINTER(FastResult)]),
     "tse_fast_p_residual": (_I, [_V, _DPH]),
     "tse_fast_p_result": (_I, [_V, _dp]),
-    "tse_fast_p_sweeps": (_I, [_V, ctypes.c_int32, _D, ctypes.c_int32,
-                               ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
+    "tse_fast_p_sweeps": (_I, [_V, ctypes.POINTER(SweepPlan), ctypes.c_int32,
+                               _D, ctypes.c_int32]),
 }
 
 _lib = None

@@ -1,0 +1,58 @@
+"""Autotuner for the fused sweep kernel's launch plan.
+
+A plan is (ns sweeps per launch, tile rows by, rows per thread ry, z-chunk
+planes).  The tuner times every supported plan on the problem already set
+up in the context (``tse_fast_p_setup``) with HIP events on the library's
+stream, a few sweeps each after one warm-up launch, and returns the fastest.
+Sweeps are timed on a scratch copy semantics-free: the FP64 candidates
+advance the iterate (harmless: the solve continues from any iterate), the
+FP32 candidates run on the correction buffers.  Results are cached per
+(grid, precision).
+"""
+from __future__ import annotations
+
+import ctypes
+
+from . import _lib
+
+PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1), (1, 32, 2), (2, 16, 2)]
+PLANS32 = PLANS64 + [(3, 16, 1), (2, 32, 2), (2, 32, 4), (1, 48, 3)]
+ZCHUNKS = (0, 128, 64, 32)
+
+_cache: dict = {}
+
+
+def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
+         verbose: bool = False):
+    """Return ((ns, by, ry, zchunk), ms_per_sweep) of the fastest plan."""
+    import torch
+    key = (hip.shape, bool(fp32))
+    if key in _cache:
+        return _cache[key]
+    lib = hip.lib
+    best = None
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    stream = torch.cuda.current_stream()
+    nz = hip.shape[2]
+    for ns, by, ry in (PLANS32 if fp32 else PLANS64):
+        for zc in zchunks:
+            if zc and zc >= nz:
+                continue
+            n = ns * max(1, sweeps // ns)
+            plan = _lib.SweepPlan(ns, by, ry, zc)
+            _lib.check(lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), ns,
+                                             sor, int(fp32)), "warmup")
+            ev0.record(stream)
+            _lib.check(lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n,
+                                             sor, int(fp32)), "sweeps")
+            ev1.record(stream)
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / n
+            if verbose:
+                print(f"  plan ns={ns} by={by} ry={ry} zchunk={zc}: "
+                      f"{ms * 1e3:.1f} us/sweep", flush=True)
+            if best is None or ms < best[1]:
+                best = ((ns, by, ry, zc), ms)
+    _cache[key] = best
+    return best

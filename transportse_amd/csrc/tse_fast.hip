@@ -148,6 +148,143 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     }
 }
 
+// Variant with RY tile rows per thread (taller tiles, less halo) and the
+// in-plane x neighbours (unknown and the west face) taken from the adjacent
+// lanes by cross-lane shuffles: a block row of 64 threads is one wavefront.
+// Only the y neighbours go through LDS (unknown and the south face).
+template <typename T, int NS, int BYT, int RY>
+__global__ void __launch_bounds__(64 * BYT)
+rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                  const T *__restrict__ Wx, const T *__restrict__ Wy,
+                  const T *__restrict__ Wz, const T *__restrict__ invD,
+                  const T *__restrict__ rhs, T omega, int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int BY = BYT * RY;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int NSLOT = R + 1;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[NSLOT][BY][BX];
+    __shared__ T sWy[NSLOT][BY][BX];
+
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x;
+    const int i = tx * TX - R + lx;
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const long plane = (long)g.nx * g.ny;
+    const int dx = min(lx, BX - 1 - lx);
+
+    int ly[RY], d[RY], jj[RY];
+    bool in_dom[RY];
+    long col[RY];
+    T p[RY][R + 2], rinv[RY][R + 1], rb[RY][R + 1], rwz[RY][R + 2];
+    T rwx[RY][R + 1], rwy[RY][R + 1];
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        ly[q] = threadIdx.y + q * BYT;
+        jj[q] = ty * TY - R + ly[q];
+        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)jj[q] < (unsigned)g.ny;
+        d[q] = min(dx, min(ly[q], BY - 1 - ly[q]));
+        col[q] = in_dom[q] ? (long)i + (long)g.nx * jj[q] : 0;
+#pragma unroll
+        for (int r = 0; r < R + 2; ++r) {
+            p[q][r] = T(0);
+            rwz[q][r] = T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < R + 1; ++r) {
+            rinv[q][r] = T(0);
+            rb[q][r] = T(0);
+            rwx[q][r] = T(0);
+            rwy[q][r] = T(0);
+        }
+    }
+
+    for (int k = kl0; k <= k1 + R - 1; ++k) {
+        const int s0 = k % NSLOT;
+#pragma unroll
+        for (int q = 0; q < RY; ++q) {
+            T p0 = T(0), i0 = T(0), b0 = T(0), wz0 = T(0), wx0 = T(0), wy0 = T(0);
+            if (in_dom[q] && k <= kl1) {
+                const long o = col[q] + plane * k;
+                p0 = zero_init ? T(0) : Pin[o];
+                i0 = invD[o];
+                b0 = rhs[o];
+                wz0 = Wz[o];
+                wx0 = Wx[o];
+                wy0 = Wy[o];
+            }
+#pragma unroll
+            for (int r = R + 1; r > 0; --r) {
+                p[q][r] = p[q][r - 1];
+                rwz[q][r] = rwz[q][r - 1];
+            }
+#pragma unroll
+            for (int r = R; r > 0; --r) {
+                rinv[q][r] = rinv[q][r - 1];
+                rb[q][r] = rb[q][r - 1];
+                rwx[q][r] = rwx[q][r - 1];
+                rwy[q][r] = rwy[q][r - 1];
+            }
+            p[q][0] = p0;
+            rwz[q][0] = wz0;
+            rinv[q][0] = i0;
+            rb[q][0] = b0;
+            rwx[q][0] = wx0;
+            rwy[q][0] = wy0;
+        }
+        __syncthreads();  // slot s0 held plane k-NSLOT, last read at step k-1
+#pragma unroll
+        for (int q = 0; q < RY; ++q) {
+            sP[s0][ly[q]][lx] = p[q][0];
+            sWy[s0][ly[q]][lx] = rwy[q][0];
+        }
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+            __syncthreads();
+            const int m = k - 1 - h;
+            const int s = m % NSLOT;
+            const bool mok = m >= kl0 && m <= kl1;
+#pragma unroll
+            for (int q = 0; q < RY; ++q) {
+                const T pc = p[q][1 + h];
+                const T pe = __shfl_down(pc, 1, 64);
+                const T pw = __shfl_up(pc, 1, 64);
+                const T wxw = __shfl_up(rwx[q][1 + h], 1, 64);
+                if (mok && d[q] >= h + 1 && in_dom[q] &&
+                    ((i + jj[q] + m) & 1) == (h & 1) && rinv[q][1 + h] > T(0)) {
+                    const int y = ly[q];
+                    T acc = rb[q][1 + h];
+                    acc += rwx[q][1 + h] * pe;
+                    acc += wxw * pw;
+                    acc += rwy[q][1 + h] * sP[s][y + 1][lx];
+                    acc += sWy[s][y - 1][lx] * sP[s][y - 1][lx];
+                    acc += rwz[q][1 + h] * p[q][h];
+                    acc += rwz[q][2 + h] * p[q][h + 2];
+                    const T pn = pc + omega * (acc * rinv[q][1 + h] - pc);
+                    p[q][1 + h] = pn;
+                    sP[s][y][lx] = pn;
+                }
+            }
+        }
+        const int mo = k - R;
+        if (mo >= k0 && mo < k1) {
+#pragma unroll
+            for (int q = 0; q < RY; ++q)
+                if (d[q] >= R && in_dom[q]) Pout[col[q] + plane * mo] = p[q][R];
+        }
+    }
+}
+
 // ---- operator assembly (from split-layout fields) ---------------------------------------
 
 __device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
@@ -484,11 +621,30 @@ static int residual(tse_ctx *ctx, FastState *s, const double *pin, const float *
 }
 
 template <typename T>
-static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
+static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by, int ry,
                          const T *Pin, T *Pout, const T *Wx, const T *Wy,
                          const T *Wz, const T *invD, const T *rhs, T omega,
                          int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
+#define TSE_ZM2(NS_, BYT_, RY_)                                                   \
+    if (ns == NS_ && by == BYT_ * RY_ && ry == RY_) {                             \
+        hipLaunchKernelGGL((rb_zmarch2_kernel<T, NS_, BYT_, RY_>), grid,          \
+                           dim3(64, BYT_), 0, st, zg, Pin, Pout, Wx, Wy, Wz,       \
+                           invD, rhs, omega, zero_init);                          \
+        TSE_HIP_TRY(hipGetLastError());                                           \
+        return 0;                                                                 \
+    }
+    if (ry > 1) {
+        TSE_ZM2(1, 16, 2)
+        TSE_ZM2(2, 8, 2)
+        if constexpr (sizeof(T) == 4) {
+            TSE_ZM2(2, 16, 2)
+            TSE_ZM2(2, 8, 4)
+            TSE_ZM2(1, 16, 3)
+        }
+        return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry)");
+    }
+#undef TSE_ZM2
 #define TSE_ZM(NS_, BY_)                                                        \
     if (ns == NS_ && by == BY_) {                                               \
         hipLaunchKernelGGL((rb_zmarch_kernel<T, NS_, BY_>), grid, dim3(64, BY_), \
@@ -507,9 +663,15 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by)");
 }
 
-bool fast_plan_ok(int ns, int by, int fp32) {
-    return (ns == 1 && (by == 8 || by == 16)) || (ns == 2 && by == 16) ||
-           (fp32 && ns == 3 && by == 16);
+bool fast_plan_ok(const tse_sweep_plan &pl, int fp32) {
+    const int ns = pl.ns, by = pl.by, ry = pl.ry;
+    if (ry <= 1)
+        return (ns == 1 && (by == 8 || by == 16)) || (ns == 2 && by == 16) ||
+               (fp32 && ns == 3 && by == 16);
+    if ((ns == 1 && by == 32 && ry == 2) || (ns == 2 && by == 16 && ry == 2))
+        return true;
+    return fp32 && ((ns == 2 && by == 32 && ry == 2) || (ns == 2 && by == 32 && ry == 4) ||
+                    (ns == 1 && by == 48 && ry == 3));
 }
 
 ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
@@ -531,11 +693,12 @@ ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
 
 // n sweeps on the iterate (plain) or on the correction e (mixed, e starts
 // at 0 when zero_first); keeps the ping-pong index.
-static int plain_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg, int ns,
-                        int by, int n, double sor) {
+static int plain_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg,
+                        const tse_sweep_plan &pl, int n, double sor) {
     FastOp op = fast_op(s);
+    const int ns = pl.ns;
     for (int q = 0; q < n / ns; ++q) {
-        int rc = launch_zmarch<double>(ctx->stream, zg, ns, by, pbuf(s, s->cur),
+        int rc = launch_zmarch<double>(ctx->stream, zg, ns, pl.by, pl.ry, pbuf(s, s->cur),
                                        pbuf(s, 1 - s->cur), op.Wx, op.Wy, op.Wz,
                                        op.invD, op.b, sor, 0);
         if (rc) return rc;
@@ -544,12 +707,14 @@ static int plain_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg, int ns,
     return 0;
 }
 
-static int correction_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg, int ns,
-                             int by, int n, float sor, int *ecur) {
+static int correction_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg,
+                             const tse_sweep_plan &pl, int n, float sor,
+                             int *ecur) {
     FastOp op = fast_op(s);
+    const int ns = pl.ns;
     int e = 0;
     for (int q = 0; q < n / ns; ++q) {
-        int rc = launch_zmarch<float>(ctx->stream, zg, ns, by, ebuf(s, e),
+        int rc = launch_zmarch<float>(ctx->stream, zg, ns, pl.by, pl.ry, ebuf(s, e),
                                       ebuf(s, 1 - e), op.Wx32, op.Wy32, op.Wz32,
                                       op.invD32, r32buf(s), sor, q == 0);
         if (rc) return rc;
@@ -616,19 +781,19 @@ int tse_fast_p_residual(tse_ctx *ctx, double *rel) {
     return 0;
 }
 
-int tse_fast_p_sweeps(tse_ctx *ctx, int32_t n_sweeps, double sor, int32_t ns,
-                      int32_t by, int32_t zchunk, int32_t fp32) {
+int tse_fast_p_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
+                      int32_t n_sweeps, double sor, int32_t fp32) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
     FastState *s = state(ctx);
-    if (!s->ready) return fail(TSE_ERR_STATE, "fast form not set up");
-    if (!fast_plan_ok(ns, by, fp32) || n_sweeps < 0 || n_sweeps % ns)
+    if (!s->ready || !plan) return fail(TSE_ERR_STATE, "fast form not set up");
+    if (!fast_plan_ok(*plan, fp32) || n_sweeps < 0 || n_sweeps % plan->ns)
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
-    const ZGeo zg = make_zgeo(ctx->geo, ns, by, zchunk);
-    if (!fp32) return plain_sweeps(ctx, s, zg, ns, by, n_sweeps, sor);
+    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk);
+    if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
     // FP32 timing path: sweeps on the correction equation with the last r
     int e;
-    return correction_sweeps(ctx, s, zg, ns, by, n_sweeps, (float)sor, &e);
+    return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
 }
 
 int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
@@ -640,11 +805,12 @@ int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
     const int fp32 = prm->form == TSE_FORM_MIXED;
     if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
         return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
-    const int ns = prm->ns, by = prm->by;
+    const tse_sweep_plan &pl = prm->plan;
+    const int ns = pl.ns;
     const int chunk = prm->inner;
-    if (!fast_plan_ok(ns, by, fp32) || chunk < ns || chunk % ns || prm->max_sweeps < 0)
+    if (!fast_plan_ok(pl, fp32) || chunk < ns || chunk % ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
-    const ZGeo zg = make_zgeo(ctx->geo, ns, by, prm->zchunk);
+    const ZGeo zg = make_zgeo(ctx->geo, ns, pl.by, pl.zchunk);
     double rn, bn;
     rc = fp32 ? residual<2>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn)
               : residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
@@ -656,14 +822,14 @@ int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
     while (rel > prm->tol && sweeps < prm->max_sweeps) {
         if (fp32) {
             int e;
-            rc = correction_sweeps(ctx, s, zg, ns, by, chunk, (float)prm->sor, &e);
+            rc = correction_sweeps(ctx, s, zg, pl, chunk, (float)prm->sor, &e);
             if (rc) return rc;
             rc = residual<1>(ctx, s, pbuf(s, s->cur), ebuf(s, e),
                              pbuf(s, 1 - s->cur), &rn, &bn);
             if (rc) return rc;
             s->cur = 1 - s->cur;
         } else {
-            rc = plain_sweeps(ctx, s, zg, ns, by, chunk, prm->sor);
+            rc = plain_sweeps(ctx, s, zg, pl, chunk, prm->sor);
             if (rc) return rc;
             rc = residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
             if (rc) return rc;

@@ -1,0 +1,121 @@
+"""Performance forms of the pressure solve (plain FP64, mixed FP32-in-FP64).
+
+``FastPressure`` drives the C-ABI ``tse_fast_p_*`` entry points for one grid:
+it uploads compute_P's inputs (reference layout, as tse.py passes them),
+assembles the volume-scaled symmetric operator, solves to a relative
+residual ||b - A p|| / ||b|| with fused red/black sweeps, and returns P.
+
+Forms:
+  * ``"plain64"`` -- FP64 sweeps on P.
+  * ``"mixed"``   -- FP32 sweeps on the correction A e = r, with r = b - A p,
+    the update p += e and the convergence norm in FP64 (defect correction).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from .device import HipContext
+
+FORMS = {"plain64": _lib.FORM_PLAIN64, "mixed": _lib.FORM_MIXED}
+P_INPUTS = ("c_f", "nue", "rho_f", "phi", "Kx", "Ky", "Kz", "BP", "WP", "c_r")
+
+
+@dataclass
+class SolveStats:
+    form: str
+    sweeps: int
+    cycles: int
+    converged: bool
+    initial_rel: float
+    rel_residual: float
+    seconds: float
+    plan: tuple
+
+
+class FastPressure:
+    """Fast-form pressure solver bound to one grid on one device."""
+
+    def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, device=0,
+                 hip: HipContext | None = None):
+        self.hip = hip or HipContext(nx, ny, nz, dx, dy, dz, g, beta_deg, device)
+        self.lib = self.hip.lib
+        self.N = self.hip.nx * self.hip.ny * self.hip.nz
+        self.d = {}
+        self.plans = {}
+
+    def load(self, P, dt, Pn=None, **fields):
+        """Upload P, Pn and the ten compute_P inputs and assemble for dt."""
+        for n in P_INPUTS:
+            self.d[n] = self.hip.upload(fields[n], out=self.d.get(n))
+        self.d["P"] = self.hip.upload(P, out=self.d.get("P"))
+        self.d["Pn"] = self.hip.upload(P if Pn is None else Pn, out=self.d.get("Pn"))
+        self.dt = float(dt)
+        self.assemble()
+
+    def assemble(self):
+        """(Re)assemble the operator and load P as the iterate."""
+        self._pf = _lib.PFields(**{n: self.d[n].data_ptr() for n in _lib.P_FIELDS})
+        _lib.check(self.lib.tse_fast_p_setup(self.hip.ctx, ctypes.byref(self._pf),
+                                             self.dt), "tse_fast_p_setup")
+
+    def plan(self, form: str, tune: bool = True):
+        """Launch plan for the form's sweep precision (autotuned once)."""
+        if form not in self.plans:
+            if tune:
+                from .autotune import tune as _tune
+                if form == "mixed":
+                    # the FP32 timing path needs a residual in place
+                    self.solve("mixed", max_sweeps=0, plan=(1, 16, 1, 0))
+                (ns, by, ry, zc), _ = _tune(self.hip, form == "mixed")
+                self.plans[form] = (ns, by, ry, zc)
+                self.assemble()  # restore the iterate the tuning advanced
+            else:
+                self.plans[form] = (2, 16, 1, 64)
+        return self.plans[form]
+
+    def solve(self, form="mixed", tol=1e-10, sor=1.996, max_sweeps=200000,
+              inner=None, plan=None) -> SolveStats:
+        pl = plan or self.plan(form)
+        ns = pl[0]
+        inner = inner or (100 if form == "mixed" else 100)
+        inner = max(ns, (inner // ns) * ns)
+        prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*pl),
+                              inner=inner, max_sweeps=int(max_sweeps),
+                              sor=float(sor), tol=float(tol))
+        res = _lib.FastResult()
+        t0 = time.perf_counter()
+        _lib.check(self.lib.tse_fast_p_solve(self.hip.ctx, ctypes.byref(prm),
+                                             ctypes.byref(res)), "tse_fast_p_solve")
+        sec = time.perf_counter() - t0
+        return SolveStats(form, res.sweeps, res.cycles, bool(res.converged),
+                          res.initial_rel, res.rel_residual, sec, tuple(pl))
+
+    def result_device(self):
+        """Write the current iterate into a split-layout device field (kept
+        apart from the input P, which ``assemble`` reloads)."""
+        if "Pres" not in self.d:
+            self.d["Pres"] = self.hip.empty()
+        _lib.check(self.lib.tse_fast_p_result(self.hip.ctx, self.d["Pres"].data_ptr()),
+                   "tse_fast_p_result")
+        return self.d["Pres"]
+
+    def result(self, out: np.ndarray | None = None) -> np.ndarray:
+        return self.hip.download(self.result_device(), out=out)
+
+    def residual(self) -> float:
+        rel = ctypes.c_double()
+        _lib.check(self.lib.tse_fast_p_residual(self.hip.ctx, ctypes.byref(rel)),
+                   "tse_fast_p_residual")
+        return rel.value
+
+    def sweeps(self, n, plan, sor=1.996, fp32=True):
+        """Timing hook: n fused sweeps without residual checks."""
+        _lib.check(self.lib.tse_fast_p_sweeps(self.hip.ctx,
+                                              ctypes.byref(_lib.SweepPlan(*plan)),
+                                              int(n), float(sor), int(fp32)),
+                   "tse_fast_p_sweeps")
