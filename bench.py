@@ -55,6 +55,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=8)
     ap.add_argument("--profile-kernel", type=int, default=0)
+    ap.add_argument("--plan", type=int, nargs=4, default=None,
+                    metavar=("NS", "BY", "RY", "ZCHUNK"),
+                    help="FP32 sweep plan instead of autotuning")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
@@ -80,8 +83,11 @@ def main(argv=None):
     torch.cuda.synchronize()
     log("uploaded and assembled")
 
+    if args.plan:
+        fp.plans["mixed"] = tuple(args.plan)
     plan32 = fp.plan("mixed")
-    log(f"autotuned FP32 plan (ns, by, ry, zchunk) = {plan32}")
+    log(f"FP32 plan (ns, by, ry, zchunk) = {plan32}"
+        + ("" if args.plan else " (autotuned)"))
 
     if args.profile_kernel:
         fp.solve("mixed", max_sweeps=0, plan=plan32)

@@ -57,7 +57,10 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     constexpr int BX = 64;
     constexpr int TX = BX - 2 * R;
     constexpr int TY = BY - 2 * R;
-    constexpr int NSLOT = R + 1;
+    // LDS ring of z-planes: the plane stored at step k (k) and the R planes
+    // in phases (k-1 .. k-R), plus one spare so the slot overwritten at step
+    // k was last read at step k-2.
+    constexpr int NSLOT = R + 2;
     static_assert(TY >= 1, "tile too small");
     __shared__ T sP[NSLOT][BY][BX];
     __shared__ T sWx[NSLOT][BY][BX];
@@ -80,6 +83,7 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     const int kl1 = min(k1 + R, g.nz) - 1;
     const long plane = (long)g.nx * g.ny;
     const long col = in_dom ? (long)i + (long)g.nx * j : 0;
+    const int par = (i + j) & 1;
 
     T p[R + 2], rinv[R + 1], rb[R + 1], rwz[R + 2];
 #pragma unroll
@@ -92,17 +96,30 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
         rinv[q] = T(0);
         rb[q] = T(0);
     }
+    // operands of the next plane, loaded one step ahead
+    T np = T(0), ni = T(0), nb = T(0), nwz = T(0), nwx = T(0), nwy = T(0);
+    if (in_dom && kl0 <= kl1) {
+        const long o = col + plane * kl0;
+        np = zero_init ? T(0) : Pin[o];
+        ni = invD[o];
+        nb = rhs[o];
+        nwz = Wz[o];
+        nwx = Wx[o];
+        nwy = Wy[o];
+    }
 
+    int s0 = kl0 % NSLOT;
     for (int k = kl0; k <= k1 + R - 1; ++k) {
-        T p0 = T(0), i0 = T(0), b0 = T(0), wz0 = T(0), wx0 = T(0), wy0 = T(0);
-        if (in_dom && k <= kl1) {
-            const long o = col + plane * k;
-            p0 = zero_init ? T(0) : Pin[o];
-            i0 = invD[o];
-            b0 = rhs[o];
-            wz0 = Wz[o];
-            wx0 = Wx[o];
-            wy0 = Wy[o];
+        const T p0 = np, i0 = ni, b0 = nb, wz0 = nwz, wx0 = nwx, wy0 = nwy;
+        np = ni = nb = nwz = nwx = nwy = T(0);
+        if (in_dom && k + 1 <= kl1) {
+            const long o = col + plane * (k + 1);
+            np = zero_init ? T(0) : Pin[o];
+            ni = invD[o];
+            nb = rhs[o];
+            nwz = Wz[o];
+            nwx = Wx[o];
+            nwy = Wy[o];
         }
 #pragma unroll
         for (int q = R + 1; q > 0; --q) {
@@ -118,18 +135,21 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
         rwz[0] = wz0;
         rinv[0] = i0;
         rb[0] = b0;
-        const int s0 = k % NSLOT;
-        __syncthreads();  // slot s0 held plane k-NSLOT, last read at step k-1
         sP[s0][ly][lx] = p0;
         sWx[s0][ly][lx] = wx0;
         sWy[s0][ly][lx] = wy0;
+        // One barrier per step: phase h reads in-plane values of plane
+        // k-1-h written by phase h-1 during step k-1 (or stored at step
+        // k-1 for h = 0) and writes only its own colour; its z neighbours
+        // are this thread's registers.
+        __syncthreads();
 #pragma unroll
         for (int h = 0; h < R; ++h) {
-            __syncthreads();
             const int m = k - 1 - h;
+            int s = s0 - 1 - h;
+            if (s < 0) s += NSLOT;
             if (m >= kl0 && m <= kl1 && d >= h + 1 && in_dom &&
-                ((i + j + m) & 1) == (h & 1) && rinv[1 + h] > T(0)) {
-                const int s = m % NSLOT;
+                ((par + m) & 1) == (h & 1) && rinv[1 + h] > T(0)) {
                 T acc = rb[1 + h];
                 acc += sWx[s][ly][lx] * sP[s][ly][lx + 1];
                 acc += sWx[s][ly][lx - 1] * sP[s][ly][lx - 1];
@@ -145,6 +165,7 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
         }
         const int mo = k - R;
         if (mo >= k0 && mo < k1 && d >= R && in_dom) Pout[col + plane * mo] = p[R];
+        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
     }
 }
 
