@@ -178,11 +178,10 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *   TSE_FORM_MIXED:   FP32 sweeps on the correction A e = r, r = b - A p in
  *                     FP64, p += e in FP64 (defect correction). */
 /* Launch plan of the fused sweep kernel: ns sweeps per launch, tile of
- * 64 x by columns with ry rows per thread (ry = 1: one row per thread, all
- * neighbours through LDS; ry > 1: x neighbours by cross-lane shuffles),
- * zchunk z planes per block (<= 0: whole column).  Supported (ns, by, ry):
- * (1,8,1) (1,16,1) (2,16,1) (1,32,2) (2,16,2), and for the FP32 sweeps
- * also (3,16,1) (2,32,2) (2,32,4) (1,48,3). */
+ * 64 x by columns with ry tile rows per thread (blocks of 64 x by/ry
+ * threads), zchunk z planes per block (<= 0: whole column).  Supported
+ * (ns, by, ry): (1,8,1) (1,16,1) (2,16,1) (1,24,2), and for the FP32
+ * sweeps also (1,32,2) (2,32,2) (2,32,4) (3,16,1). */
 typedef struct {
     int32_t ns, by, ry, zchunk;
 } tse_sweep_plan;

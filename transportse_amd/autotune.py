@@ -15,8 +15,8 @@ import ctypes
 
 from . import _lib
 
-PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1), (1, 32, 2), (2, 16, 2)]
-PLANS32 = PLANS64 + [(3, 16, 1), (2, 32, 2), (2, 32, 4), (1, 48, 3)]
+PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1), (1, 24, 2)]
+PLANS32 = PLANS64 + [(1, 32, 2), (2, 32, 2), (2, 32, 4), (3, 16, 1)]
 ZCHUNKS = (0, 128, 64, 32)
 
 _cache: dict = {}

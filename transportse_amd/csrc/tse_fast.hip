@@ -47,19 +47,20 @@ __device__ inline int xcd_tile(const ZGeo &g) {
     return (b & 7) * g.per_xcd + (b >> 3);
 }
 
-template <typename T, int NS, int BY>
-__global__ void __launch_bounds__(64 * BY)
+template <typename T, int NS, int BYT, int RY>
+__global__ void __launch_bounds__(64 * BYT)
 rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                  const T *__restrict__ Wx, const T *__restrict__ Wy,
                  const T *__restrict__ Wz, const T *__restrict__ invD,
                  const T *__restrict__ rhs, T omega, int zero_init) {
     constexpr int R = 2 * NS;          // halo ring width / number of phases
     constexpr int BX = 64;
+    constexpr int BY = BYT * RY;       // tile rows; thread rows ty + q*BYT
     constexpr int TX = BX - 2 * R;
     constexpr int TY = BY - 2 * R;
-    // LDS ring of z-planes: the plane stored at step k (k) and the R planes
-    // in phases (k-1 .. k-R), plus one spare so the slot overwritten at step
-    // k was last read at step k-2.
+    // LDS ring of z-planes: the plane stored at step k and the R planes in
+    // phases (k-1 .. k-R), plus one spare so the slot overwritten at step k
+    // was last read at step k-2.
     constexpr int NSLOT = R + 2;
     static_assert(TY >= 1, "tile too small");
     __shared__ T sP[NSLOT][BY][BX];
@@ -72,129 +73,6 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     const int tx = tile - zc * g.tiles_x;
     const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);    // / tiles_y
     const int ty = zc - zc2 * g.tiles_y;
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int i = tx * TX - R + lx;
-    const int j = ty * TY - R + ly;
-    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
-    const int k0 = zc2 * g.zchunk;
-    const int k1 = min(k0 + g.zchunk, g.nz);
-    const int kl0 = max(k0 - R, 0);
-    const int kl1 = min(k1 + R, g.nz) - 1;
-    const long plane = (long)g.nx * g.ny;
-    const long col = in_dom ? (long)i + (long)g.nx * j : 0;
-    const int par = (i + j) & 1;
-
-    T p[R + 2], rinv[R + 1], rb[R + 1], rwz[R + 2];
-#pragma unroll
-    for (int q = 0; q < R + 2; ++q) {
-        p[q] = T(0);
-        rwz[q] = T(0);
-    }
-#pragma unroll
-    for (int q = 0; q < R + 1; ++q) {
-        rinv[q] = T(0);
-        rb[q] = T(0);
-    }
-    // operands of the next plane, loaded one step ahead
-    T np = T(0), ni = T(0), nb = T(0), nwz = T(0), nwx = T(0), nwy = T(0);
-    if (in_dom && kl0 <= kl1) {
-        const long o = col + plane * kl0;
-        np = zero_init ? T(0) : Pin[o];
-        ni = invD[o];
-        nb = rhs[o];
-        nwz = Wz[o];
-        nwx = Wx[o];
-        nwy = Wy[o];
-    }
-
-    int s0 = kl0 % NSLOT;
-    for (int k = kl0; k <= k1 + R - 1; ++k) {
-        const T p0 = np, i0 = ni, b0 = nb, wz0 = nwz, wx0 = nwx, wy0 = nwy;
-        np = ni = nb = nwz = nwx = nwy = T(0);
-        if (in_dom && k + 1 <= kl1) {
-            const long o = col + plane * (k + 1);
-            np = zero_init ? T(0) : Pin[o];
-            ni = invD[o];
-            nb = rhs[o];
-            nwz = Wz[o];
-            nwx = Wx[o];
-            nwy = Wy[o];
-        }
-#pragma unroll
-        for (int q = R + 1; q > 0; --q) {
-            p[q] = p[q - 1];
-            rwz[q] = rwz[q - 1];
-        }
-#pragma unroll
-        for (int q = R; q > 0; --q) {
-            rinv[q] = rinv[q - 1];
-            rb[q] = rb[q - 1];
-        }
-        p[0] = p0;
-        rwz[0] = wz0;
-        rinv[0] = i0;
-        rb[0] = b0;
-        sP[s0][ly][lx] = p0;
-        sWx[s0][ly][lx] = wx0;
-        sWy[s0][ly][lx] = wy0;
-        // One barrier per step: phase h reads in-plane values of plane
-        // k-1-h written by phase h-1 during step k-1 (or stored at step
-        // k-1 for h = 0) and writes only its own colour; its z neighbours
-        // are this thread's registers.
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < R; ++h) {
-            const int m = k - 1 - h;
-            int s = s0 - 1 - h;
-            if (s < 0) s += NSLOT;
-            if (m >= kl0 && m <= kl1 && d >= h + 1 && in_dom &&
-                ((par + m) & 1) == (h & 1) && rinv[1 + h] > T(0)) {
-                T acc = rb[1 + h];
-                acc += sWx[s][ly][lx] * sP[s][ly][lx + 1];
-                acc += sWx[s][ly][lx - 1] * sP[s][ly][lx - 1];
-                acc += sWy[s][ly][lx] * sP[s][ly + 1][lx];
-                acc += sWy[s][ly - 1][lx] * sP[s][ly - 1][lx];
-                acc += rwz[1 + h] * p[h];
-                acc += rwz[2 + h] * p[h + 2];
-                const T pc = p[1 + h];
-                const T pn = pc + omega * (acc * rinv[1 + h] - pc);
-                p[1 + h] = pn;
-                sP[s][ly][lx] = pn;
-            }
-        }
-        const int mo = k - R;
-        if (mo >= k0 && mo < k1 && d >= R && in_dom) Pout[col + plane * mo] = p[R];
-        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
-    }
-}
-
-// Variant with RY tile rows per thread (taller tiles, less halo) and the
-// in-plane x neighbours (unknown and the west face) taken from the adjacent
-// lanes by cross-lane shuffles: a block row of 64 threads is one wavefront.
-// Only the y neighbours go through LDS (unknown and the south face).
-template <typename T, int NS, int BYT, int RY>
-__global__ void __launch_bounds__(64 * BYT)
-rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
-                  const T *__restrict__ Wx, const T *__restrict__ Wy,
-                  const T *__restrict__ Wz, const T *__restrict__ invD,
-                  const T *__restrict__ rhs, T omega, int zero_init) {
-    constexpr int R = 2 * NS;
-    constexpr int BX = 64;
-    constexpr int BY = BYT * RY;
-    constexpr int TX = BX - 2 * R;
-    constexpr int TY = BY - 2 * R;
-    constexpr int NSLOT = R + 1;
-    static_assert(TY >= 1, "tile too small");
-    __shared__ T sP[NSLOT][BY][BX];
-    __shared__ T sWy[NSLOT][BY][BX];
-
-    const int tile = xcd_tile(g);
-    if (tile >= g.ntiles) return;
-    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
-    const int tx = tile - zc * g.tiles_x;
-    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
-    const int ty = zc - zc2 * g.tiles_y;
     const int lx = threadIdx.x;
     const int i = tx * TX - R + lx;
     const int k0 = zc2 * g.zchunk;
@@ -202,20 +80,21 @@ rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     const int kl0 = max(k0 - R, 0);
     const int kl1 = min(k1 + R, g.nz) - 1;
     const long plane = (long)g.nx * g.ny;
-    const int dx = min(lx, BX - 1 - lx);
+    const int dxe = min(lx, BX - 1 - lx);
 
-    int ly[RY], d[RY], jj[RY];
+    int ly[RY], d[RY], par[RY];
     bool in_dom[RY];
     long col[RY];
     T p[RY][R + 2], rinv[RY][R + 1], rb[RY][R + 1], rwz[RY][R + 2];
-    T rwx[RY][R + 1], rwy[RY][R + 1];
+    T np[RY], ni[RY], nb[RY], nwz[RY], nwx[RY], nwy[RY];
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
         ly[q] = threadIdx.y + q * BYT;
-        jj[q] = ty * TY - R + ly[q];
-        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)jj[q] < (unsigned)g.ny;
-        d[q] = min(dx, min(ly[q], BY - 1 - ly[q]));
-        col[q] = in_dom[q] ? (long)i + (long)g.nx * jj[q] : 0;
+        const int j = ty * TY - R + ly[q];
+        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+        d[q] = min(dxe, min(ly[q], BY - 1 - ly[q]));
+        col[q] = in_dom[q] ? (long)i + (long)g.nx * j : 0;
+        par[q] = (i + j) & 1;
 #pragma unroll
         for (int r = 0; r < R + 2; ++r) {
             p[q][r] = T(0);
@@ -225,24 +104,35 @@ rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
         for (int r = 0; r < R + 1; ++r) {
             rinv[q][r] = T(0);
             rb[q][r] = T(0);
-            rwx[q][r] = T(0);
-            rwy[q][r] = T(0);
+        }
+        // operands of the next plane, loaded one step ahead
+        np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
+        if (in_dom[q] && kl0 <= kl1) {
+            const long o = col[q] + plane * kl0;
+            np[q] = zero_init ? T(0) : Pin[o];
+            ni[q] = invD[o];
+            nb[q] = rhs[o];
+            nwz[q] = Wz[o];
+            nwx[q] = Wx[o];
+            nwy[q] = Wy[o];
         }
     }
 
+    int s0 = kl0 % NSLOT;
     for (int k = kl0; k <= k1 + R - 1; ++k) {
-        const int s0 = k % NSLOT;
 #pragma unroll
         for (int q = 0; q < RY; ++q) {
-            T p0 = T(0), i0 = T(0), b0 = T(0), wz0 = T(0), wx0 = T(0), wy0 = T(0);
-            if (in_dom[q] && k <= kl1) {
-                const long o = col[q] + plane * k;
-                p0 = zero_init ? T(0) : Pin[o];
-                i0 = invD[o];
-                b0 = rhs[o];
-                wz0 = Wz[o];
-                wx0 = Wx[o];
-                wy0 = Wy[o];
+            const T p0 = np[q], i0 = ni[q], b0 = nb[q], wz0 = nwz[q];
+            const T wx0 = nwx[q], wy0 = nwy[q];
+            np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
+            if (in_dom[q] && k + 1 <= kl1) {
+                const long o = col[q] + plane * (k + 1);
+                np[q] = zero_init ? T(0) : Pin[o];
+                ni[q] = invD[o];
+                nb[q] = rhs[o];
+                nwz[q] = Wz[o];
+                nwx[q] = Wx[o];
+                nwy[q] = Wy[o];
             }
 #pragma unroll
             for (int r = R + 1; r > 0; --r) {
@@ -253,44 +143,39 @@ rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
             for (int r = R; r > 0; --r) {
                 rinv[q][r] = rinv[q][r - 1];
                 rb[q][r] = rb[q][r - 1];
-                rwx[q][r] = rwx[q][r - 1];
-                rwy[q][r] = rwy[q][r - 1];
             }
             p[q][0] = p0;
             rwz[q][0] = wz0;
             rinv[q][0] = i0;
             rb[q][0] = b0;
-            rwx[q][0] = wx0;
-            rwy[q][0] = wy0;
+            sP[s0][ly[q]][lx] = p0;
+            sWx[s0][ly[q]][lx] = wx0;
+            sWy[s0][ly[q]][lx] = wy0;
         }
-        __syncthreads();  // slot s0 held plane k-NSLOT, last read at step k-1
-#pragma unroll
-        for (int q = 0; q < RY; ++q) {
-            sP[s0][ly[q]][lx] = p[q][0];
-            sWy[s0][ly[q]][lx] = rwy[q][0];
-        }
+        // One barrier per step: phase h reads in-plane values of plane
+        // k-1-h written by phase h-1 during step k-1 (or stored at step
+        // k-1 for h = 0) and writes only its own colour; its z neighbours
+        // are this thread's registers.
+        __syncthreads();
 #pragma unroll
         for (int h = 0; h < R; ++h) {
-            __syncthreads();
             const int m = k - 1 - h;
-            const int s = m % NSLOT;
+            int s = s0 - 1 - h;
+            if (s < 0) s += NSLOT;
             const bool mok = m >= kl0 && m <= kl1;
 #pragma unroll
             for (int q = 0; q < RY; ++q) {
-                const T pc = p[q][1 + h];
-                const T pe = __shfl_down(pc, 1, 64);
-                const T pw = __shfl_up(pc, 1, 64);
-                const T wxw = __shfl_up(rwx[q][1 + h], 1, 64);
                 if (mok && d[q] >= h + 1 && in_dom[q] &&
-                    ((i + jj[q] + m) & 1) == (h & 1) && rinv[q][1 + h] > T(0)) {
+                    ((par[q] + m) & 1) == (h & 1) && rinv[q][1 + h] > T(0)) {
                     const int y = ly[q];
                     T acc = rb[q][1 + h];
-                    acc += rwx[q][1 + h] * pe;
-                    acc += wxw * pw;
-                    acc += rwy[q][1 + h] * sP[s][y + 1][lx];
+                    acc += sWx[s][y][lx] * sP[s][y][lx + 1];
+                    acc += sWx[s][y][lx - 1] * sP[s][y][lx - 1];
+                    acc += sWy[s][y][lx] * sP[s][y + 1][lx];
                     acc += sWy[s][y - 1][lx] * sP[s][y - 1][lx];
                     acc += rwz[q][1 + h] * p[q][h];
                     acc += rwz[q][2 + h] * p[q][h + 2];
+                    const T pc = p[q][1 + h];
                     const T pn = pc + omega * (acc * rinv[q][1 + h] - pc);
                     p[q][1 + h] = pn;
                     sP[s][y][lx] = pn;
@@ -303,6 +188,7 @@ rb_zmarch2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
             for (int q = 0; q < RY; ++q)
                 if (d[q] >= R && in_dom[q]) Pout[col[q] + plane * mo] = p[q][R];
         }
+        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
     }
 }
 
@@ -647,52 +533,38 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by, int ry,
                          const T *Wz, const T *invD, const T *rhs, T omega,
                          int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
-#define TSE_ZM2(NS_, BYT_, RY_)                                                   \
+#define TSE_ZM(NS_, BYT_, RY_)                                                    \
     if (ns == NS_ && by == BYT_ * RY_ && ry == RY_) {                             \
-        hipLaunchKernelGGL((rb_zmarch2_kernel<T, NS_, BYT_, RY_>), grid,          \
+        hipLaunchKernelGGL((rb_zmarch_kernel<T, NS_, BYT_, RY_>), grid,           \
                            dim3(64, BYT_), 0, st, zg, Pin, Pout, Wx, Wy, Wz,       \
                            invD, rhs, omega, zero_init);                          \
         TSE_HIP_TRY(hipGetLastError());                                           \
         return 0;                                                                 \
     }
-    if (ry > 1) {
-        TSE_ZM2(1, 16, 2)
-        TSE_ZM2(2, 8, 2)
-        if constexpr (sizeof(T) == 4) {
-            TSE_ZM2(2, 16, 2)
-            TSE_ZM2(2, 8, 4)
-            TSE_ZM2(1, 16, 3)
-        }
-        return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry)");
-    }
-#undef TSE_ZM2
-#define TSE_ZM(NS_, BY_)                                                        \
-    if (ns == NS_ && by == BY_) {                                               \
-        hipLaunchKernelGGL((rb_zmarch_kernel<T, NS_, BY_>), grid, dim3(64, BY_), \
-                           0, st, zg, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,  \
-                           zero_init);                                          \
-        TSE_HIP_TRY(hipGetLastError());                                         \
-        return 0;                                                               \
-    }
-    TSE_ZM(1, 8)
-    TSE_ZM(1, 16)
-    TSE_ZM(2, 16)
+    TSE_ZM(1, 8, 1)
+    TSE_ZM(1, 16, 1)
+    TSE_ZM(2, 16, 1)
+    TSE_ZM(1, 12, 2)
     if constexpr (sizeof(T) == 4) {
-        TSE_ZM(3, 16)
+        TSE_ZM(1, 16, 2)
+        TSE_ZM(2, 16, 2)
+        TSE_ZM(2, 8, 4)
+        TSE_ZM(3, 16, 1)
     }
 #undef TSE_ZM
-    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by)");
+    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry)");
 }
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32) {
-    const int ns = pl.ns, by = pl.by, ry = pl.ry;
-    if (ry <= 1)
-        return (ns == 1 && (by == 8 || by == 16)) || (ns == 2 && by == 16) ||
-               (fp32 && ns == 3 && by == 16);
-    if ((ns == 1 && by == 32 && ry == 2) || (ns == 2 && by == 16 && ry == 2))
-        return true;
-    return fp32 && ((ns == 2 && by == 32 && ry == 2) || (ns == 2 && by == 32 && ry == 4) ||
-                    (ns == 1 && by == 48 && ry == 3));
+    static const int both[][3] = {{1, 8, 1}, {1, 16, 1}, {2, 16, 1}, {1, 24, 2}};
+    static const int f32[][3] = {{1, 32, 2}, {2, 32, 2}, {2, 32, 4},
+                                 {3, 16, 1}};
+    for (auto &c : both)
+        if (pl.ns == c[0] && pl.by == c[1] && pl.ry == c[2]) return true;
+    if (fp32)
+        for (auto &c : f32)
+            if (pl.ns == c[0] && pl.by == c[1] && pl.ry == c[2]) return true;
+    return false;
 }
 
 ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
