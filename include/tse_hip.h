@@ -219,6 +219,29 @@ int tse_fast_p_result(tse_ctx *ctx, double *P);
 int tse_fast_p_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
                       int32_t n_sweeps, double sor, int32_t fp32);
 
+/* ---- performance forms of the transport solves ---------------------------- */
+/* compute_C (tse.py:783-916) and compute_T (tse.py:919-1081) at their fixed
+ * point: (R - omega M) x - omega sum_f a_f x_f = X + R Xn + W - E with the
+ * reference's coefficients (the faithful records, evaluated with the
+ * reference arithmetic), non-symmetric (upwind advection).  Each cell keeps
+ * its six neighbour coefficients, 1/diagonal and rhs in the natural layout;
+ * the sweeps are fused red/black z-marches as for pressure, the stop test is
+ * the relative residual, forms PLAIN64 or MIXED.  Supported plans (ns, by,
+ * ry): (1,8,1) (1,16,1) (2,16,1), and for FP32 also (3,16,1).  One transport
+ * system per context: a setup replaces the previous one.
+ * Setup copies the unknown at entry (C or T) into Cn / Tn, assembles at dt
+ * with time weighting omega, and loads the unknown as the iterate. */
+int tse_fast_c_setup(tse_ctx *ctx, const tse_c_fields *f, double dt,
+                     double omega);
+int tse_fast_t_setup(tse_ctx *ctx, const tse_t_fields *f, double dt,
+                     double omega);
+int tse_fast_tr_solve(tse_ctx *ctx, const tse_fast_params *prm,
+                      tse_fast_result *out);
+int tse_fast_tr_residual(tse_ctx *ctx, double *rel);
+int tse_fast_tr_result(tse_ctx *ctx, double *X);
+int tse_fast_tr_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
+                       int32_t n_sweeps, double sor, int32_t fp32);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
  * fixed dt (no norm, no control), for timing and sweep-level parity.

@@ -1,0 +1,141 @@
+"""GPU: the performance forms of the transport solves (compute_C, compute_T).
+
+The fast form solves the fixed point of the reference update (tse.py:870-875,
+1038-1043) with the reference's coefficients, by fused red/black z-marches,
+to a relative residual.  It must land on the oracle's fully converged
+compute_C / compute_T result (update norm 1e-15, far below what the fast
+tolerances leave):
+  * plain FP64 to rel. residual 1e-13: within 1e-10 of max|x|,
+  * FP32 sweeps in FP64 defect correction to 1e-11: within 1e-8 of max|x|,
+for every kernel variant, with and without z-chunks, weak and strong
+advection, omega < 1, both imbalance-flow settings, Dirichlet cells.
+The reported residual must agree with a fresh residual evaluation.
+"""
+import types
+
+import numpy as np
+import pytest
+
+from helpers import Fields
+
+pytestmark = pytest.mark.gpu
+
+# (ns, by, ry, zchunk): every transport kernel variant, with and without z-chunks
+PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 5), (2, 16, 1, 0), (2, 16, 1, 6)]
+PLANS32 = [(1, 16, 1, 0), (2, 16, 1, 7), (3, 16, 1, 0), (3, 16, 1, 9)]
+SHAPES = [(70, 23, 19), (33, 20, 17), (9, 40, 12)]
+
+
+def _frozen(f):
+    return types.SimpleNamespace(c_r=f.c_r, l_r=f.l_r, cp_r=f.cp_r,
+                                 rho_r=f.rho_r, lf_l=f.lf_l, lf_t=f.lf_t,
+                                 lt_l=f.lt_l, lt_t=f.lt_t)
+
+
+def _oracle_c(f, dt, omega, imbl, n=0, sor=1.1):
+    from oracle import Oracle
+    o = Oracle(*f.grid_args(), frozen_source=_frozen(f), maxit_c=20000,
+               dt_max=1e12, c_imblflo=imbl)
+    C = f.C[..., n].copy()
+    r = o.compute_C(C, f.D_fx[..., n], f.D_fy[..., n], f.D_fz[..., n], f.phi,
+                    f.rho_f, f.qx, f.qy, f.qz, dt, f.BC, f.WC[..., n], f.R, f.e,
+                    omega=omega, SOR=sor, max_it=20000, C_conv=1e-15)
+    assert r[3] and o.last.num_halvings == 0
+    return C
+
+
+def _oracle_t(f, dt, omega, imbl):
+    from oracle import Oracle
+    o = Oracle(*f.grid_args(), frozen_source=_frozen(f), maxit_t=20000,
+               dt_max=1e12, t_imblflo=imbl)
+    T = f.T.copy()
+    r = o.compute_T(T, f.l_f, f.cp_f, f.rho_f, f.phi, f.qx, f.qy, f.qz, dt,
+                    f.BT, f.WT, f.e, omega=omega, SOR=1.1, max_it=20000,
+                    T_conv=1e-15)
+    assert r[3] and o.last.num_halvings == 0
+    return T
+
+
+def _fast_c(f, dt, omega, imbl, n=0):
+    from transportse_amd.transport import FastTransport
+    ft = FastTransport(*f.grid_args())
+    ft.load_c(f.C[..., n], f.D_fx[..., n], f.D_fy[..., n], f.D_fz[..., n],
+              f.phi, f.rho_f, f.qx, f.qy, f.qz, dt, f.BC, f.WC[..., n], f.R,
+              f.e, omega, lf_l=f.lf_l, lf_t=f.lf_t, imblflo=imbl)
+    return ft
+
+
+def _fast_t(f, dt, omega, imbl):
+    from transportse_amd.transport import FastTransport
+    ft = FastTransport(*f.grid_args())
+    ft.load_t(f.T, f.l_f, f.cp_f, f.rho_f, f.phi, f.qx, f.qy, f.qz, dt, f.BT,
+              f.WT, f.e, omega, l_r=f.l_r, cp_r=f.cp_r, rho_r=f.rho_r,
+              lt_l=f.lt_l, lt_t=f.lt_t, imblflo=imbl)
+    return ft
+
+
+def _check(ft, ref, form, tol, bound, plan, what):
+    st = ft.solve(form, tol=tol, sor=1.3, max_sweeps=4000, inner=8, plan=plan)
+    x = ft.result()
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    again = ft.residual()
+    print(what, form, plan, "sweeps", st.sweeps, "rel", st.rel_residual,
+          "err", err, flush=True)
+    assert st.converged and st.rel_residual <= tol
+    assert abs(again - st.rel_residual) <= 1e-3 * st.rel_residual + 1e-300
+    assert err <= bound
+    return x
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("plan", PLANS64)
+def test_c_plain64_reaches_oracle_fixed_point(shape, plan):
+    f = Fields(*shape, seed=41)
+    dt = 86400.0 * 30
+    ref = _oracle_c(f, dt, 1.0, True)
+    _check(_fast_c(f, dt, 1.0, True), ref, "plain64", 1e-13, 1e-10, plan, "C")
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("plan", PLANS32)
+def test_t_mixed_reaches_oracle_fixed_point(shape, plan):
+    f = Fields(*shape, seed=42)
+    dt = 86400.0 * 30
+    ref = _oracle_t(f, dt, 1.0, False)
+    _check(_fast_t(f, dt, 1.0, False), ref, "mixed", 1e-11, 1e-8, plan, "T")
+
+
+@pytest.mark.parametrize("which", ["C", "T"])
+@pytest.mark.parametrize("omega,imbl,qscale", [(1.0, True, 100.0),
+                                               (0.6, False, 1.0),
+                                               (0.7, True, 30.0)])
+def test_transport_cases(which, omega, imbl, qscale):
+    f = Fields(48, 26, 21, seed=43)
+    f.qx *= qscale
+    f.qy *= qscale
+    f.qz *= qscale
+    dt = 86400.0 * 10
+    if which == "C":
+        ref = _oracle_c(f, dt, omega, imbl, n=1)
+        ft = _fast_c(f, dt, omega, imbl, n=1)
+    else:
+        ref = _oracle_t(f, dt, omega, imbl)
+        ft = _fast_t(f, dt, omega, imbl)
+    _check(ft, ref, "plain64", 1e-13, 1e-10, (2, 16, 1, 0), which)
+    ft.assemble()
+    _check(ft, ref, "mixed", 1e-11, 1e-8, (3, 16, 1, 8), which)
+
+
+def test_explicit_omega_zero():
+    """omega = 0: the update has no neighbour term, so one unrelaxed sweep is
+    the reference's accepted result (tse.py:897 accepts at once)."""
+    f = Fields(33, 20, 17, seed=44)
+    dt = 3600.0
+    ref = _oracle_c(f, dt, 0.0, True, sor=1.0)
+    ft = _fast_c(f, dt, 0.0, True)
+    st = ft.solve("plain64", tol=1e-14, sor=1.0, max_sweeps=8, inner=1,
+                  plan=(1, 16, 1, 0))
+    x = ft.result()
+    err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
+    print("omega=0 sweeps", st.sweeps, "err", err, flush=True)
+    assert st.converged and err <= 1e-14

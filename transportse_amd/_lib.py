@@ -107,6 +107,14 @@ SIGNATURES = {
     "tse_fast_p_result": (_I, [_V, _dp]),
     "tse_fast_p_sweeps": (_I, [_V, ctypes.POINTER(SweepPlan), ctypes.c_int32,
                                _D, ctypes.c_int32]),
+    "tse_fast_c_setup": (_I, [_V, ctypes.POINTER(CFields), _D, _D]),
+    "tse_fast_t_setup": (_I, [_V, ctypes.POINTER(TFields), _D, _D]),
+    "tse_fast_tr_solve": (_I, [_V, ctypes.POINTER(FastParams),
+                               ctypes.POINTER(FastResult)]),
+    "tse_fast_tr_residual": (_I, [_V, _DPH]),
+    "tse_fast_tr_result": (_I, [_V, _dp]),
+    "tse_fast_tr_sweeps": (_I, [_V, ctypes.POINTER(SweepPlan), ctypes.c_int32,
+                                _D, ctypes.c_int32]),
 }
 
 _lib = None

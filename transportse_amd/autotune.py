@@ -1,13 +1,13 @@
-"""Autotuner for the fused sweep kernel's launch plan.
+"""Autotuner for the fused sweep kernels' launch plans.
 
 A plan is (ns sweeps per launch, tile rows by, rows per thread ry, z-chunk
-planes).  The tuner times every supported plan on the problem already set
-up in the context (``tse_fast_p_setup``) with HIP events on the library's
-stream, a few sweeps each after one warm-up launch, and returns the fastest.
-Sweeps are timed on a scratch copy semantics-free: the FP64 candidates
-advance the iterate (harmless: the solve continues from any iterate), the
-FP32 candidates run on the correction buffers.  Results are cached per
-(grid, precision).
+planes).  The tuner times every supported plan on the system already set up
+in the context (``tse_fast_p_setup`` / ``tse_fast_c_setup`` /
+``tse_fast_t_setup``) with HIP events on the library's stream, a few sweeps
+each after one warm-up launch, and returns the fastest.  The FP64
+candidates advance the iterate (harmless: the solve continues from any
+iterate), the FP32 candidates run on the correction buffers.  Results are
+cached per (system kind, grid, precision).
 """
 from __future__ import annotations
 
@@ -17,40 +17,47 @@ from . import _lib
 
 PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1), (1, 24, 2)]
 PLANS32 = PLANS64 + [(1, 32, 2), (2, 32, 2), (2, 32, 4), (3, 16, 1)]
+# transport: the cell's own coefficients stay in registers, one row per thread
+TR_PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1)]
+TR_PLANS32 = TR_PLANS64 + [(3, 16, 1)]
 ZCHUNKS = (0, 128, 64, 32)
+
+KINDS = {
+    "p": ("tse_fast_p_sweeps", PLANS64, PLANS32),
+    "tr": ("tse_fast_tr_sweeps", TR_PLANS64, TR_PLANS32),
+}
 
 _cache: dict = {}
 
 
 def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
-         verbose: bool = False):
+         verbose: bool = False, kind: str = "p"):
     """Return ((ns, by, ry, zchunk), ms_per_sweep) of the fastest plan."""
     import torch
-    key = (hip.shape, bool(fp32))
+    key = (kind, hip.shape, bool(fp32))
     if key in _cache:
         return _cache[key]
-    lib = hip.lib
+    fn_name, p64, p32 = KINDS[kind]
+    fn = getattr(hip.lib, fn_name)
     best = None
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     stream = torch.cuda.current_stream()
     nz = hip.shape[2]
-    for ns, by, ry in (PLANS32 if fp32 else PLANS64):
+    for ns, by, ry in (p32 if fp32 else p64):
         for zc in zchunks:
             if zc and zc >= nz:
                 continue
             n = ns * max(1, sweeps // ns)
             plan = _lib.SweepPlan(ns, by, ry, zc)
-            _lib.check(lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), ns,
-                                             sor, int(fp32)), "warmup")
+            _lib.check(fn(hip.ctx, ctypes.byref(plan), ns, sor, int(fp32)), "warmup")
             ev0.record(stream)
-            _lib.check(lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n,
-                                             sor, int(fp32)), "sweeps")
+            _lib.check(fn(hip.ctx, ctypes.byref(plan), n, sor, int(fp32)), "sweeps")
             ev1.record(stream)
             torch.cuda.synchronize()
             ms = ev0.elapsed_time(ev1) / n
             if verbose:
-                print(f"  plan ns={ns} by={by} ry={ry} zchunk={zc}: "
+                print(f"  {kind} plan ns={ns} by={by} ry={ry} zchunk={zc}: "
                       f"{ms * 1e3:.1f} us/sweep", flush=True)
             if best is None or ms < best[1]:
                 best = ((ns, by, ry, zc), ms)

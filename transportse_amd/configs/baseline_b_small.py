@@ -1,0 +1,14 @@
+"""Workload B at 64x64x32 (same physics and settings as baseline_b, sized
+for the CPU oracle): stretched dz, slope 5 deg, salt + heat, dispersion,
+SIA, 20 time steps."""
+import numpy
+from numba import njit  # noqa: F401
+
+from transportse_amd.configs._common import build
+
+NZ = 32
+build(globals(), 64, 64, NZ, dx=10.0, dy=10.0,
+      dz=2.0 * 1.05 ** numpy.arange(NZ)[::-1], beta=5.0, sigma=1.0, corr=5,
+      seed=2, heat=True, sia=True, sia_maxit=3, sia_conv=1e-6, nsteps=20,
+      dt=86400.0, dt_max=86400.0, sor_p=1.8, conv_p=1e-8, conv_c=1e-8,
+      conv_t=1e-8, disp=(1.0, 0.1))

@@ -202,7 +202,6 @@ __device__ inline void block_sum2(double a, double b, double *out) {
 // Record fields (each a 2*S array): A_, C_, E_, G_, I_, K_ (face coefficients
 // toward x+, x-, y+, y-, z+, z-), A_*B_, C_*D_, I_*J_, K_*L_ (gravity
 // products), R*Pn, WP, R+M.
-constexpr int kPRec = 13;
 
 __global__ void __launch_bounds__(kBlock)
 p_assemble_kernel(Geo G, const double *__restrict__ Pn,
@@ -479,7 +478,6 @@ flow_error_kernel(Geo G, const double *__restrict__ P,
 // z- ((A_-N_), (B_+M_), (C_-P_), (D_+O_), (E_-R_), (F_+Q_)), then
 // X = (1-omega)*(Xn*M + Mold + ... + Rold), R*Xn, W, E (imbalance term),
 // R - omega*M.
-constexpr int kTRec = 11;
 
 // Shared tail of both assemble kernels (tse.py:862-875 / 1026-1043).
 __device__ inline void transport_tail(double *v, const double a[6],

@@ -423,27 +423,284 @@ reduce2_kernel(const double *__restrict__ partials, int n, double *__restrict__ 
     }
 }
 
+// ---- transport systems (compute_C / compute_T at their fixed point) -------------------------
+// The faithful transport records (tse_internal.h kTRec) give, per cell, the
+// neighbour coefficients a_f, the constant part X + R*Xn + W - E and the
+// diagonal R - omega*M.  At the fixed point of the update (tse.py:870-875,
+// 1038-1043) the system is
+//     (R - omega M) x_c - sum_f omega a_f x_f = X + R Xn + W - E,
+// non-symmetric (upwind), so each cell keeps its own six coefficients
+// c_f = omega * a_f (order x+, x-, y+, y-, z+, z-), 1/diagonal and rhs.
+
+struct TrOp {  // natural layout, coefficient f of cell t at c[f*N + t]
+    double *c, *D, *invD, *b;
+    float *c32, *invD32;
+    long N;
+};
+
+__global__ void __launch_bounds__(kBlock)
+tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
+                       const double *__restrict__ rec,
+                       const uint8_t *__restrict__ mask, double omega, TrOp op) {
+    const long N = op.N;
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * G.nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * G.ny;
+        const long c = slot(G, i, j, k);
+        const double *r = rec + c;
+        const bool active = (mask[c] & 2) != 0;
+        const double den = r[10 * F];
+        const double inv = (active && den != 0.0) ? 1.0 / den : 0.0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            const double cf = omega * r[f * F];
+            op.c[f * N + t] = cf;
+            op.c32[f * N + t] = (float)cf;
+        }
+        op.D[t] = den;
+        op.invD[t] = inv;
+        op.invD32[t] = (float)inv;
+        op.b[t] = r[6 * F] + r[7 * F] + r[8 * F] - r[9 * F];
+    }
+}
+
+// Fused red/black z-march for the non-symmetric transport operator.  Same
+// pipeline as rb_zmarch_kernel (one barrier per plane step, NS sweeps per
+// launch, 2*NS halo ring, ping-pong); the coefficients are the cell's own,
+// so only the unknown goes through LDS.
+template <typename T, int NS, int BY>
+__global__ void __launch_bounds__(64 * BY)
+tr_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                 const T *__restrict__ coef, long N, const T *__restrict__ invD,
+                 const T *__restrict__ rhs, T omega, int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int NSLOT = R + 2;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[NSLOT][BY][BX];
+
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int i = tx * TX - R + lx;
+    const int j = ty * TY - R + ly;
+    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const long plane = (long)g.nx * g.ny;
+    const long col = in_dom ? (long)i + (long)g.nx * j : 0;
+    const int par = (i + j) & 1;
+
+    T p[R + 2], rinv[R + 1], rb[R + 1], rc[6][R + 1];
+#pragma unroll
+    for (int q = 0; q < R + 2; ++q) p[q] = T(0);
+#pragma unroll
+    for (int q = 0; q < R + 1; ++q) {
+        rinv[q] = T(0);
+        rb[q] = T(0);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) rc[f][q] = T(0);
+    }
+    T np = T(0), ni = T(0), nb = T(0), nc[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nc[f] = T(0);
+    if (in_dom && kl0 <= kl1) {
+        const long o = col + plane * kl0;
+        np = zero_init ? T(0) : Pin[o];
+        ni = invD[o];
+        nb = rhs[o];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
+    }
+    int s0 = kl0 % NSLOT;
+    for (int k = kl0; k <= k1 + R - 1; ++k) {
+        const T p0 = np, i0 = ni, b0 = nb;
+        T c0[6];
+#pragma unroll
+        for (int f = 0; f < 6; ++f) {
+            c0[f] = nc[f];
+            nc[f] = T(0);
+        }
+        np = ni = nb = T(0);
+        if (in_dom && k + 1 <= kl1) {
+            const long o = col + plane * (k + 1);
+            np = zero_init ? T(0) : Pin[o];
+            ni = invD[o];
+            nb = rhs[o];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
+        }
+#pragma unroll
+        for (int q = R + 1; q > 0; --q) p[q] = p[q - 1];
+#pragma unroll
+        for (int q = R; q > 0; --q) {
+            rinv[q] = rinv[q - 1];
+            rb[q] = rb[q - 1];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) rc[f][q] = rc[f][q - 1];
+        }
+        p[0] = p0;
+        rinv[0] = i0;
+        rb[0] = b0;
+#pragma unroll
+        for (int f = 0; f < 6; ++f) rc[f][0] = c0[f];
+        sP[s0][ly][lx] = p0;
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+            const int m = k - 1 - h;
+            int s = s0 - 1 - h;
+            if (s < 0) s += NSLOT;
+            if (m >= kl0 && m <= kl1 && d >= h + 1 && in_dom &&
+                ((par + m) & 1) == (h & 1) && rinv[1 + h] != T(0)) {
+                T acc = rb[1 + h];
+                acc += rc[0][1 + h] * sP[s][ly][lx + 1];
+                acc += rc[1][1 + h] * sP[s][ly][lx - 1];
+                acc += rc[2][1 + h] * sP[s][ly + 1][lx];
+                acc += rc[3][1 + h] * sP[s][ly - 1][lx];
+                acc += rc[4][1 + h] * p[h];
+                acc += rc[5][1 + h] * p[h + 2];
+                const T pc = p[1 + h];
+                const T pn = pc + omega * (acc * rinv[1 + h] - pc);
+                p[1 + h] = pn;
+                sP[s][ly][lx] = pn;
+            }
+        }
+        const int mo = k - R;
+        if (mo >= k0 && mo < k1 && d >= R && in_dom) Pout[col + plane * mo] = p[R];
+        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
+    }
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(kBlock)
+tr_residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
+                   const double *__restrict__ pin, const float *__restrict__ e,
+                   double *__restrict__ pout, TrOp op, float *__restrict__ r32,
+                   double *__restrict__ partials) {
+    const long N = op.N;
+    const long plane = (long)nx * ny;
+    double sr = 0.0, sb = 0.0;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
+         t += (long)gridDim.x * blockDim.x) {
+        const int row = (int)fdiv((uint32_t)t, dnx);
+        const int i = (int)t - row * nx;
+        const int k = (int)fdiv((uint32_t)row, dny);
+        const int j = row - k * ny;
+        auto X = [&](long o) -> double {
+            return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
+        };
+        const double xc = X(t);
+        if (MODE == 1) pout[t] = xc;
+        double r = 0.0;
+        if (op.invD[t] != 0.0) {
+            double acc = op.b[t];
+            if (i < nx - 1) acc += op.c[0 * N + t] * X(t + 1);
+            if (i > 0) acc += op.c[1 * N + t] * X(t - 1);
+            if (j < ny - 1) acc += op.c[2 * N + t] * X(t + nx);
+            if (j > 0) acc += op.c[3 * N + t] * X(t - nx);
+            if (k < nz - 1) acc += op.c[4 * N + t] * X(t + plane);
+            if (k > 0) acc += op.c[5 * N + t] * X(t - plane);
+            r = acc - op.D[t] * xc;
+            sr += r * r;
+            sb += op.b[t] * op.b[t];
+        }
+        if (MODE != 0) r32[t] = (float)r;
+    }
+    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
+    sr = wsum(sr);
+    sb = wsum(sb);
+    if ((threadIdx.x & 63) == 0) {
+        s1[threadIdx.x >> 6] = sr;
+        s2[threadIdx.x >> 6] = sb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0;
+        for (int q = 0; q < kBlock / 64; ++q) {
+            a += s1[q];
+            c += s2[q];
+        }
+        partials[2 * blockIdx.x] = a;
+        partials[2 * blockIdx.x + 1] = c;
+    }
+}
+
 // ---- host side ---------------------------------------------------------------------------------------
 
-// Fast-form buffers owned by the context.
-struct FastState {
+enum SysKind { SYS_P = 0, SYS_TR = 1 };
+
+// One linear system in the performance form, owned by the context.
+//   pressure:  buf64 = Wx Wy Wz D invD b | p0 p1      buf32 = Wx Wy Wz invD r | e0 e1
+//   transport: buf64 = c[6] D invD b | p0 p1          buf32 = c[6] invD r | e0 e1
+struct FastSys {
+    int kind = SYS_P;
     long N = 0;
-    double *buf64 = nullptr;  // Wx Wy Wz D invD b | p[0] p[1]   (8 N)
-    float *buf32 = nullptr;   // Wx Wy Wz invD r | e[0] e[1]     (7 N)
+    double *buf64 = nullptr;
+    float *buf32 = nullptr;
     double *partials = nullptr;
     double *red = nullptr;
     double *red_host = nullptr;
-    int cur = 0;              // which of p[0], p[1] holds the iterate
+    int cur = 0;          // which of p0, p1 holds the iterate
     bool ready = false;
+
+    int n64() const { return kind == SYS_P ? 6 : 9; }
+    int n32() const { return kind == SYS_P ? 5 : 8; }
+    double *p(int q) const { return buf64 + (n64() + q) * N; }
+    float *r32() const { return buf32 + (n32() - 1) * N; }
+    float *e(int q) const { return buf32 + (n32() + q) * N; }
+    FastOp pop() const {
+        FastOp op;
+        op.Wx = buf64;
+        op.Wy = buf64 + N;
+        op.Wz = buf64 + 2 * N;
+        op.D = buf64 + 3 * N;
+        op.invD = buf64 + 4 * N;
+        op.b = buf64 + 5 * N;
+        op.Wx32 = buf32;
+        op.Wy32 = buf32 + N;
+        op.Wz32 = buf32 + 2 * N;
+        op.invD32 = buf32 + 3 * N;
+        return op;
+    }
+    TrOp trop() const {
+        TrOp op;
+        op.c = buf64;
+        op.D = buf64 + 6 * N;
+        op.invD = buf64 + 7 * N;
+        op.b = buf64 + 8 * N;
+        op.c32 = buf32;
+        op.invD32 = buf32 + 6 * N;
+        op.N = N;
+        return op;
+    }
 };
 
-static FastState *state(tse_ctx *ctx) {
-    if (!ctx->fast) ctx->fast = new FastState();
-    return (FastState *)ctx->fast;
+static FastSys *system_of(tse_ctx *ctx, int kind) {
+    void *&slot_ = kind == SYS_P ? ctx->fast : ctx->fast_tr;
+    if (!slot_) {
+        FastSys *s = new FastSys();
+        s->kind = kind;
+        slot_ = s;
+    }
+    return (FastSys *)slot_;
 }
 
-void fast_free(tse_ctx *ctx) {
-    FastState *s = (FastState *)ctx->fast;
+static void free_sys(void *&ptr) {
+    FastSys *s = (FastSys *)ptr;
     if (!s) return;
     if (s->buf64) (void)hipFree(s->buf64);
     if (s->buf32) (void)hipFree(s->buf32);
@@ -451,10 +708,15 @@ void fast_free(tse_ctx *ctx) {
     if (s->red) (void)hipFree(s->red);
     if (s->red_host) (void)hipHostFree(s->red_host);
     delete s;
-    ctx->fast = nullptr;
+    ptr = nullptr;
 }
 
-static int ensure(tse_ctx *ctx, FastState *s) {
+void fast_free(tse_ctx *ctx) {
+    free_sys(ctx->fast);
+    free_sys(ctx->fast_tr);
+}
+
+static int ensure(tse_ctx *ctx, FastSys *s) {
     const Geo &G = ctx->geo;
     const long N = (long)G.nx * G.ny * G.nz;
     if (!s->partials) {
@@ -471,34 +733,14 @@ static int ensure(tse_ctx *ctx, FastState *s) {
         s->buf64 = nullptr;
         s->buf32 = nullptr;
     }
-    if (hipMalloc((void **)&s->buf64, sizeof(double) * 8 * N) != hipSuccess)
-        return fail(TSE_ERR_NOMEM, "fast-form FP64 operator allocation failed");
-    if (hipMalloc((void **)&s->buf32, sizeof(float) * 7 * N) != hipSuccess)
-        return fail(TSE_ERR_NOMEM, "fast-form FP32 operator allocation failed");
     s->N = N;
+    if (hipMalloc((void **)&s->buf64, sizeof(double) * (s->n64() + 2) * N) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "fast-form FP64 operator allocation failed");
+    if (hipMalloc((void **)&s->buf32, sizeof(float) * (s->n32() + 2) * N) != hipSuccess)
+        return fail(TSE_ERR_NOMEM, "fast-form FP32 operator allocation failed");
     s->ready = false;
     return 0;
 }
-
-static FastOp fast_op(FastState *s) {
-    const long N = s->N;
-    FastOp op;
-    op.Wx = s->buf64;
-    op.Wy = s->buf64 + N;
-    op.Wz = s->buf64 + 2 * N;
-    op.D = s->buf64 + 3 * N;
-    op.invD = s->buf64 + 4 * N;
-    op.b = s->buf64 + 5 * N;
-    op.Wx32 = s->buf32;
-    op.Wy32 = s->buf32 + N;
-    op.Wz32 = s->buf32 + 2 * N;
-    op.invD32 = s->buf32 + 3 * N;
-    return op;
-}
-
-static inline double *pbuf(FastState *s, int q) { return s->buf64 + (6 + q) * s->N; }
-static inline float *r32buf(FastState *s) { return s->buf32 + 4 * s->N; }
-static inline float *ebuf(FastState *s, int q) { return s->buf32 + (5 + q) * s->N; }
 
 static int grid_n(long n) {
     long b = (n + kBlock - 1) / kBlock;
@@ -507,14 +749,19 @@ static int grid_n(long n) {
 
 // Launches a residual pass and returns ||r||, ||b|| (synchronises).
 template <int MODE>
-static int residual(tse_ctx *ctx, FastState *s, const double *pin, const float *e,
+static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
                     double *pout, double *rn, double *bn) {
     const Geo &G = ctx->geo;
     const int nb = grid_n(s->N);
-    hipLaunchKernelGGL((residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
-                       ctx->stream, G.nx, G.ny, G.nz, make_fastdiv(G.nx),
-                       make_fastdiv(G.ny), pin, e, pout, fast_op(s), r32buf(s),
-                       s->partials);
+    const FastDiv dnx = make_fastdiv(G.nx), dny = make_fastdiv(G.ny);
+    if (s->kind == SYS_P)
+        hipLaunchKernelGGL((residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
+                           ctx->stream, G.nx, G.ny, G.nz, dnx, dny, pin, e, pout,
+                           s->pop(), s->r32(), s->partials);
+    else
+        hipLaunchKernelGGL((tr_residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
+                           ctx->stream, G.nx, G.ny, G.nz, dnx, dny, pin, e, pout,
+                           s->trop(), s->r32(), s->partials);
     TSE_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
                        s->partials, nb, s->red);
@@ -555,10 +802,36 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by, int ry,
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry)");
 }
 
-bool fast_plan_ok(const tse_sweep_plan &pl, int fp32) {
+template <typename T>
+static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
+                            const T *Pin, T *Pout, const T *coef, long N,
+                            const T *invD, const T *rhs, T omega, int zero_init) {
+    const dim3 grid(zg.per_xcd * 8);
+#define TSE_TZ(NS_, BY_)                                                          \
+    if (ns == NS_ && by == BY_) {                                                 \
+        hipLaunchKernelGGL((tr_zmarch_kernel<T, NS_, BY_>), grid, dim3(64, BY_),   \
+                           0, st, zg, Pin, Pout, coef, N, invD, rhs, omega,       \
+                           zero_init);                                            \
+        TSE_HIP_TRY(hipGetLastError());                                           \
+        return 0;                                                                 \
+    }
+    TSE_TZ(1, 8)
+    TSE_TZ(1, 16)
+    TSE_TZ(2, 16)
+    if constexpr (sizeof(T) == 4) {
+        TSE_TZ(3, 16)
+    }
+#undef TSE_TZ
+    return fail(TSE_ERR_ARG, "unsupported transport sweep plan (ns, by)");
+}
+
+static bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
+    if (kind == SYS_TR)
+        return pl.ry == 1 && ((pl.ns == 1 && (pl.by == 8 || pl.by == 16)) ||
+                              (pl.ns == 2 && pl.by == 16) ||
+                              (fp32 && pl.ns == 3 && pl.by == 16));
     static const int both[][3] = {{1, 8, 1}, {1, 16, 1}, {2, 16, 1}, {1, 24, 2}};
-    static const int f32[][3] = {{1, 32, 2}, {2, 32, 2}, {2, 32, 4},
-                                 {3, 16, 1}};
+    static const int f32[][3] = {{1, 32, 2}, {2, 32, 2}, {2, 32, 4}, {3, 16, 1}};
     for (auto &c : both)
         if (pl.ns == c[0] && pl.by == c[1] && pl.ry == c[2]) return true;
     if (fp32)
@@ -567,7 +840,7 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32) {
     return false;
 }
 
-ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
+static ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
     ZGeo z{};
     z.nx = G.nx;
     z.ny = G.ny;
@@ -584,36 +857,139 @@ ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
     return z;
 }
 
-// n sweeps on the iterate (plain) or on the correction e (mixed, e starts
-// at 0 when zero_first); keeps the ping-pong index.
-static int plain_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg,
+// n FP64 sweeps on the iterate (ping-pong index kept in the system).
+static int plain_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
                         const tse_sweep_plan &pl, int n, double sor) {
-    FastOp op = fast_op(s);
-    const int ns = pl.ns;
-    for (int q = 0; q < n / ns; ++q) {
-        int rc = launch_zmarch<double>(ctx->stream, zg, ns, pl.by, pl.ry, pbuf(s, s->cur),
-                                       pbuf(s, 1 - s->cur), op.Wx, op.Wy, op.Wz,
-                                       op.invD, op.b, sor, 0);
+    for (int q = 0; q < n / pl.ns; ++q) {
+        int rc;
+        if (s->kind == SYS_P) {
+            FastOp op = s->pop();
+            rc = launch_zmarch<double>(ctx->stream, zg, pl.ns, pl.by, pl.ry,
+                                       s->p(s->cur), s->p(1 - s->cur), op.Wx,
+                                       op.Wy, op.Wz, op.invD, op.b, sor, 0);
+        } else {
+            TrOp op = s->trop();
+            rc = launch_tr_zmarch<double>(ctx->stream, zg, pl.ns, pl.by,
+                                          s->p(s->cur), s->p(1 - s->cur), op.c,
+                                          s->N, op.invD, op.b, sor, 0);
+        }
         if (rc) return rc;
         s->cur = 1 - s->cur;
     }
     return 0;
 }
 
-static int correction_sweeps(tse_ctx *ctx, FastState *s, const ZGeo &zg,
+// n FP32 sweeps on the correction equation A e = r from e = 0; *ecur
+// receives the buffer holding e.
+static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
                              const tse_sweep_plan &pl, int n, float sor,
                              int *ecur) {
-    FastOp op = fast_op(s);
-    const int ns = pl.ns;
     int e = 0;
-    for (int q = 0; q < n / ns; ++q) {
-        int rc = launch_zmarch<float>(ctx->stream, zg, ns, pl.by, pl.ry, ebuf(s, e),
-                                      ebuf(s, 1 - e), op.Wx32, op.Wy32, op.Wz32,
-                                      op.invD32, r32buf(s), sor, q == 0);
+    for (int q = 0; q < n / pl.ns; ++q) {
+        int rc;
+        if (s->kind == SYS_P) {
+            FastOp op = s->pop();
+            rc = launch_zmarch<float>(ctx->stream, zg, pl.ns, pl.by, pl.ry,
+                                      s->e(e), s->e(1 - e), op.Wx32, op.Wy32,
+                                      op.Wz32, op.invD32, s->r32(), sor, q == 0);
+        } else {
+            TrOp op = s->trop();
+            rc = launch_tr_zmarch<float>(ctx->stream, zg, pl.ns, pl.by, s->e(e),
+                                         s->e(1 - e), op.c32, s->N, op.invD32,
+                                         s->r32(), sor, q == 0);
+        }
         if (rc) return rc;
         e = 1 - e;
     }
     *ecur = e;
+    return 0;
+}
+
+static int load_iterate(tse_ctx *ctx, FastSys *s, const double *X_split) {
+    const Geo &G = ctx->geo;
+    s->cur = 0;
+    hipLaunchKernelGGL(split_to_nat_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       X_split, s->p(0));
+    TSE_HIP_TRY(hipGetLastError());
+    s->ready = true;
+    return 0;
+}
+
+static int sys_result(tse_ctx *ctx, FastSys *s, double *X_split) {
+    if (!s->ready || !X_split) return fail(TSE_ERR_STATE, "fast form not set up");
+    const Geo &G = ctx->geo;
+    hipLaunchKernelGGL(nat_to_split_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       s->p(s->cur), X_split);
+    TSE_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int sys_residual(tse_ctx *ctx, FastSys *s, double *rel) {
+    if (!s->ready || !rel) return fail(TSE_ERR_STATE, "fast form not set up");
+    double rn, bn;
+    int rc = residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
+    if (rc) return rc;
+    *rel = rn / (bn > 0 ? bn : 1.0);
+    return 0;
+}
+
+static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
+                      int n_sweeps, double sor, int fp32) {
+    if (!s->ready || !plan) return fail(TSE_ERR_STATE, "fast form not set up");
+    if (!fast_plan_ok(*plan, fp32, s->kind) || n_sweeps < 0 || n_sweeps % plan->ns)
+        return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
+    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk);
+    if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
+    int e;  // FP32 timing path: correction sweeps on the last residual
+    return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
+}
+
+static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
+                     tse_fast_result *out) {
+    if (!s->ready || !prm || !out) return fail(TSE_ERR_STATE, "fast form not set up");
+    const int fp32 = prm->form == TSE_FORM_MIXED;
+    if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
+        return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
+    const tse_sweep_plan &pl = prm->plan;
+    const int chunk = prm->inner;
+    if (!fast_plan_ok(pl, fp32, s->kind) || chunk < pl.ns || chunk % pl.ns ||
+        prm->max_sweeps < 0)
+        return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk);
+    double rn, bn;
+    int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
+                  : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
+    if (rc) return rc;
+    const double bs = bn > 0 ? bn : 1.0;
+    out->initial_rel = rn / bs;
+    double rel = rn / bs;
+    int sweeps = 0, cycles = 0;
+    while (rel > prm->tol && sweeps < prm->max_sweeps) {
+        if (fp32) {
+            int e;
+            rc = correction_sweeps(ctx, s, zg, pl, chunk, (float)prm->sor, &e);
+            if (rc) return rc;
+            rc = residual<1>(ctx, s, s->p(s->cur), s->e(e), s->p(1 - s->cur),
+                             &rn, &bn);
+            if (rc) return rc;
+            s->cur = 1 - s->cur;
+        } else {
+            rc = plain_sweeps(ctx, s, zg, pl, chunk, prm->sor);
+            if (rc) return rc;
+            rc = residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
+            if (rc) return rc;
+        }
+        sweeps += chunk;
+        cycles += 1;
+        rel = rn / bs;
+        if (!(rel == rel)) break;  // NaN
+    }
+    out->sweeps = sweeps;
+    out->cycles = cycles;
+    out->rel_residual = rel;
+    out->converged = rel <= prm->tol ? 1 : 0;
     return 0;
 }
 
@@ -630,113 +1006,108 @@ int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt) {
         !f->Kx || !f->Ky || !f->Kz || !f->BP || !f->WP || !f->c_r)
         return fail(TSE_ERR_ARG, "null field pointer");
     if (!(dt > 0)) return fail(TSE_ERR_ARG, "dt must be > 0");
-    FastState *s = state(ctx);
+    FastSys *s = system_of(ctx, SYS_P);
     rc = ensure(ctx, s);
     if (rc) return rc;
     const Geo &G = ctx->geo;
-    const FastDiv dnx = make_fastdiv(G.nx), dny = make_fastdiv(G.ny);
     const double *Pn = f->Pn ? f->Pn : f->P;
     hipLaunchKernelGGL(fast_assemble_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, dnx, dny, Pn, f->c_f, f->nue, f->rho_f,
-                       f->phi, f->Kx, f->Ky, f->Kz, f->BP, f->WP, f->c_r, dt,
-                       fast_op(s));
+                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny), Pn,
+                       f->c_f, f->nue, f->rho_f, f->phi, f->Kx, f->Ky, f->Kz,
+                       f->BP, f->WP, f->c_r, dt, s->pop());
     TSE_HIP_TRY(hipGetLastError());
-    s->cur = 0;
-    hipLaunchKernelGGL(split_to_nat_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, dnx, dny, f->P, pbuf(s, 0));
-    TSE_HIP_TRY(hipGetLastError());
-    s->ready = true;
-    return 0;
+    return load_iterate(ctx, s, f->P);
 }
 
 int tse_fast_p_result(tse_ctx *ctx, double *P) {
     int rc = check_ctx(ctx);
-    if (rc) return rc;
-    FastState *s = state(ctx);
-    if (!s->ready || !P) return fail(TSE_ERR_STATE, "fast form not set up");
-    const Geo &G = ctx->geo;
-    hipLaunchKernelGGL(nat_to_split_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
-                       pbuf(s, s->cur), P);
-    TSE_HIP_TRY(hipGetLastError());
-    return 0;
+    return rc ? rc : sys_result(ctx, system_of(ctx, SYS_P), P);
 }
 
 int tse_fast_p_residual(tse_ctx *ctx, double *rel) {
     int rc = check_ctx(ctx);
-    if (rc) return rc;
-    FastState *s = state(ctx);
-    if (!s->ready || !rel) return fail(TSE_ERR_STATE, "fast form not set up");
-    double rn, bn;
-    rc = residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
-    if (rc) return rc;
-    *rel = rn / (bn > 0 ? bn : 1.0);
-    return 0;
+    return rc ? rc : sys_residual(ctx, system_of(ctx, SYS_P), rel);
 }
 
 int tse_fast_p_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
                       int32_t n_sweeps, double sor, int32_t fp32) {
     int rc = check_ctx(ctx);
-    if (rc) return rc;
-    FastState *s = state(ctx);
-    if (!s->ready || !plan) return fail(TSE_ERR_STATE, "fast form not set up");
-    if (!fast_plan_ok(*plan, fp32) || n_sweeps < 0 || n_sweeps % plan->ns)
-        return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
-    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk);
-    if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
-    // FP32 timing path: sweeps on the correction equation with the last r
-    int e;
-    return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
+    return rc ? rc : sys_sweeps(ctx, system_of(ctx, SYS_P), plan, n_sweeps, sor, fp32);
 }
 
 int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
                      tse_fast_result *out) {
     int rc = check_ctx(ctx);
+    return rc ? rc : sys_solve(ctx, system_of(ctx, SYS_P), prm, out);
+}
+
+// Transport: assemble with the faithful record kernels (reference arithmetic,
+// Xn = the unknown at entry), convert to the natural-layout operator, load
+// the unknown as the iterate.  f->Cn / f->Tn receive the entry copy.
+static int tr_setup_common(tse_ctx *ctx, double *X, double omega) {
+    FastSys *s = system_of(ctx, SYS_TR);
+    int rc = ensure(ctx, s);
     if (rc) return rc;
-    FastState *s = state(ctx);
-    if (!s->ready || !prm || !out) return fail(TSE_ERR_STATE, "fast form not set up");
-    const int fp32 = prm->form == TSE_FORM_MIXED;
-    if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
-        return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
-    const tse_sweep_plan &pl = prm->plan;
-    const int ns = pl.ns;
-    const int chunk = prm->inner;
-    if (!fast_plan_ok(pl, fp32) || chunk < ns || chunk % ns || prm->max_sweeps < 0)
-        return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
-    const ZGeo zg = make_zgeo(ctx->geo, ns, pl.by, pl.zchunk);
-    double rn, bn;
-    rc = fp32 ? residual<2>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn)
-              : residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
+    const Geo &G = ctx->geo;
+    hipLaunchKernelGGL(tr_from_records_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
+                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       ctx->rec, ctx->mask, omega, s->trop());
+    TSE_HIP_TRY(hipGetLastError());
+    return load_iterate(ctx, s, X);
+}
+
+int tse_fast_c_setup(tse_ctx *ctx, const tse_c_fields *f, double dt,
+                     double omega) {
+    int rc = check_ctx(ctx);
     if (rc) return rc;
-    const double bs = bn > 0 ? bn : 1.0;
-    out->initial_rel = rn / bs;
-    double rel = rn / bs;
-    int sweeps = 0, cycles = 0;
-    while (rel > prm->tol && sweeps < prm->max_sweeps) {
-        if (fp32) {
-            int e;
-            rc = correction_sweeps(ctx, s, zg, pl, chunk, (float)prm->sor, &e);
-            if (rc) return rc;
-            rc = residual<1>(ctx, s, pbuf(s, s->cur), ebuf(s, e),
-                             pbuf(s, 1 - s->cur), &rn, &bn);
-            if (rc) return rc;
-            s->cur = 1 - s->cur;
-        } else {
-            rc = plain_sweeps(ctx, s, zg, pl, chunk, prm->sor);
-            if (rc) return rc;
-            rc = residual<0>(ctx, s, pbuf(s, s->cur), nullptr, nullptr, &rn, &bn);
-            if (rc) return rc;
-        }
-        sweeps += chunk;
-        cycles += 1;
-        rel = rn / bs;
-        if (!(rel == rel)) break;  // NaN
-    }
-    out->sweeps = sweeps;
-    out->cycles = cycles;
-    out->rel_residual = rel;
-    out->converged = rel <= prm->tol ? 1 : 0;
-    return 0;
+    if (!f || !f->C || !f->Cn || !f->D_fx || !f->D_fy || !f->D_fz || !f->phi ||
+        !f->rho_f || !f->qx || !f->qy || !f->qz || !f->BC || !f->WC ||
+        !f->RC || !f->e || !f->lf_l || !f->lf_t)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    if (!(dt > 0)) return fail(TSE_ERR_ARG, "dt must be > 0");
+    TSE_HIP_TRY(hipMemcpyAsync(f->Cn, f->C, 2ull * ctx->geo.S * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    rc = launch_c_assemble(ctx, f, dt, omega);
+    if (rc) return rc;
+    return tr_setup_common(ctx, f->C, omega);
+}
+
+int tse_fast_t_setup(tse_ctx *ctx, const tse_t_fields *f, double dt,
+                     double omega) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !f->T || !f->Tn || !f->l_f || !f->cp_f || !f->rho_f || !f->phi ||
+        !f->qx || !f->qy || !f->qz || !f->BT || !f->WT || !f->e || !f->l_r ||
+        !f->cp_r || !f->rho_r || !f->lt_l || !f->lt_t)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    if (!(dt > 0)) return fail(TSE_ERR_ARG, "dt must be > 0");
+    TSE_HIP_TRY(hipMemcpyAsync(f->Tn, f->T, 2ull * ctx->geo.S * sizeof(double),
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    rc = launch_t_assemble(ctx, f, dt, omega);
+    if (rc) return rc;
+    return tr_setup_common(ctx, f->T, omega);
+}
+
+int tse_fast_tr_solve(tse_ctx *ctx, const tse_fast_params *prm,
+                      tse_fast_result *out) {
+    int rc = check_ctx(ctx);
+    return rc ? rc : sys_solve(ctx, system_of(ctx, SYS_TR), prm, out);
+}
+
+int tse_fast_tr_result(tse_ctx *ctx, double *X) {
+    int rc = check_ctx(ctx);
+    return rc ? rc : sys_result(ctx, system_of(ctx, SYS_TR), X);
+}
+
+int tse_fast_tr_residual(tse_ctx *ctx, double *rel) {
+    int rc = check_ctx(ctx);
+    return rc ? rc : sys_residual(ctx, system_of(ctx, SYS_TR), rel);
+}
+
+int tse_fast_tr_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
+                       int32_t n_sweeps, double sor, int32_t fp32) {
+    int rc = check_ctx(ctx);
+    return rc ? rc : sys_sweeps(ctx, system_of(ctx, SYS_TR), plan, n_sweeps, sor, fp32);
 }
 
 }  // extern "C"

@@ -104,6 +104,12 @@ __host__ __device__ inline bool is_check_iter(int mode, int it) {
 
 // ---- launch shapes ------------------------------------------------------------------
 constexpr int kBlock = 256;          // 4 wavefronts of 64
+// Per-cell coefficient records of the faithful solvers (tse_faithful.hip):
+// compute_P: A_, C_, E_, G_, I_, K_, A_B_, C_D_, I_J_, K_L_, R*Pn, WP, R+M;
+// transport: a(x+), a(x-), a(y+), a(y-), a(z+), a(z-), X, R*Xn, W, E,
+// R - omega*M.  Each field is a 2*S array in the split layout.
+constexpr int kPRec = 13;
+constexpr int kTRec = 11;
 constexpr int kMaxSweepBlocks = 4096;
 
 inline int sweep_blocks(int S) {
@@ -132,7 +138,8 @@ struct tse_ctx {
     tse::Ctl *ctl_host = nullptr; // pinned host mirror
     double *red_out = nullptr;    // device scratch for tse_l2_norm
     double *red_host = nullptr;   // pinned
-    void *fast = nullptr;         // performance-form buffers (tse_fast.hip)
+    void *fast = nullptr;         // performance-form pressure system (tse_fast.hip)
+    void *fast_tr = nullptr;      // performance-form transport system
 };
 
 namespace tse {

@@ -37,8 +37,11 @@ class SolveStats:
     plan: tuple
 
 
-class FastPressure:
-    """Fast-form pressure solver bound to one grid on one device."""
+class FastSystem:
+    """Shared driver of one performance-form system (``tse_fast_<kind>_*``)."""
+
+    KIND = ""
+    DEFAULT_PLAN = (2, 16, 1, 64)
 
     def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, device=0,
                  hip: HipContext | None = None):
@@ -47,6 +50,76 @@ class FastPressure:
         self.N = self.hip.nx * self.hip.ny * self.hip.nz
         self.d = {}
         self.plans = {}
+
+    def _fn(self, what):
+        return getattr(self.lib, f"tse_fast_{self.KIND}_{what}")
+
+    def assemble(self):  # pragma: no cover - per system
+        raise NotImplementedError
+
+    def plan(self, form: str, tune: bool = True):
+        """Launch plan for the form's sweep precision (autotuned once)."""
+        if form not in self.plans:
+            if tune:
+                from .autotune import tune as _tune
+                if form == "mixed":
+                    # the FP32 timing path needs a residual in place
+                    self.solve("mixed", max_sweeps=0, plan=(1, 16, 1, 0))
+                (ns, by, ry, zc), _ = _tune(self.hip, form == "mixed",
+                                            kind=self.KIND)
+                self.plans[form] = (ns, by, ry, zc)
+                self.assemble()  # restore the iterate the tuning advanced
+            else:
+                self.plans[form] = self.DEFAULT_PLAN
+        return self.plans[form]
+
+    def solve(self, form="mixed", tol=1e-10, sor=1.996, max_sweeps=200000,
+              inner=None, plan=None) -> SolveStats:
+        pl = plan or self.plan(form)
+        ns = pl[0]
+        inner = inner or 100
+        inner = max(ns, (inner // ns) * ns)
+        prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*pl),
+                              inner=inner, max_sweeps=int(max_sweeps),
+                              sor=float(sor), tol=float(tol))
+        res = _lib.FastResult()
+        t0 = time.perf_counter()
+        _lib.check(self._fn("solve")(self.hip.ctx, ctypes.byref(prm),
+                                     ctypes.byref(res)), f"tse_fast_{self.KIND}_solve")
+        sec = time.perf_counter() - t0
+        return SolveStats(form, res.sweeps, res.cycles, bool(res.converged),
+                          res.initial_rel, res.rel_residual, sec, tuple(pl))
+
+    def result_device(self):
+        """Write the current iterate into a split-layout device field (kept
+        apart from the input field, which ``assemble`` reloads)."""
+        if "res" not in self.d:
+            self.d["res"] = self.hip.empty()
+        _lib.check(self._fn("result")(self.hip.ctx, self.d["res"].data_ptr()),
+                   f"tse_fast_{self.KIND}_result")
+        return self.d["res"]
+
+    def result(self, out: np.ndarray | None = None) -> np.ndarray:
+        return self.hip.download(self.result_device(), out=out)
+
+    def residual(self) -> float:
+        rel = ctypes.c_double()
+        _lib.check(self._fn("residual")(self.hip.ctx, ctypes.byref(rel)),
+                   f"tse_fast_{self.KIND}_residual")
+        return rel.value
+
+    def sweeps(self, n, plan, sor=1.996, fp32=True):
+        """Timing hook: n fused sweeps without residual checks."""
+        _lib.check(self._fn("sweeps")(self.hip.ctx,
+                                      ctypes.byref(_lib.SweepPlan(*plan)),
+                                      int(n), float(sor), int(fp32)),
+                   f"tse_fast_{self.KIND}_sweeps")
+
+
+class FastPressure(FastSystem):
+    """Fast-form pressure solver bound to one grid on one device."""
+
+    KIND = "p"
 
     def load(self, P, dt, Pn=None, **fields):
         """Upload P, Pn and the ten compute_P inputs and assemble for dt."""
@@ -62,60 +135,3 @@ class FastPressure:
         self._pf = _lib.PFields(**{n: self.d[n].data_ptr() for n in _lib.P_FIELDS})
         _lib.check(self.lib.tse_fast_p_setup(self.hip.ctx, ctypes.byref(self._pf),
                                              self.dt), "tse_fast_p_setup")
-
-    def plan(self, form: str, tune: bool = True):
-        """Launch plan for the form's sweep precision (autotuned once)."""
-        if form not in self.plans:
-            if tune:
-                from .autotune import tune as _tune
-                if form == "mixed":
-                    # the FP32 timing path needs a residual in place
-                    self.solve("mixed", max_sweeps=0, plan=(1, 16, 1, 0))
-                (ns, by, ry, zc), _ = _tune(self.hip, form == "mixed")
-                self.plans[form] = (ns, by, ry, zc)
-                self.assemble()  # restore the iterate the tuning advanced
-            else:
-                self.plans[form] = (2, 16, 1, 64)
-        return self.plans[form]
-
-    def solve(self, form="mixed", tol=1e-10, sor=1.996, max_sweeps=200000,
-              inner=None, plan=None) -> SolveStats:
-        pl = plan or self.plan(form)
-        ns = pl[0]
-        inner = inner or (100 if form == "mixed" else 100)
-        inner = max(ns, (inner // ns) * ns)
-        prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*pl),
-                              inner=inner, max_sweeps=int(max_sweeps),
-                              sor=float(sor), tol=float(tol))
-        res = _lib.FastResult()
-        t0 = time.perf_counter()
-        _lib.check(self.lib.tse_fast_p_solve(self.hip.ctx, ctypes.byref(prm),
-                                             ctypes.byref(res)), "tse_fast_p_solve")
-        sec = time.perf_counter() - t0
-        return SolveStats(form, res.sweeps, res.cycles, bool(res.converged),
-                          res.initial_rel, res.rel_residual, sec, tuple(pl))
-
-    def result_device(self):
-        """Write the current iterate into a split-layout device field (kept
-        apart from the input P, which ``assemble`` reloads)."""
-        if "Pres" not in self.d:
-            self.d["Pres"] = self.hip.empty()
-        _lib.check(self.lib.tse_fast_p_result(self.hip.ctx, self.d["Pres"].data_ptr()),
-                   "tse_fast_p_result")
-        return self.d["Pres"]
-
-    def result(self, out: np.ndarray | None = None) -> np.ndarray:
-        return self.hip.download(self.result_device(), out=out)
-
-    def residual(self) -> float:
-        rel = ctypes.c_double()
-        _lib.check(self.lib.tse_fast_p_residual(self.hip.ctx, ctypes.byref(rel)),
-                   "tse_fast_p_residual")
-        return rel.value
-
-    def sweeps(self, n, plan, sor=1.996, fp32=True):
-        """Timing hook: n fused sweeps without residual checks."""
-        _lib.check(self.lib.tse_fast_p_sweeps(self.hip.ctx,
-                                              ctypes.byref(_lib.SweepPlan(*plan)),
-                                              int(n), float(sor), int(fp32)),
-                   "tse_fast_p_sweeps")
