@@ -242,6 +242,44 @@ int tse_fast_tr_result(tse_ctx *ctx, double *X);
 int tse_fast_tr_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
                        int32_t n_sweeps, double sor, int32_t fp32);
 
+/* ---- batched performance-form solves ------------------------------------- */
+/* No counterpart in the reference (its runs are one grid per process): the
+ * ensemble and serving workloads solve many independent grids at once.  A
+ * batch holds member contexts whose fast system of one kind (0 = pressure,
+ * set up by tse_fast_p_setup; 1 = transport, by tse_fast_c/t_setup) is ready.
+ * A cycle runs, for every active member, the same `inner` sweeps, residual
+ * and stop test as tse_fast_p_solve / tse_fast_tr_solve run per cycle, so a
+ * member's result does not depend on the other members.  The sweeps of all
+ * members run in one persistent launch per sweep group that takes (member,
+ * tile) work items from an atomic queue.  Members' solves stop at their own
+ * tolerance; a stopped member is skipped until it is re-armed or removed.
+ * Calls are not thread-safe per batch; a member context must not be used by
+ * other fast-form calls of the same kind while it is in a batch (use
+ * tse_batch_writeback first to read its result). */
+typedef struct tse_batch tse_batch;
+int tse_batch_create(tse_batch **out, int32_t device, void *stream);
+int tse_batch_destroy(tse_batch *b);
+/* Sets kind, form, plan, inner, max_sweeps, sor, tol; the batch must be empty. */
+int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm);
+/* Adds a member and evaluates its initial residual; returns its slot (>= 0). */
+int tse_batch_add(tse_batch *b, tse_ctx *member);
+/* Restarts a member after its fast system was set up again (next solve). */
+int tse_batch_rearm(tse_batch *b, int32_t slot);
+/* Hands the member's iterate back to its context (tse_fast_*_result). */
+int tse_batch_writeback(tse_batch *b, int32_t slot);
+/* Writes back and removes a member. */
+int tse_batch_remove(tse_batch *b, int32_t slot);
+/* One cycle for all active members; *n_active = members still active.
+ * use_graph != 0 replays a hipGraph of the cycle, captured on first use and
+ * kept while the member set is unchanged (the stream must not be the null
+ * stream). */
+int tse_batch_cycle(tse_batch *b, int32_t use_graph, int32_t *n_active);
+/* Cycles until no member is active; *cycles = cycles run. */
+int tse_batch_run(tse_batch *b, int32_t use_graph, int32_t *cycles);
+/* Member state: sweeps, cycles, converged, pad = still active, initial and
+ * last relative residual. */
+int tse_batch_status(tse_batch *b, int32_t slot, tse_fast_result *out);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
  * fixed dt (no norm, no control), for timing and sweep-level parity.

@@ -115,6 +115,16 @@ SIGNATURES = {
     "tse_fast_tr_result": (_I, [_V, _dp]),
     "tse_fast_tr_sweeps": (_I, [_V, ctypes.POINTER(SweepPlan), ctypes.c_int32,
                                 _D, ctypes.c_int32]),
+    "tse_batch_create": (_I, [ctypes.POINTER(_V), ctypes.c_int32, _V]),
+    "tse_batch_destroy": (_I, [_V]),
+    "tse_batch_begin": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastParams)]),
+    "tse_batch_add": (_I, [_V, _V]),
+    "tse_batch_rearm": (_I, [_V, ctypes.c_int32]),
+    "tse_batch_writeback": (_I, [_V, ctypes.c_int32]),
+    "tse_batch_remove": (_I, [_V, ctypes.c_int32]),
+    "tse_batch_cycle": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "tse_batch_run": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
+    "tse_batch_status": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastResult)]),
 }
 
 _lib = None

@@ -32,6 +32,7 @@ UNITS = {
     "tse_core.hip": ["-ffp-contract=off"],
     "tse_faithful.hip": ["-ffp-contract=off"],
     "tse_fast.hip": ["-ffp-contract=fast"],
+    "tse_batch.hip": ["-ffp-contract=fast"],
 }
 
 

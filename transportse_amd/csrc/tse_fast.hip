@@ -28,24 +28,9 @@
 #include <cmath>
 #include <vector>
 
-#include "tse_internal.h"
+#include "tse_fast.h"
 
 namespace tse {
-
-struct ZGeo {
-    int nx, ny, nz;
-    int tiles_x, tiles_y, zchunk, nzc;
-    int ntiles, per_xcd;
-    FastDiv div_tx, div_ty;
-};
-
-// Blocks are dispatched round-robin over the 8 XCDs; map block b to tile
-// (b % 8) * per_xcd + b / 8 so each XCD (own L2) works on a contiguous band of
-// tiles and shares their halos.
-__device__ inline int xcd_tile(const ZGeo &g) {
-    const int b = blockIdx.x;
-    return (b & 7) * g.per_xcd + (b >> 3);
-}
 
 template <typename T, int NS, int BYT, int RY>
 __global__ void __launch_bounds__(64 * BYT)
@@ -53,143 +38,10 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                  const T *__restrict__ Wx, const T *__restrict__ Wy,
                  const T *__restrict__ Wz, const T *__restrict__ invD,
                  const T *__restrict__ rhs, T omega, int zero_init) {
-    constexpr int R = 2 * NS;          // halo ring width / number of phases
-    constexpr int BX = 64;
-    constexpr int BY = BYT * RY;       // tile rows; thread rows ty + q*BYT
-    constexpr int TX = BX - 2 * R;
-    constexpr int TY = BY - 2 * R;
-    // LDS ring of z-planes: the plane stored at step k and the R planes in
-    // phases (k-1 .. k-R), plus one spare so the slot overwritten at step k
-    // was last read at step k-2.
-    constexpr int NSLOT = R + 2;
-    static_assert(TY >= 1, "tile too small");
-    __shared__ T sP[NSLOT][BY][BX];
-    __shared__ T sWx[NSLOT][BY][BX];
-    __shared__ T sWy[NSLOT][BY][BX];
-
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;  // uniform per block
-    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);   // tile / tiles_x
-    const int tx = tile - zc * g.tiles_x;
-    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);    // / tiles_y
-    const int ty = zc - zc2 * g.tiles_y;
-    const int lx = threadIdx.x;
-    const int i = tx * TX - R + lx;
-    const int k0 = zc2 * g.zchunk;
-    const int k1 = min(k0 + g.zchunk, g.nz);
-    const int kl0 = max(k0 - R, 0);
-    const int kl1 = min(k1 + R, g.nz) - 1;
-    const long plane = (long)g.nx * g.ny;
-    const int dxe = min(lx, BX - 1 - lx);
-
-    int ly[RY], d[RY], par[RY];
-    bool in_dom[RY];
-    long col[RY];
-    T p[RY][R + 2], rinv[RY][R + 1], rb[RY][R + 1], rwz[RY][R + 2];
-    T np[RY], ni[RY], nb[RY], nwz[RY], nwx[RY], nwy[RY];
-#pragma unroll
-    for (int q = 0; q < RY; ++q) {
-        ly[q] = threadIdx.y + q * BYT;
-        const int j = ty * TY - R + ly[q];
-        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-        d[q] = min(dxe, min(ly[q], BY - 1 - ly[q]));
-        col[q] = in_dom[q] ? (long)i + (long)g.nx * j : 0;
-        par[q] = (i + j) & 1;
-#pragma unroll
-        for (int r = 0; r < R + 2; ++r) {
-            p[q][r] = T(0);
-            rwz[q][r] = T(0);
-        }
-#pragma unroll
-        for (int r = 0; r < R + 1; ++r) {
-            rinv[q][r] = T(0);
-            rb[q][r] = T(0);
-        }
-        // operands of the next plane, loaded one step ahead
-        np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
-        if (in_dom[q] && kl0 <= kl1) {
-            const long o = col[q] + plane * kl0;
-            np[q] = zero_init ? T(0) : Pin[o];
-            ni[q] = invD[o];
-            nb[q] = rhs[o];
-            nwz[q] = Wz[o];
-            nwx[q] = Wx[o];
-            nwy[q] = Wy[o];
-        }
-    }
-
-    int s0 = kl0 % NSLOT;
-    for (int k = kl0; k <= k1 + R - 1; ++k) {
-#pragma unroll
-        for (int q = 0; q < RY; ++q) {
-            const T p0 = np[q], i0 = ni[q], b0 = nb[q], wz0 = nwz[q];
-            const T wx0 = nwx[q], wy0 = nwy[q];
-            np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
-            if (in_dom[q] && k + 1 <= kl1) {
-                const long o = col[q] + plane * (k + 1);
-                np[q] = zero_init ? T(0) : Pin[o];
-                ni[q] = invD[o];
-                nb[q] = rhs[o];
-                nwz[q] = Wz[o];
-                nwx[q] = Wx[o];
-                nwy[q] = Wy[o];
-            }
-#pragma unroll
-            for (int r = R + 1; r > 0; --r) {
-                p[q][r] = p[q][r - 1];
-                rwz[q][r] = rwz[q][r - 1];
-            }
-#pragma unroll
-            for (int r = R; r > 0; --r) {
-                rinv[q][r] = rinv[q][r - 1];
-                rb[q][r] = rb[q][r - 1];
-            }
-            p[q][0] = p0;
-            rwz[q][0] = wz0;
-            rinv[q][0] = i0;
-            rb[q][0] = b0;
-            sP[s0][ly[q]][lx] = p0;
-            sWx[s0][ly[q]][lx] = wx0;
-            sWy[s0][ly[q]][lx] = wy0;
-        }
-        // One barrier per step: phase h reads in-plane values of plane
-        // k-1-h written by phase h-1 during step k-1 (or stored at step
-        // k-1 for h = 0) and writes only its own colour; its z neighbours
-        // are this thread's registers.
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < R; ++h) {
-            const int m = k - 1 - h;
-            int s = s0 - 1 - h;
-            if (s < 0) s += NSLOT;
-            const bool mok = m >= kl0 && m <= kl1;
-#pragma unroll
-            for (int q = 0; q < RY; ++q) {
-                if (mok && d[q] >= h + 1 && in_dom[q] &&
-                    ((par[q] + m) & 1) == (h & 1) && rinv[q][1 + h] > T(0)) {
-                    const int y = ly[q];
-                    T acc = rb[q][1 + h];
-                    acc += sWx[s][y][lx] * sP[s][y][lx + 1];
-                    acc += sWx[s][y][lx - 1] * sP[s][y][lx - 1];
-                    acc += sWy[s][y][lx] * sP[s][y + 1][lx];
-                    acc += sWy[s][y - 1][lx] * sP[s][y - 1][lx];
-                    acc += rwz[q][1 + h] * p[q][h];
-                    acc += rwz[q][2 + h] * p[q][h + 2];
-                    const T pc = p[q][1 + h];
-                    const T pn = pc + omega * (acc * rinv[q][1 + h] - pc);
-                    p[q][1 + h] = pn;
-                    sP[s][y][lx] = pn;
-                }
-            }
-        }
-        const int mo = k - R;
-        if (mo >= k0 && mo < k1) {
-#pragma unroll
-            for (int q = 0; q < RY; ++q)
-                if (d[q] >= R && in_dom[q]) Pout[col[q] + plane * mo] = p[q][R];
-        }
-        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
-    }
+    rb_zmarch_tile<T, NS, BYT, RY>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs,
+                                   omega, zero_init);
 }
 
 // ---- operator assembly (from split-layout fields) ---------------------------------------
@@ -239,11 +91,6 @@ __device__ inline double face_G(const Geo &G, const double *rho, int i, int j,
     return f_amean(G, rho, i, j, k + 1, i, j, k, 2) * G.g * 0.5 *
            (G.dz[k + 1] + G.dz[k]) * G.cosb;
 }
-
-struct FastOp {  // natural-layout operator arrays
-    double *Wx, *Wy, *Wz, *D, *invD, *b;
-    float *Wx32, *Wy32, *Wz32, *invD32;
-};
 
 __global__ void __launch_bounds__(kBlock)
 fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny,
@@ -334,15 +181,6 @@ nat_to_split_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ 
 
 // ---- FP64 residual r = b - A p (and p <- p + e) ------------------------------------------------
 
-__device__ inline double wsum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-// mode 0: check only (||r||, ||b||) on p_in.
-// mode 1: defect correction: p_out = p_in + e; r = b - A p_out written as FP32.
-// mode 2: r = b - A p_in written as FP32 (first correction cycle).
 template <int MODE>
 __global__ void __launch_bounds__(kBlock)
 residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
@@ -350,7 +188,6 @@ residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
                 double *__restrict__ pout, FastOp op, float *__restrict__ r32,
                 double *__restrict__ partials) {
     const long N = (long)nx * ny * nz;
-    const long plane = (long)nx * ny;
     double sr = 0.0, sb = 0.0;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
@@ -358,43 +195,9 @@ residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
         const int i = (int)t - row * nx;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * ny;
-        auto P = [&](long o) -> double {
-            return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
-        };
-        const double pc = P(t);
-        if (MODE == 1) pout[t] = pc;
-        double r = 0.0;
-        if (op.invD[t] > 0.0) {
-            double acc = op.b[t];
-            if (i < nx - 1) acc += op.Wx[t] * P(t + 1);
-            if (i > 0) acc += op.Wx[t - 1] * P(t - 1);
-            if (j < ny - 1) acc += op.Wy[t] * P(t + nx);
-            if (j > 0) acc += op.Wy[t - nx] * P(t - nx);
-            if (k < nz - 1) acc += op.Wz[t] * P(t + plane);
-            if (k > 0) acc += op.Wz[t - plane] * P(t - plane);
-            r = acc - op.D[t] * pc;
-            sr += r * r;
-            sb += op.b[t] * op.b[t];
-        }
-        if (MODE != 0) r32[t] = (float)r;
+        p_residual_cell<MODE>(nx, ny, nz, i, j, k, t, pin, e, pout, op, r32, sr, sb);
     }
-    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
-    sr = wsum(sr);
-    sb = wsum(sb);
-    if ((threadIdx.x & 63) == 0) {
-        s1[threadIdx.x >> 6] = sr;
-        s2[threadIdx.x >> 6] = sb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0;
-        for (int q = 0; q < kBlock / 64; ++q) {
-            a += s1[q];
-            c += s2[q];
-        }
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = c;
-    }
+    block_sum2_to(sr, sb, partials + 2 * blockIdx.x);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -404,23 +207,7 @@ reduce2_kernel(const double *__restrict__ partials, int n, double *__restrict__ 
         a += partials[2 * q];
         c += partials[2 * q + 1];
     }
-    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
-    a = wsum(a);
-    c = wsum(c);
-    if ((threadIdx.x & 63) == 0) {
-        s1[threadIdx.x >> 6] = a;
-        s2[threadIdx.x >> 6] = c;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double x = 0, y = 0;
-        for (int q = 0; q < kBlock / 64; ++q) {
-            x += s1[q];
-            y += s2[q];
-        }
-        out[0] = x;
-        out[1] = y;
-    }
+    block_sum2_to(a, c, out);
 }
 
 // ---- transport systems (compute_C / compute_T at their fixed point) -------------------------
@@ -431,12 +218,6 @@ reduce2_kernel(const double *__restrict__ partials, int n, double *__restrict__ 
 //     (R - omega M) x_c - sum_f omega a_f x_f = X + R Xn + W - E,
 // non-symmetric (upwind), so each cell keeps its own six coefficients
 // c_f = omega * a_f (order x+, x-, y+, y-, z+, z-), 1/diagonal and rhs.
-
-struct TrOp {  // natural layout, coefficient f of cell t at c[f*N + t]
-    double *c, *D, *invD, *b;
-    float *c32, *invD32;
-    long N;
-};
 
 __global__ void __launch_bounds__(kBlock)
 tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
@@ -468,121 +249,15 @@ tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
     }
 }
 
-// Fused red/black z-march for the non-symmetric transport operator.  Same
-// pipeline as rb_zmarch_kernel (one barrier per plane step, NS sweeps per
-// launch, 2*NS halo ring, ping-pong); the coefficients are the cell's own,
-// so only the unknown goes through LDS.
 template <typename T, int NS, int BY>
 __global__ void __launch_bounds__(64 * BY)
 tr_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                  const T *__restrict__ coef, long N, const T *__restrict__ invD,
                  const T *__restrict__ rhs, T omega, int zero_init) {
-    constexpr int R = 2 * NS;
-    constexpr int BX = 64;
-    constexpr int TX = BX - 2 * R;
-    constexpr int TY = BY - 2 * R;
-    constexpr int NSLOT = R + 2;
-    static_assert(TY >= 1, "tile too small");
-    __shared__ T sP[NSLOT][BY][BX];
-
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;
-    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
-    const int tx = tile - zc * g.tiles_x;
-    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
-    const int ty = zc - zc2 * g.tiles_y;
-    const int lx = threadIdx.x, ly = threadIdx.y;
-    const int i = tx * TX - R + lx;
-    const int j = ty * TY - R + ly;
-    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
-    const int k0 = zc2 * g.zchunk;
-    const int k1 = min(k0 + g.zchunk, g.nz);
-    const int kl0 = max(k0 - R, 0);
-    const int kl1 = min(k1 + R, g.nz) - 1;
-    const long plane = (long)g.nx * g.ny;
-    const long col = in_dom ? (long)i + (long)g.nx * j : 0;
-    const int par = (i + j) & 1;
-
-    T p[R + 2], rinv[R + 1], rb[R + 1], rc[6][R + 1];
-#pragma unroll
-    for (int q = 0; q < R + 2; ++q) p[q] = T(0);
-#pragma unroll
-    for (int q = 0; q < R + 1; ++q) {
-        rinv[q] = T(0);
-        rb[q] = T(0);
-#pragma unroll
-        for (int f = 0; f < 6; ++f) rc[f][q] = T(0);
-    }
-    T np = T(0), ni = T(0), nb = T(0), nc[6];
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nc[f] = T(0);
-    if (in_dom && kl0 <= kl1) {
-        const long o = col + plane * kl0;
-        np = zero_init ? T(0) : Pin[o];
-        ni = invD[o];
-        nb = rhs[o];
-#pragma unroll
-        for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
-    }
-    int s0 = kl0 % NSLOT;
-    for (int k = kl0; k <= k1 + R - 1; ++k) {
-        const T p0 = np, i0 = ni, b0 = nb;
-        T c0[6];
-#pragma unroll
-        for (int f = 0; f < 6; ++f) {
-            c0[f] = nc[f];
-            nc[f] = T(0);
-        }
-        np = ni = nb = T(0);
-        if (in_dom && k + 1 <= kl1) {
-            const long o = col + plane * (k + 1);
-            np = zero_init ? T(0) : Pin[o];
-            ni = invD[o];
-            nb = rhs[o];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
-        }
-#pragma unroll
-        for (int q = R + 1; q > 0; --q) p[q] = p[q - 1];
-#pragma unroll
-        for (int q = R; q > 0; --q) {
-            rinv[q] = rinv[q - 1];
-            rb[q] = rb[q - 1];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) rc[f][q] = rc[f][q - 1];
-        }
-        p[0] = p0;
-        rinv[0] = i0;
-        rb[0] = b0;
-#pragma unroll
-        for (int f = 0; f < 6; ++f) rc[f][0] = c0[f];
-        sP[s0][ly][lx] = p0;
-        __syncthreads();
-#pragma unroll
-        for (int h = 0; h < R; ++h) {
-            const int m = k - 1 - h;
-            int s = s0 - 1 - h;
-            if (s < 0) s += NSLOT;
-            if (m >= kl0 && m <= kl1 && d >= h + 1 && in_dom &&
-                ((par + m) & 1) == (h & 1) && rinv[1 + h] != T(0)) {
-                T acc = rb[1 + h];
-                acc += rc[0][1 + h] * sP[s][ly][lx + 1];
-                acc += rc[1][1 + h] * sP[s][ly][lx - 1];
-                acc += rc[2][1 + h] * sP[s][ly + 1][lx];
-                acc += rc[3][1 + h] * sP[s][ly - 1][lx];
-                acc += rc[4][1 + h] * p[h];
-                acc += rc[5][1 + h] * p[h + 2];
-                const T pc = p[1 + h];
-                const T pn = pc + omega * (acc * rinv[1 + h] - pc);
-                p[1 + h] = pn;
-                sP[s][ly][lx] = pn;
-            }
-        }
-        const int mo = k - R;
-        if (mo >= k0 && mo < k1 && d >= R && in_dom) Pout[col + plane * mo] = p[R];
-        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
-    }
+    tr_zmarch_tile<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
+                              zero_init);
 }
 
 template <int MODE>
@@ -592,7 +267,6 @@ tr_residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
                    double *__restrict__ pout, TrOp op, float *__restrict__ r32,
                    double *__restrict__ partials) {
     const long N = op.N;
-    const long plane = (long)nx * ny;
     double sr = 0.0, sb = 0.0;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
@@ -600,96 +274,14 @@ tr_residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
         const int i = (int)t - row * nx;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * ny;
-        auto X = [&](long o) -> double {
-            return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
-        };
-        const double xc = X(t);
-        if (MODE == 1) pout[t] = xc;
-        double r = 0.0;
-        if (op.invD[t] != 0.0) {
-            double acc = op.b[t];
-            if (i < nx - 1) acc += op.c[0 * N + t] * X(t + 1);
-            if (i > 0) acc += op.c[1 * N + t] * X(t - 1);
-            if (j < ny - 1) acc += op.c[2 * N + t] * X(t + nx);
-            if (j > 0) acc += op.c[3 * N + t] * X(t - nx);
-            if (k < nz - 1) acc += op.c[4 * N + t] * X(t + plane);
-            if (k > 0) acc += op.c[5 * N + t] * X(t - plane);
-            r = acc - op.D[t] * xc;
-            sr += r * r;
-            sb += op.b[t] * op.b[t];
-        }
-        if (MODE != 0) r32[t] = (float)r;
+        tr_residual_cell<MODE>(nx, ny, nz, i, j, k, t, pin, e, pout, op, r32, sr, sb);
     }
-    __shared__ double s1[kBlock / 64], s2[kBlock / 64];
-    sr = wsum(sr);
-    sb = wsum(sb);
-    if ((threadIdx.x & 63) == 0) {
-        s1[threadIdx.x >> 6] = sr;
-        s2[threadIdx.x >> 6] = sb;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0;
-        for (int q = 0; q < kBlock / 64; ++q) {
-            a += s1[q];
-            c += s2[q];
-        }
-        partials[2 * blockIdx.x] = a;
-        partials[2 * blockIdx.x + 1] = c;
-    }
+    block_sum2_to(sr, sb, partials + 2 * blockIdx.x);
 }
 
 // ---- host side ---------------------------------------------------------------------------------------
 
-enum SysKind { SYS_P = 0, SYS_TR = 1 };
-
-// One linear system in the performance form, owned by the context.
-//   pressure:  buf64 = Wx Wy Wz D invD b | p0 p1      buf32 = Wx Wy Wz invD r | e0 e1
-//   transport: buf64 = c[6] D invD b | p0 p1          buf32 = c[6] invD r | e0 e1
-struct FastSys {
-    int kind = SYS_P;
-    long N = 0;
-    double *buf64 = nullptr;
-    float *buf32 = nullptr;
-    double *partials = nullptr;
-    double *red = nullptr;
-    double *red_host = nullptr;
-    int cur = 0;          // which of p0, p1 holds the iterate
-    bool ready = false;
-
-    int n64() const { return kind == SYS_P ? 6 : 9; }
-    int n32() const { return kind == SYS_P ? 5 : 8; }
-    double *p(int q) const { return buf64 + (n64() + q) * N; }
-    float *r32() const { return buf32 + (n32() - 1) * N; }
-    float *e(int q) const { return buf32 + (n32() + q) * N; }
-    FastOp pop() const {
-        FastOp op;
-        op.Wx = buf64;
-        op.Wy = buf64 + N;
-        op.Wz = buf64 + 2 * N;
-        op.D = buf64 + 3 * N;
-        op.invD = buf64 + 4 * N;
-        op.b = buf64 + 5 * N;
-        op.Wx32 = buf32;
-        op.Wy32 = buf32 + N;
-        op.Wz32 = buf32 + 2 * N;
-        op.invD32 = buf32 + 3 * N;
-        return op;
-    }
-    TrOp trop() const {
-        TrOp op;
-        op.c = buf64;
-        op.D = buf64 + 6 * N;
-        op.invD = buf64 + 7 * N;
-        op.b = buf64 + 8 * N;
-        op.c32 = buf32;
-        op.invD32 = buf32 + 6 * N;
-        op.N = N;
-        return op;
-    }
-};
-
-static FastSys *system_of(tse_ctx *ctx, int kind) {
+FastSys *system_of(tse_ctx *ctx, int kind) {
     void *&slot_ = kind == SYS_P ? ctx->fast : ctx->fast_tr;
     if (!slot_) {
         FastSys *s = new FastSys();
@@ -716,7 +308,7 @@ void fast_free(tse_ctx *ctx) {
     free_sys(ctx->fast_tr);
 }
 
-static int ensure(tse_ctx *ctx, FastSys *s) {
+int ensure_sys(tse_ctx *ctx, FastSys *s) {
     const Geo &G = ctx->geo;
     const long N = (long)G.nx * G.ny * G.nz;
     if (!s->partials) {
@@ -825,7 +417,7 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
     return fail(TSE_ERR_ARG, "unsupported transport sweep plan (ns, by)");
 }
 
-static bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
+bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     if (kind == SYS_TR)
         return pl.ry == 1 && ((pl.ns == 1 && (pl.by == 8 || pl.by == 16)) ||
                               (pl.ns == 2 && pl.by == 16) ||
@@ -840,7 +432,7 @@ static bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     return false;
 }
 
-static ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
+ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
     ZGeo z{};
     z.nx = G.nx;
     z.ny = G.ny;
@@ -1007,7 +599,7 @@ int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt) {
         return fail(TSE_ERR_ARG, "null field pointer");
     if (!(dt > 0)) return fail(TSE_ERR_ARG, "dt must be > 0");
     FastSys *s = system_of(ctx, SYS_P);
-    rc = ensure(ctx, s);
+    rc = ensure_sys(ctx, s);
     if (rc) return rc;
     const Geo &G = ctx->geo;
     const double *Pn = f->Pn ? f->Pn : f->P;
@@ -1046,7 +638,7 @@ int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
 // the unknown as the iterate.  f->Cn / f->Tn receive the entry copy.
 static int tr_setup_common(tse_ctx *ctx, double *X, double omega) {
     FastSys *s = system_of(ctx, SYS_TR);
-    int rc = ensure(ctx, s);
+    int rc = ensure_sys(ctx, s);
     if (rc) return rc;
     const Geo &G = ctx->geo;
     hipLaunchKernelGGL(tr_from_records_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
