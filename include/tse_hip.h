@@ -280,6 +280,22 @@ int tse_batch_run(tse_batch *b, int32_t use_graph, int32_t *cycles);
  * last relative residual. */
 int tse_batch_status(tse_batch *b, int32_t slot, tse_fast_result *out);
 
+/* ---- device-resident SIA step pieces ------------------------------------ */
+/* Closed-form linear fluid properties of the Picard update (the
+ * fluid_properties callback of tse.py:178-229, 1422-1423, for configs whose
+ * EOS is linear): rho_f = rho0 + drho_dc * C and, if l_f is not NULL,
+ * l_f = lf0 * (1 + dlf_dt * (T - t_ref)).  C is the salt species field. */
+typedef struct {
+    double rho0, drho_dc, lf0, dlf_dt, t_ref;
+} tse_eos_params;
+int tse_linear_eos(tse_ctx *ctx, const double *C, const double *T,
+                   double *rho_f, double *l_f, const tse_eos_params *p);
+/* FP64 mass-balance sums (the mass_balance callback, tse.py:1461-1463, of
+ * the benchmark configs): out3 = [sum C phi V, sum rho_f phi V, outflow
+ * rate of C through the two x faces (upwind, face area dy*dz)]. */
+int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
+                  const double *phi, const double *qx, double *out3);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
  * fixed dt (no norm, no control), for timing and sweep-level parity.

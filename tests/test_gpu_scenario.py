@@ -1,0 +1,119 @@
+"""GPU: device-resident SIA steps on the performance forms (scenario engine).
+
+The oracle runs the same SIA steps with the reference functions (tse.py
+order: compute_P, compute_q, flow_error, compute_C per species, compute_T,
+fluid properties, density norm, reset on a rejected Picard iteration) at
+fixed dt with update-norm tolerances near machine precision, so its result
+is the exact FP64 fixed point of each solve.  The scenario engine solves the
+same systems to a relative residual; every field must agree within what the
+residual target leaves:
+  * plain FP64 to 1e-12: P within 1e-11, C and T within 1e-9 (relative L2),
+  * FP32-in-FP64 to 1e-10: P within 1e-9, C and T within 1e-7.
+The linear EOS and the mass sums run on the device and must equal the same
+expressions evaluated by numpy on the downloaded fields.
+"""
+import types
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _spec():
+    from transportse_amd.scenario import ScenarioSpec
+    return ScenarioSpec(24, 20, 16, spacing=5.0, beta=3.0, sigma=1.5, corr=4,
+                        seed=7, species=2, heat=True, salt_patch=3,
+                        dt=43200.0, sia_maxit=2, sia_conv=1e-12)
+
+
+def _oracle_steps(spec, nsteps):
+    from oracle import Oracle
+    from transportse_amd.scenario import EOS, UNIFORM, host_fields
+    hf = host_fields(spec)
+    shp = (spec.nx, spec.ny, spec.nz)
+    U = {k: np.full(shp, v) for k, v in UNIFORM.items()}
+    disp_l, disp_t = np.full(shp, spec.disp[0]), np.full(shp, spec.disp[1])
+    frozen = types.SimpleNamespace(c_r=U["c_r"], l_r=U["l_r"], cp_r=U["cp_r"],
+                                   rho_r=U["rho_r"], lf_l=disp_l, lf_t=disp_t,
+                                   lt_l=disp_l, lt_t=disp_t)
+    d = np.full(spec.nx, spec.spacing)
+    o = Oracle(spec.nx, spec.ny, spec.nz, d, np.full(spec.ny, spec.spacing),
+               np.full(spec.nz, spec.spacing), 9.80665, spec.beta,
+               frozen_source=frozen, dt_max=spec.dt, c_imblflo=spec.imblflo,
+               t_imblflo=spec.imblflo)
+    K, P, T = hf["K"], hf["P"].copy(), hf["T"].copy()
+    C = [c.copy() for c in hf["C"]]
+    rho, l_f = hf["rho_f"].copy(), hf["l_f"].copy()
+    rho_prev = np.zeros(shp)
+    qx, qy, qz = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+    dt = spec.dt
+    for _ in range(nsteps):
+        Pn0, Cn0, Tn0 = P.copy(), [c.copy() for c in C], T.copy()
+        for it in range(spec.sia_maxit):
+            r = o.compute_P(P, U["c_f"], U["nue"], rho, U["phi"], K, K, K, qx, qy,
+                            qz, dt, hf["BP"], U["zero"], SOR=1.8, max_it=10 ** 6,
+                            inc_it=1.25, max_dt=dt, P_conv=1e-15)
+            assert r[4]
+            P, Pn = r[2], r[3]
+            o.compute_q(P, hf["BP"], K, K, K, qx, qy, qz, U["nue"], rho)
+            e = o.flow_error(P, Pn, qx, qy, qz, U["zero"], U["phi"], U["c_f"],
+                             U["c_r"], rho, dt)
+            for s in range(spec.species):
+                r = o.compute_C(C[s], U["D_f"], U["D_f"], U["D_f"], U["phi"], rho,
+                                qx, qy, qz, dt, hf["BC"], U["zero"], U["R"], e,
+                                omega=spec.omega, SOR=1.0, max_it=10 ** 5,
+                                max_dt=dt, C_conv=1e-15)
+                assert r[3]
+                C[s] = r[2]
+            r = o.compute_T(T, l_f, U["cp_f"], rho, U["phi"], qx, qy, qz, dt,
+                            hf["BT"], U["zero"], e, omega=spec.omega, SOR=1.0,
+                            max_it=10 ** 5, max_dt=dt, T_conv=1e-15)
+            assert r[3]
+            T = r[2]
+            rho = EOS["rho0"] + EOS["drho_dc"] * C[0]
+            l_f = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
+            diff = o.l2_norm(rho, rho_prev)
+            if diff < spec.sia_conv or it == spec.sia_maxit - 1:
+                break
+            rho_prev = rho.copy()
+            P, T, C = Pn0.copy(), Tn0.copy(), [c.copy() for c in Cn0]
+    return {"P": P, "T": T, "C": C, "rho_f": rho, "l_f": l_f}
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+@pytest.mark.parametrize("form,tol,bp,bx", [("plain64", 1e-12, 1e-11, 1e-9),
+                                            ("mixed", 1e-10, 1e-9, 1e-7)])
+def test_scenario_steps_match_oracle(form, tol, bp, bx):
+    from transportse_amd.scenario import DeviceScenario
+    spec = _spec()
+    ref = _oracle_steps(spec, 2)
+    scn = DeviceScenario(spec)
+    scn.run(2, form=form, tol=tol, plan_p=(2, 16, 1, 0), plan_tr=(2, 16, 1, 0),
+            sor_p=1.9, sor_tr=1.2, inner_p=20, inner_tr=4)
+    got = scn.download()
+    errs = {"P": _rel(got["P"], ref["P"]), "T": _rel(got["T"], ref["T"])}
+    for s in range(spec.species):
+        errs[f"C{s}"] = _rel(got["C"][s], ref["C"][s])
+    print(form, "relative L2 error vs oracle:", errs,
+          "SIA iterations", [h.sia_iters for h in scn.history], flush=True)
+    assert errs["P"] <= bp
+    for k, v in errs.items():
+        assert v <= bx, k
+    # device EOS = numpy on the downloaded fields, bit for bit
+    assert np.array_equal(got["rho_f"], 1000.0 + 700.0 * got["C"][0])
+    # device mass sums = numpy on the downloaded fields
+    m = scn.history[-1].mass
+    qx = scn.hip.download(scn.qx)
+    V = spec.spacing ** 3
+    area = spec.spacing ** 2
+    C0 = got["C"][0]
+    exp = (float(np.sum(C0 * 0.2 * V)), float(np.sum(got["rho_f"] * 0.2 * V)),
+           float(np.sum(np.maximum(qx[-2], 0.0) * C0[-2] * area)
+                 + np.sum(np.maximum(-qx[0], 0.0) * C0[1] * area)))
+    for a, b in zip(m, exp):
+        assert abs(a - b) <= 1e-12 * max(abs(b), 1e-300)
+    scn.close()

@@ -61,6 +61,12 @@ class FastResult(ctypes.Structure):
                 ("rel_residual", ctypes.c_double)]
 
 
+class EosParams(ctypes.Structure):
+    _fields_ = [("rho0", ctypes.c_double), ("drho_dc", ctypes.c_double),
+                ("lf0", ctypes.c_double), ("dlf_dt", ctypes.c_double),
+                ("t_ref", ctypes.c_double)]
+
+
 class PFields(ctypes.Structure):
     _fields_ = [(n, _dp) for n in P_FIELDS]
 
@@ -125,6 +131,8 @@ SIGNATURES = {
     "tse_batch_cycle": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_batch_run": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_batch_status": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastResult)]),
+    "tse_linear_eos": (_I, [_V, _dp, _dp, _dp, _dp, ctypes.POINTER(EosParams)]),
+    "tse_mass_sums": (_I, [_V, _dp, _dp, _dp, _dp, _DPH]),
 }
 
 _lib = None

@@ -33,6 +33,7 @@ UNITS = {
     "tse_faithful.hip": ["-ffp-contract=off"],
     "tse_fast.hip": ["-ffp-contract=fast"],
     "tse_batch.hip": ["-ffp-contract=fast"],
+    "tse_step.hip": ["-ffp-contract=off"],
 }
 
 

@@ -226,8 +226,8 @@ int tse_ctx_create(tse_ctx **out, int device, void *stream) {
         hipHostMalloc((void **)&ctx->ctl_host, sizeof(Ctl)) != hipSuccess ||
         hipMalloc((void **)&ctx->partials,
                   sizeof(double) * 2 * 2 * kMaxSweepBlocks) != hipSuccess ||
-        hipMalloc((void **)&ctx->red_out, 2 * sizeof(double)) != hipSuccess ||
-        hipHostMalloc((void **)&ctx->red_host, 2 * sizeof(double)) !=
+        hipMalloc((void **)&ctx->red_out, 4 * sizeof(double)) != hipSuccess ||
+        hipHostMalloc((void **)&ctx->red_host, 4 * sizeof(double)) !=
             hipSuccess) {
         tse_ctx_destroy(ctx);
         return fail(TSE_ERR_NOMEM, "context allocation failed");

@@ -136,7 +136,7 @@ struct tse_ctx {
     double *partials = nullptr;  // [2 colours][kMaxSweepBlocks][2]
     tse::Ctl *ctl = nullptr;      // device
     tse::Ctl *ctl_host = nullptr; // pinned host mirror
-    double *red_out = nullptr;    // device scratch for tse_l2_norm
+    double *red_out = nullptr;    // device scratch (4 doubles) for reductions
     double *red_host = nullptr;   // pinned
     void *fast = nullptr;         // performance-form pressure system (tse_fast.hip)
     void *fast_tr = nullptr;      // performance-form transport system

@@ -1,0 +1,331 @@
+"""Device-resident scenarios: the SIA time step on the performance forms.
+
+A scenario is one TRANSPORTSE run (grid, permeability realization, boundary
+values) whose fields stay in HBM across time steps.  ``DeviceScenario.steps``
+is the SIA step of tse.py:1181-1494 as a generator: between solves it runs
+the device pieces (compute_q, flow_error, the linear EOS, the SIA density
+norm, the resets of P/C/T on a rejected Picard iteration, mass sums) and at
+each solve it yields ``(kind, system)`` -- ``kind`` 0 for the pressure
+system, 1 for a transport system -- to whoever drives it: a plain loop that
+calls ``system.solve``, a lockstep ensemble that batches the same stage of
+many scenarios, or a serving loop that batches whatever stage each request
+is in.
+
+Solver form: the pressure and transport systems are solved to a relative
+residual (FP32 sweeps in FP64 defect correction, or plain FP64) instead of
+the reference's update-norm / dt-halving control, so dt stays fixed (the
+reference keeps dt = dt_max too when every solve converges and SIA is on:
+tse.py:1262-1271, 1486-1492).  Everything else follows tse.py: the solver
+call order, the time level of each input, rhof_prev starting at zero and
+persisting across steps (tse.py:1179, 1430), accept on SIA_conv or the last
+iteration (tse.py:1429).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from .device import HipContext
+from .pressure import FastPressure
+from .transport import FastTransport
+from .workloads import hydrostatic_P, lognormal_K
+
+EOS = dict(rho0=1000.0, drho_dc=700.0, lf0=2.5, dlf_dt=0.002, t_ref=10.0)
+
+
+@dataclass
+class ScenarioSpec:
+    """Inputs of one scenario (sizes in cells, lengths in m, dt in s)."""
+    nx: int
+    ny: int
+    nz: int
+    spacing: float = 5.0
+    beta: float = 0.0
+    sigma: float = 1.0
+    corr: float = 5.0
+    seed: int = 1
+    species: int = 1          # species 0 is salt (linear EOS), others tracers
+    heat: bool = True
+    salt_patch: int = 4       # top layers with salt C = 1 over the central x half
+    c_top: float | None = None  # salt Dirichlet value on the top face
+    t0: float = 10.0
+    t_bottom: float = 60.0    # Dirichlet temperature of the bottom face
+    disp: tuple = (1.0, 0.1)  # longitudinal / transversal dispersivity
+    dt: float = 86400.0
+    sia_maxit: int = 2
+    sia_conv: float = 1e-6
+    omega: float = 1.0        # OMEGA_C = OMEGA_T
+    imblflo: bool = True
+
+    @property
+    def cells(self) -> int:
+        return self.nx * self.ny * self.nz
+
+
+def host_fields(spec: ScenarioSpec) -> dict:
+    """Initial state of the variable fields (reference layout, float64)."""
+    shp = (spec.nx, spec.ny, spec.nz)
+    nx, nz = spec.nx, spec.nz
+    dx = np.full(spec.nx, float(spec.spacing))
+    dz = np.full(spec.nz, float(spec.spacing))
+    f = {"K": lognormal_K(shp, spec.sigma, spec.corr, spec.seed)}
+    f["P"] = hydrostatic_P(dx, dz, spec.beta, ny=spec.ny)
+    f["BP"] = np.zeros(shp)
+    f["BP"][0] = 1.0
+    f["BP"][-1] = 1.0
+    f["BC"] = np.zeros(shp)
+    C = [np.zeros(shp) for _ in range(spec.species)]
+    if spec.salt_patch:
+        C[0][nx // 4: nx - nx // 4, :, nz - spec.salt_patch:] = 1.0
+    if spec.c_top is not None:
+        C[0][:, :, nz - 1] = spec.c_top
+        f["BC"][:, :, nz - 1] = 1.0
+    for s in range(1, spec.species):  # tracer blobs, one per species
+        cx = (s * nx) // (spec.species + 1)
+        w = max(1, min(nx, spec.ny, nz) // 8)
+        C[s][max(0, cx - w): cx + w, spec.ny // 2 - w: spec.ny // 2 + w,
+             nz // 2 - w: nz // 2 + w] = 1.0
+    f["C"] = C
+    T = np.full(shp, spec.t0)
+    f["BT"] = np.zeros(shp)
+    if spec.heat:
+        T[:, :, 0] = spec.t_bottom
+        f["BT"][:, :, 0] = 1.0
+    f["T"] = T
+    f["rho_f"] = EOS["rho0"] + EOS["drho_dc"] * C[0]
+    f["l_f"] = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
+    return f
+
+
+# uniform fields, shared by every scenario of the same grid shape
+UNIFORM = dict(phi=0.2, c_f=4.5e-10, c_r=1e-10, nue=1e-3, cp_f=4150.0,
+               l_r=2.0, cp_r=1000.0, rho_r=2500.0, R=1.0, D_f=1e-9, zero=0.0)
+_shared: dict = {}
+
+
+def _uniform(hip: HipContext, name: str, value: float):
+    key = (hip.device_index, hip.shape, name, float(value))
+    t = _shared.get(key)
+    if t is None:
+        t = hip.upload(np.full(hip.shape, float(value)))
+        _shared[key] = t
+    return t
+
+
+def field_bytes(spec: ScenarioSpec) -> int:
+    """HBM a scenario holds: its own split fields plus both fast systems and
+    the coefficient records (uniform fields are shared and not counted)."""
+    n = spec.cells
+    S = spec.species
+    own = 16 + 3 * S                      # split-layout FP64 fields
+    fast_p = 8 * 8 + 7 * 4                # bytes per cell
+    fast_tr = 11 * 8 + 10 * 4
+    rec = 13 * 8 + 1
+    return int(n * (own * 8 + fast_p + fast_tr + rec) * 1.02) + (1 << 20)
+
+
+@dataclass
+class StepStats:
+    sia_iters: int = 0
+    sweeps: dict = field(default_factory=dict)   # stage -> sweeps
+    mass: tuple = ()
+
+
+class DeviceScenario:
+    """One scenario's fields in HBM and its SIA step as a generator."""
+
+    def __init__(self, spec: ScenarioSpec, device: int = 0, fields=None):
+        import torch
+        self.spec = spec
+        d = np.full(spec.nx, float(spec.spacing))
+        self.hip = HipContext(spec.nx, spec.ny, spec.nz, d,
+                              np.full(spec.ny, float(spec.spacing)),
+                              np.full(spec.nz, float(spec.spacing)), 9.80665,
+                              spec.beta, device)
+        hip = self.hip
+        hf = fields if fields is not None else host_fields(spec)
+        u = {k: _uniform(hip, k, v) for k, v in UNIFORM.items()}
+        self.u = u
+        self.K = hip.upload(hf["K"])
+        self.P = hip.upload(hf["P"])
+        self.BP = hip.upload(hf["BP"])
+        self.BC = hip.upload(hf["BC"])
+        self.BT = hip.upload(hf["BT"])
+        self.C = [hip.upload(c) for c in hf["C"]]
+        self.T = hip.upload(hf["T"])
+        self.rho = hip.upload(hf["rho_f"])
+        self.l_f = hip.upload(hf["l_f"])
+        self.Pn = hip.empty()
+        self.Cn = [hip.empty() for _ in self.C]
+        self.Tn = hip.empty()
+        self.rho_prev = hip.zeros()                   # tse.py:1179
+        self.Xn = hip.empty()                          # transport entry copy
+        self.qx, self.qy, self.qz = hip.zeros(), hip.zeros(), hip.zeros()
+        self.e = hip.zeros()
+        self.fp = FastPressure(spec.nx, spec.ny, spec.nz, None, None, None,
+                               None, None, hip=hip)
+        self.ft = FastTransport(spec.nx, spec.ny, spec.nz, None, None, None,
+                                None, None, hip=hip)
+        # the autotuner restores the system it timed through `assemble`
+        self.fp.assemble = self.setup_p
+        self.t = 0.0
+        self.n_steps = 0
+        self.history: list[StepStats] = []
+        self.eos = _lib.EosParams(**EOS)
+        self.lib = hip.lib
+        self.stream = torch.cuda.current_stream(hip.device)
+
+    # device pieces -----------------------------------------------------------
+    def _p(self, t):
+        return t.data_ptr()
+
+    def setup_p(self):
+        u = self.u
+        f = _lib.PFields(P=self._p(self.P), Pn=None, c_f=self._p(u["c_f"]),
+                         nue=self._p(u["nue"]), rho_f=self._p(self.rho),
+                         phi=self._p(u["phi"]), Kx=self._p(self.K),
+                         Ky=self._p(self.K), Kz=self._p(self.K),
+                         BP=self._p(self.BP), WP=self._p(u["zero"]),
+                         c_r=self._p(u["c_r"]))
+        _lib.check(self.lib.tse_fast_p_setup(self.hip.ctx, ctypes.byref(f),
+                                             self.spec.dt), "tse_fast_p_setup")
+
+    def finish_p(self):
+        _lib.check(self.lib.tse_fast_p_result(self.hip.ctx, self._p(self.P)),
+                   "tse_fast_p_result")
+
+    def flow(self):
+        """compute_q and flow_error on the new P (tse.py:1228-1237); the
+        storage reference Pn is P at the start of the step."""
+        u, L, p = self.u, self.lib, self._p
+        _lib.check(L.tse_compute_q(self.hip.ctx, p(self.P), p(self.BP), p(self.K),
+                                   p(self.K), p(self.K), p(self.qx), p(self.qy),
+                                   p(self.qz), p(u["nue"]), p(self.rho)),
+                   "tse_compute_q")
+        _lib.check(L.tse_flow_error(self.hip.ctx, p(self.P), p(self.Pn), p(self.qx),
+                                    p(self.qy), p(self.qz), p(u["zero"]),
+                                    p(u["phi"]), p(u["c_f"]), p(u["c_r"]),
+                                    p(self.rho), self.spec.dt, p(self.e)),
+                   "tse_flow_error")
+
+    def setup_c(self, s):
+        u, p, sp = self.u, self._p, self.spec
+        f = _lib.CFields(C=p(self.C[s]), Cn=p(self.Xn), D_fx=p(u["D_f"]),
+                         D_fy=p(u["D_f"]), D_fz=p(u["D_f"]), phi=p(u["phi"]),
+                         rho_f=p(self.rho), qx=p(self.qx), qy=p(self.qy),
+                         qz=p(self.qz), BC=p(self.BC), WC=p(u["zero"]),
+                         RC=p(u["R"]), e=p(self.e), lf_l=p(self._disp(0)),
+                         lf_t=p(self._disp(1)), imblflo=int(sp.imblflo))
+        _lib.check(self.lib.tse_fast_c_setup(self.hip.ctx, ctypes.byref(f), sp.dt,
+                                             sp.omega), "tse_fast_c_setup")
+
+    def setup_t(self):
+        u, p, sp = self.u, self._p, self.spec
+        f = _lib.TFields(T=p(self.T), Tn=p(self.Xn), l_f=p(self.l_f),
+                         cp_f=p(u["cp_f"]), rho_f=p(self.rho), phi=p(u["phi"]),
+                         qx=p(self.qx), qy=p(self.qy), qz=p(self.qz),
+                         BT=p(self.BT), WT=p(u["zero"]), e=p(self.e),
+                         l_r=p(u["l_r"]), cp_r=p(u["cp_r"]), rho_r=p(u["rho_r"]),
+                         lt_l=p(self._disp(0)), lt_t=p(self._disp(1)),
+                         imblflo=int(sp.imblflo))
+        _lib.check(self.lib.tse_fast_t_setup(self.hip.ctx, ctypes.byref(f), sp.dt,
+                                             sp.omega), "tse_fast_t_setup")
+
+    def _disp(self, n):
+        return _uniform(self.hip, f"disp{n}", self.spec.disp[n])
+
+    def finish_tr(self, dst):
+        _lib.check(self.lib.tse_fast_tr_result(self.hip.ctx, self._p(dst)),
+                   "tse_fast_tr_result")
+
+    def eos_update(self):
+        _lib.check(self.lib.tse_linear_eos(self.hip.ctx, self._p(self.C[0]),
+                                           self._p(self.T), self._p(self.rho),
+                                           self._p(self.l_f), ctypes.byref(self.eos)),
+                   "tse_linear_eos")
+
+    def density_norm(self) -> float:
+        out = ctypes.c_double()
+        _lib.check(self.lib.tse_l2_norm(self.hip.ctx, self._p(self.rho),
+                                        self._p(self.rho_prev), ctypes.byref(out)),
+                   "tse_l2_norm")
+        return out.value
+
+    def mass(self):
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.tse_mass_sums(self.hip.ctx, self._p(self.C[0]),
+                                          self._p(self.rho), self._p(self.u["phi"]),
+                                          self._p(self.qx), out), "tse_mass_sums")
+        return tuple(out)
+
+    # the SIA step ----------------------------------------------------------------
+    def steps(self, nsteps: int):
+        """Generator over ``nsteps`` SIA steps; yields (kind, system) for each
+        solve and resumes once the system holds its solution."""
+        sp = self.spec
+        for _ in range(nsteps):
+            st = StepStats()
+            self.Pn.copy_(self.P)
+            for s in range(sp.species):
+                self.Cn[s].copy_(self.C[s])
+            self.Tn.copy_(self.T)
+            for it in range(sp.sia_maxit):
+                st.sia_iters = it + 1
+                self.setup_p()
+                yield 0, self.fp
+                self.finish_p()
+                self.flow()
+                for s in range(sp.species):
+                    self.setup_c(s)
+                    self.ft.assemble = lambda s=s: self.setup_c(s)
+                    yield 1, self.ft
+                    self.finish_tr(self.C[s])
+                if sp.heat:
+                    self.setup_t()
+                    self.ft.assemble = self.setup_t
+                    yield 1, self.ft
+                    self.finish_tr(self.T)
+                self.eos_update()
+                diff = self.density_norm()
+                if diff < sp.sia_conv or it == sp.sia_maxit - 1:
+                    break
+                self.rho_prev.copy_(self.rho)            # tse.py:1430-1438
+                self.P.copy_(self.Pn)
+                self.T.copy_(self.Tn)
+                for s in range(sp.species):
+                    self.C[s].copy_(self.Cn[s])
+            st.mass = self.mass()
+            self.t += sp.dt
+            self.n_steps += 1
+            self.history.append(st)
+
+    def run(self, nsteps: int, form="mixed", tol=1e-10, plan_p=None,
+            plan_tr=None, sor_p=1.996, sor_tr=1.2, inner_p=100, inner_tr=10,
+            max_sweeps=400000):
+        """Drive ``steps`` with single-system solves; returns per-solve stats."""
+        out = []
+        for kind, sysm in self.steps(nsteps):
+            if kind == 0:
+                r = sysm.solve(form, tol=tol, sor=sor_p, inner=inner_p,
+                               plan=plan_p, max_sweeps=max_sweeps)
+            else:
+                r = sysm.solve(form, tol=tol, sor=sor_tr, inner=inner_tr,
+                               plan=plan_tr, max_sweeps=max_sweeps)
+            if not r.converged:
+                raise RuntimeError(f"{'P' if kind == 0 else 'transport'} solve did"
+                                   f" not converge: rel {r.rel_residual:.3e}"
+                                   f" after {r.sweeps} sweeps")
+            out.append((kind, r))
+        return out
+
+    def download(self) -> dict:
+        h = self.hip
+        return {"P": h.download(self.P), "T": h.download(self.T),
+                "C": [h.download(c) for c in self.C], "rho_f": h.download(self.rho)}
+
+    def close(self):
+        self.hip.close()
