@@ -19,6 +19,20 @@ Reported (one JSON line on stdout):
                to solution on the same problem and the speedup over it, and
                the mixed solution's deviation from the FP64 solution.
 
+Further sections of the same line (each bounded; --no-sia / --no-ensemble /
+--no-serve skip them):
+  sia          device-resident SIA steps (heat + salt + tracer) on the
+               512x512x256 problem in the mixed form and in plain FP64 solved
+               to 1e-13, and the relative L2 / max error of P, T and each C
+               of the mixed run against the FP64 run
+  ensemble     64 realizations of uneven grids (nx, ny in 32..160, nz in
+               32..96, seed 4) in lockstep batched solves with hipGraph-replayed
+               persistent tile-queue cycles: cell-updates/s, s per step
+  serving      Poisson request arrivals (grids 64^3 / 128^3 / 256x256x128,
+               heat + 3 species, 10 SIA steps each, mixed to 1e-10) under a
+               200 GB HBM admission budget: p50 / p99 time to solution and
+               solves/s at the given arrival rate(s)
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
        python bench.py --profile-kernel N   (N launches of the sweep kernel
                                               and a 1 GiB copy; for rocprof)
@@ -37,6 +51,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 BYTES_FP64_SWEEP = 56
+# HBM bytes per launch of rb_zmarch_kernel<float, 2, 16, 1> at 512x512x256,
+# z-chunk 64: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, calibrated on a 1 GiB
+# copy (profiles/rb_zmarch_fp32_pressure_512x512x256.md)
+TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64): 3.337e9}
 
 
 def log(*a):
@@ -58,6 +76,15 @@ def main(argv=None):
     ap.add_argument("--plan", type=int, nargs=4, default=None,
                     metavar=("NS", "BY", "RY", "ZCHUNK"),
                     help="FP32 sweep plan instead of autotuning")
+    ap.add_argument("--no-sia", action="store_true")
+    ap.add_argument("--sia-steps", type=int, default=1)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=2)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rates", type=float, nargs="+", default=[5.0])
+    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-steps", type=int, default=10)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not "
@@ -184,6 +211,13 @@ def main(argv=None):
                          f"{cpu_s:.1f}s"}
         log(f"cpu baseline {cpu['value']:.3e} cell-updates/s ({threads} threads)")
 
+    del fp
+    torch.cuda.empty_cache()
+    sia = None if args.no_sia else sia_section(args, nx, ny, nz)
+    ens = None if args.no_ensemble else ensemble_section(args)
+    serving = None if args.no_serve else serve_section(args)
+
+    traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32))
     line = {
         "metric": "SOR cell-updates/s",
         "value": value,
@@ -210,13 +244,134 @@ def main(argv=None):
         "plain64": plain,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "kernel": "rb_zmarch_kernel<float>",
+                     "traffic": traffic,
+                     "traffic_source": "rocprofv3 --pmc, profiles/"
+                                       "rb_zmarch_fp32_pressure_512x512x256.md"
+                                       if traffic else None,
+                     "kernel": "rb_zmarch_kernel<float>",
                      "bytes_per_update": BYTES_FP32_SWEEP,
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
+        "sia": sia,
+        "ensemble": ens,
+        "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def sia_section(args, nx, ny, nz):
+    """SIA steps with heat + salt + tracer, mixed vs plain FP64 (to 1e-13)."""
+    import numpy as np
+    import torch
+
+    from transportse_amd.scenario import DeviceScenario, ScenarioSpec, host_fields
+    spec = ScenarioSpec(nx, ny, nz, spacing=5.0, sigma=2.5, corr=16, seed=3,
+                        species=2, heat=True, salt_patch=4, dt=86400.0,
+                        sia_maxit=2, sia_conv=1e-6)
+    t0 = time.time()
+    hf = host_fields(spec)
+    log(f"sia: inputs in {time.time() - t0:.1f}s")
+    runs = {}
+    ref_tol = 1e-12
+    for form, tol in (("mixed", args.tol), ("plain64", ref_tol)):
+        scn = DeviceScenario(spec, fields=hf)
+        scn.setup_p()
+        plan = scn.fp.plan(form)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = scn.run(args.sia_steps, form=form, tol=tol, plan_p=plan,
+                      plan_tr=(2, 16, 1, 64), inner_p=100, inner_tr=10,
+                      max_sweeps=30000, strict=(form == "mixed"),
+                      progress=lambda k, r: log(
+                          f"  sia {form} {'P' if k == 0 else 'C/T'} solve: "
+                          f"{r.sweeps} sweeps, rel {r.rel_residual:.2e}, "
+                          f"{r.seconds:.2f}s"))
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        runs[form] = (scn.download(), sec, out, plan, scn.omega_p.omega)
+        scn.close()
+        del scn
+        torch.cuda.empty_cache()
+        log(f"sia {form}: {args.sia_steps} steps in {sec:.2f}s, P sweeps "
+            f"{[r.sweeps for k, r in out if k == 0]}")
+    m, ref = runs["mixed"][0], runs["plain64"][0]
+
+    def err(a, b):
+        return {"rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                "max_abs": float(np.max(np.abs(a - b)))}
+    errors = {"P": err(m["P"], ref["P"]), "T": err(m["T"], ref["T"])}
+    for s_ in range(spec.species):
+        errors[f"C{s_}"] = err(m["C"][s_], ref["C"][s_])
+    log(f"sia errors mixed vs FP64: " + ", ".join(
+        f"{k} {v['rel_l2']:.2e}" for k, v in errors.items()))
+    return {"grid": [nx, ny, nz], "steps": args.sia_steps,
+            "species": spec.species, "heat": True, "sia_maxit": spec.sia_maxit,
+            "seconds_mixed": runs["mixed"][1],
+            "seconds_plain64": runs["plain64"][1],
+            "p_sweeps_mixed": [r.sweeps for k, r in runs["mixed"][2] if k == 0],
+            "p_sweeps_plain64": [r.sweeps for k, r in runs["plain64"][2] if k == 0],
+            "tr_sweeps_mixed": [r.sweeps for k, r in runs["mixed"][2] if k == 1],
+            "reference": f"plain FP64 form solved to rel. residual {ref_tol:g}"
+                         " (or 30000 sweeps)",
+            "reference_rel_residuals": [r.rel_residual for _, r in runs["plain64"][2]],
+            "error_mixed_vs_fp64": errors}
+
+
+def ensemble_section(args):
+    import torch
+
+    from transportse_amd.ensemble import Ensemble, ensemble_specs
+    specs = ensemble_specs(args.ens_members)
+    cells = sum(s.cells for s in specs)
+    t0 = time.time()
+    ens = Ensemble(specs, graph=True, inner_p=100, inner_tr=10)
+    log(f"ensemble: {len(specs)} realizations, {cells / 1e6:.1f}M cells, set up "
+        f"in {time.time() - t0:.1f}s")
+    tuned = ens.tune_plans(cycles=2, zchunks=(0, 32))
+    log(f"ensemble plans: P {ens.plan_p}, transport {ens.plan_tr}")
+    st = ens.run(args.ens_steps)
+    ens.close()
+    del ens
+    torch.cuda.empty_cache()
+    log(f"ensemble: {args.ens_steps} steps in {st.seconds:.2f}s, "
+        f"{st.cell_updates_per_s:.3e} cell-updates/s")
+    return {"realizations": len(specs), "cells": cells, "steps": args.ens_steps,
+            "seconds": st.seconds, "seconds_per_step": st.seconds / args.ens_steps,
+            "cell_updates_per_s": st.cell_updates_per_s, "solves": st.solves,
+            "sweeps_p": st.sweeps_p, "sweeps_tr": st.sweeps_tr,
+            "cycles": st.cycles, "graph": True,
+            "plan_p": list(tuned[0][0]) if tuned.get(0) else None,
+            "plan_tr": list(tuned[1][0]) if tuned.get(1) else None}
+
+
+def serve_section(args):
+    import torch
+
+    from transportse_amd.serve import Server, make_requests, prepare
+    out = []
+    for rate in args.serve_rates:
+        reqs = make_requests(args.serve_requests, rate, steps=args.serve_steps)
+        t0 = time.time()
+        prepare(reqs)
+        log(f"serving: {len(reqs)} requests at {rate}/s prepared in "
+            f"{time.time() - t0:.1f}s")
+        srv = Server(budget_bytes=200e9)
+        st = srv.run(reqs)
+        srv.close()
+        del srv, reqs
+        torch.cuda.empty_cache()
+        log(f"serving {rate}/s: p50 {st.p50:.2f}s p99 {st.p99:.2f}s, "
+            f"{st.solves_per_s:.2f} solves/s, max {st.max_concurrent} concurrent")
+        out.append({"rate_per_s": rate, "requests": st.completed,
+                    "p50_s": st.p50, "p99_s": st.p99, "mean_s": st.mean,
+                    "solves_per_s": st.solves_per_s, "seconds": st.seconds,
+                    "peak_admitted_bytes": st.peak_bytes,
+                    "max_concurrent": st.max_concurrent})
+    return {"budget_bytes": 200e9, "steps_per_request": args.serve_steps,
+            "grids": [[64, 64, 64], [128, 128, 128], [256, 256, 128]],
+            "species": 3, "heat": True, "tol": 1e-10, "form": "mixed",
+            "runs": out}
 
 
 if __name__ == "__main__":

@@ -267,6 +267,8 @@ int tse_batch_add(tse_batch *b, tse_ctx *member);
 int tse_batch_rearm(tse_batch *b, int32_t slot);
 /* Hands the member's iterate back to its context (tse_fast_*_result). */
 int tse_batch_writeback(tse_batch *b, int32_t slot);
+/* Over-relaxation factor of one member (default: the batch's sor). */
+int tse_batch_set_sor(tse_batch *b, int32_t slot, double sor);
 /* Writes back and removes a member. */
 int tse_batch_remove(tse_batch *b, int32_t slot);
 /* One cycle for all active members; *n_active = members still active.

@@ -128,6 +128,7 @@ SIGNATURES = {
     "tse_batch_rearm": (_I, [_V, ctypes.c_int32]),
     "tse_batch_writeback": (_I, [_V, ctypes.c_int32]),
     "tse_batch_remove": (_I, [_V, ctypes.c_int32]),
+    "tse_batch_set_sor": (_I, [_V, ctypes.c_int32, _D]),
     "tse_batch_cycle": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_batch_run": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_batch_status": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastResult)]),

@@ -74,6 +74,10 @@ class Batch:
     def writeback(self, slot: int):
         _lib.check(self.lib.tse_batch_writeback(self.h, slot), "tse_batch_writeback")
 
+    def set_sor(self, slot: int, sor: float):
+        _lib.check(self.lib.tse_batch_set_sor(self.h, slot, float(sor)),
+                   "tse_batch_set_sor")
+
     def remove(self, slot: int):
         _lib.check(self.lib.tse_batch_remove(self.h, slot), "tse_batch_remove")
         self.members.pop(slot, None)

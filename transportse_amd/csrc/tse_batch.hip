@@ -32,6 +32,7 @@ struct BMember {
     FastDiv dnx, dny;
     int nx, ny, nz, rchunk0;
     long N;
+    double sor;     // this member's over-relaxation factor
     double *p[2];   // FP64 iterate ping-pong
     float *e[2];    // FP32 correction ping-pong
     float *r32;
@@ -70,13 +71,14 @@ __global__ void __launch_bounds__(64 * BYT)
 batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
                        const BStat *__restrict__ st,
                        const int2 *__restrict__ items, int n_items,
-                       unsigned *queue, unsigned base, int launch, T omega) {
+                       unsigned *queue, unsigned base, int launch) {
     for (;;) {
         const int it = fetch_item(queue, base);
         if (it >= n_items) break;
         const int2 w = items[it];
         const BMember &m = mem[w.x];
         if (!st[w.x].active) continue;  // uniform per block
+        const T omega = (T)m.sor;
         if constexpr (sizeof(T) == 4) {
             rb_zmarch_tile<T, NS, BYT, RY>(
                 m.g, w.y, m.e[launch & 1], m.e[(launch + 1) & 1], m.fop.Wx32,
@@ -96,13 +98,14 @@ __global__ void __launch_bounds__(64 * BY)
 batch_tr_zmarch_kernel(const BMember *__restrict__ mem,
                        const BStat *__restrict__ st,
                        const int2 *__restrict__ items, int n_items,
-                       unsigned *queue, unsigned base, int launch, T omega) {
+                       unsigned *queue, unsigned base, int launch) {
     for (;;) {
         const int it = fetch_item(queue, base);
         if (it >= n_items) break;
         const int2 w = items[it];
         const BMember &m = mem[w.x];
         if (!st[w.x].active) continue;
+        const T omega = (T)m.sor;
         if constexpr (sizeof(T) == 4) {
             tr_zmarch_tile<T, NS, BY>(m.g, w.y, m.e[launch & 1],
                                       m.e[(launch + 1) & 1], m.top.c32, m.N,
@@ -333,15 +336,14 @@ int occupancy_grid(K kernel, int threads, int n_items, int *grid) {
 
 // Launch one persistent sweep launch (index `launch` within the cycle).
 template <typename T>
-int launch_sweeps(tse_batch *b, int launch, unsigned base, T omega, bool size_only) {
+int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
     const int ns = b->plan.ns, by = b->plan.by, ry = b->plan.ry;
 #define TSE_BP(NS_, BYT_, RY_)                                                         \
     if (b->kind == SYS_P && ns == NS_ && by == BYT_ * RY_ && ry == RY_) {              \
         auto k = batch_rb_zmarch_kernel<T, NS_, BYT_, RY_>;                            \
         if (size_only) return occupancy_grid(k, 64 * BYT_, b->n_items, &b->grid);     \
         hipLaunchKernelGGL(k, dim3(b->grid), dim3(64, BYT_), 0, b->stream, b->dmem,    \
-                           b->dst, b->ditems, b->n_items, b->dqueue, base, launch,     \
-                           omega);                                                     \
+                           b->dst, b->ditems, b->n_items, b->dqueue, base, launch);    \
         TSE_B_TRY(hipGetLastError());                                                  \
         return 0;                                                                      \
     }
@@ -350,8 +352,7 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, T omega, bool size_on
         auto k = batch_tr_zmarch_kernel<T, NS_, BY_>;                                  \
         if (size_only) return occupancy_grid(k, 64 * BY_, b->n_items, &b->grid);      \
         hipLaunchKernelGGL(k, dim3(b->grid), dim3(64, BY_), 0, b->stream, b->dmem,     \
-                           b->dst, b->ditems, b->n_items, b->dqueue, base, launch,     \
-                           omega);                                                     \
+                           b->dst, b->ditems, b->n_items, b->dqueue, base, launch);    \
         TSE_B_TRY(hipGetLastError());                                                  \
         return 0;                                                                      \
     }
@@ -408,8 +409,8 @@ int enqueue_cycle(tse_batch *b) {
     TSE_B_TRY(hipGetLastError());
     for (int l = 0; l < L; ++l) {
         const unsigned base = (unsigned)l * (unsigned)(b->n_items + b->grid);
-        const int rc = fp32 ? launch_sweeps<float>(b, l, base, (float)b->sor, false)
-                            : launch_sweeps<double>(b, l, base, b->sor, false);
+        const int rc = fp32 ? launch_sweeps<float>(b, l, base, false)
+                            : launch_sweeps<double>(b, l, base, false);
         if (rc) return rc;
     }
     int rc = fp32 ? residual_any(b, 1, 0, b->n_ritems, 0, L & 1, 0)
@@ -430,8 +431,8 @@ int prepare(tse_batch *b) {
     if (rc) return rc;
     if (b->n_items == 0) return 0;
     const bool fp32 = b->form == TSE_FORM_MIXED;
-    return fp32 ? launch_sweeps<float>(b, 0, 0, 0.f, true)
-                : launch_sweeps<double>(b, 0, 0, 0.0, true);
+    return fp32 ? launch_sweeps<float>(b, 0, 0, true)
+                : launch_sweeps<double>(b, 0, 0, true);
 }
 
 int check_batch(tse_batch *b) {
@@ -458,6 +459,7 @@ BMember member_entry(const tse_batch *b, tse_ctx *c, FastSys *s) {
     m.r32 = s->r32();
     if (b->kind == SYS_P) m.fop = s->pop();
     else m.top = s->trop();
+    m.sor = b->sor;
     return m;
 }
 
@@ -471,8 +473,9 @@ int init_member(tse_batch *b, int slot) {
     tse_ctx *c = b->ctx[slot];
     FastSys *s = member_sys(b, c);
     if (c->stream != b->stream) TSE_B_TRY(hipStreamSynchronize(c->stream));
-    const BMember m = member_entry(b, c, s);
+    BMember m = member_entry(b, c, s);
     const BMember &o = b->hmem[slot];
+    m.sor = o.sor;
     if (m.N != o.N || m.p[0] != o.p[0] || m.p[1] != o.p[1] || m.e[0] != o.e[0] ||
         m.nx != o.nx || m.ny != o.ny) {
         b->hmem[slot] = m;
@@ -608,6 +611,23 @@ int tse_batch_writeback(tse_batch *b, int32_t slot) {
         return fail(TSE_ERR_ARG, "no member in that slot");
     TSE_B_TRY(hipStreamSynchronize(b->stream));
     member_sys(b, b->ctx[slot])->cur = b->hst[slot].cur;
+    return 0;
+}
+
+// Sets the over-relaxation factor of one member (default: the batch's).
+// Takes effect from the next cycle; a captured cycle stays valid.
+int tse_batch_set_sor(tse_batch *b, int32_t slot, double sor) {
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (slot < 0 || slot >= b->cap || !b->hused[slot])
+        return fail(TSE_ERR_ARG, "no member in that slot");
+    if (!(sor > 0.0 && sor < 2.0)) return fail(TSE_ERR_ARG, "sor must be in (0, 2)");
+    b->hmem[slot].sor = sor;
+    if (!b->dirty) {
+        TSE_B_TRY(hipMemcpyAsync(&b->dmem[slot].sor, &b->hmem[slot].sor,
+                                 sizeof(double), hipMemcpyHostToDevice, b->stream));
+        TSE_B_TRY(hipStreamSynchronize(b->stream));
+    }
     return 0;
 }
 

@@ -1,0 +1,205 @@
+"""Batched ensemble: many independent realizations on one GPU.
+
+Every realization is a ``DeviceScenario`` with its own grid, permeability,
+slope and boundary value.  They advance in lockstep stages: each round, all
+realizations waiting on a pressure solve are solved together in the
+pressure ``Batch`` and all waiting on a transport solve in the transport
+``Batch`` (one persistent tile-queue launch per sweep group over every
+member's tiles), then each realization runs its device work up to its next
+solve.  A realization keeps its slot in both batches for the whole run and
+is re-armed for each new solve, so each batch's cycle is captured once as a
+hipGraph and replayed for every cycle of every step.
+
+Launch plans are chosen by ``tune_plans``: each candidate plan is timed over
+a few batched cycles of the actual ensemble, and the fastest per sweep
+wins.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+from .autotune import PLANS32, PLANS64, TR_PLANS32, TR_PLANS64
+from .batch import KIND_P, KIND_TR, Batch
+from .scenario import DeviceScenario, ScenarioSpec
+
+
+def ensemble_specs(n: int = 64, seed: int = 4, species: int = 1,
+                   dt: float = 3600.0, sia_maxit: int = 2) -> list[ScenarioSpec]:
+    """Realizations drawn from master seed `seed` (SeedSequence.spawn):
+    nx, ny ~ U{32..160}, nz ~ U{32..96}, sigma_lnK ~ U[0.5, 2.0],
+    beta ~ U[0, 10] deg, top salt Dirichlet value ~ U[0.2, 1.0].
+
+    dt = 1 h: a dense top boundary over fresh hydrostatic water starts with
+    a flow imbalance (flow_error, tse.py:409-446) of up to ~4e-5 1/s per unit
+    density; the transport update (tse.py:870-875) stays a contraction only
+    while its storage term phi/dt exceeds that (phi/dt = 5.6e-5 1/s at 1 h;
+    at 1 day the reference's own iteration accepts C far outside the
+    boundary values)."""
+    out = []
+    for child in np.random.SeedSequence(seed).spawn(n):
+        rng = np.random.default_rng(child)
+        nx, ny = (int(v) for v in rng.integers(32, 161, size=2))
+        nz = int(rng.integers(32, 97))
+        out.append(ScenarioSpec(
+            nx, ny, nz, spacing=5.0, beta=float(rng.uniform(0.0, 10.0)),
+            sigma=float(rng.uniform(0.5, 2.0)), corr=5.0,
+            seed=int(rng.integers(0, 2 ** 31 - 1)), species=species, heat=True,
+            salt_patch=0, c_top=float(rng.uniform(0.2, 1.0)), dt=dt,
+            sia_maxit=sia_maxit))
+    return out
+
+
+@dataclass
+class EnsembleStats:
+    seconds: float
+    rounds: int
+    solves: int
+    cell_sweeps: float        # sum over solves of sweeps * cells
+    sweeps_p: int
+    sweeps_tr: int
+    cycles: int
+
+    @property
+    def cell_updates_per_s(self) -> float:
+        return self.cell_sweeps / self.seconds
+
+
+class Ensemble:
+    def __init__(self, specs, form="mixed", tol=1e-10, plan_p=(2, 16, 1, 0),
+                 plan_tr=(2, 16, 1, 0), inner_p=50, inner_tr=10, sor_p=1.95,
+                 sor_tr=1.0, graph=True, device=0, max_sweeps=400000,
+                 tune_omega=True):
+        import torch
+        self.tune_omega = tune_omega
+        self.stream = torch.cuda.Stream(device)   # default priority
+        self.device = device
+        self.form, self.tol, self.graph = form, tol, graph
+        self.sor_p, self.sor_tr = sor_p, sor_tr
+        self.inner_p, self.inner_tr = inner_p, inner_tr
+        self.max_sweeps = max_sweeps
+        with torch.cuda.stream(self.stream):
+            self.scn = [DeviceScenario(s, device) for s in specs]
+        self.plan_p, self.plan_tr = tuple(plan_p), tuple(plan_tr)
+        self.bp = self.btr = None
+
+    def _batch(self, kind, plan, inner, sor, tol=None):
+        return Batch(kind, self.form, plan, inner=inner,
+                     tol=self.tol if tol is None else tol, sor=sor,
+                     max_sweeps=self.max_sweeps, device=self.device,
+                     stream=self.stream)
+
+    def tune_plans(self, cycles: int = 2, zchunks=(0, 32), verbose=False):
+        """Time batched cycles of every candidate plan on the ensemble's own
+        first pressure and transport systems; keep the fastest."""
+        import torch
+        fp32 = self.form == "mixed"
+        res = {}
+        with torch.cuda.stream(self.stream):
+            for kind, plans, setup, sys_of, inner, sor in (
+                    (KIND_P, PLANS32 if fp32 else PLANS64,
+                     lambda s: s.setup_p(), lambda s: s.fp, self.inner_p, self.sor_p),
+                    (KIND_TR, TR_PLANS32 if fp32 else TR_PLANS64,
+                     lambda s: s.setup_c(0), lambda s: s.ft, self.inner_tr,
+                     self.sor_tr)):
+                best = None
+                for ns, by, ry in plans:
+                    for zc in zchunks:
+                        plan = (ns, by, ry, zc)
+                        for s in self.scn:
+                            setup(s)
+                        # tolerance 0: every timed cycle does its full sweeps
+                        b = self._batch(kind, plan, max(ns, inner // ns * ns), sor,
+                                        tol=0.0)
+                        for s in self.scn:
+                            b.add(sys_of(s))
+                        b.cycle(graph=self.graph)        # capture + warm-up
+                        ev0 = torch.cuda.Event(enable_timing=True)
+                        ev1 = torch.cuda.Event(enable_timing=True)
+                        ev0.record(self.stream)
+                        n = 0
+                        for _ in range(cycles):
+                            if b.cycle(graph=self.graph) == 0:
+                                break
+                            n += 1
+                        ev1.record(self.stream)
+                        torch.cuda.synchronize()
+                        b.close()
+                        if n == 0:
+                            continue
+                        ms = ev0.elapsed_time(ev1) / (n * b.inner)
+                        if verbose:
+                            print(f"  tune kind={kind} plan={plan}: {ms * 1e3:.1f}"
+                                  " us/sweep (all members)", flush=True)
+                        if best is None or ms < best[1]:
+                            best = (plan, ms)
+                res[kind] = best
+        if res.get(KIND_P):
+            self.plan_p = res[KIND_P][0]
+        if res.get(KIND_TR):
+            self.plan_tr = res[KIND_TR][0]
+        return res
+
+    def run(self, nsteps: int) -> EnsembleStats:
+        import torch
+        with torch.cuda.stream(self.stream):
+            if self.bp is None:
+                self.bp = self._batch(KIND_P, self.plan_p, self.inner_p, self.sor_p)
+                self.btr = self._batch(KIND_TR, self.plan_tr, self.inner_tr,
+                                       self.sor_tr)
+            slots = {KIND_P: {}, KIND_TR: {}}
+            batches = {KIND_P: self.bp, KIND_TR: self.btr}
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            gens = {i: s.steps(nsteps) for i, s in enumerate(self.scn)}
+            pend = {i: next(g) for i, g in gens.items()}
+            rounds = solves = cycles = 0
+            cell_sweeps = 0.0
+            sweeps = {KIND_P: 0, KIND_TR: 0}
+            while pend:
+                rounds += 1
+                for kind in (KIND_P, KIND_TR):
+                    ids = [i for i, (k, _) in pend.items() if k == kind]
+                    if not ids:
+                        continue
+                    b = batches[kind]
+                    for i in ids:
+                        if i not in slots[kind]:
+                            slots[kind][i] = b.add(pend[i][1])
+                            if kind == KIND_P and self.tune_omega:
+                                b.set_sor(slots[kind][i], self.scn[i].omega_p.omega)
+                        else:
+                            if kind == KIND_P and self.tune_omega:
+                                b.set_sor(slots[kind][i], self.scn[i].omega_p.omega)
+                            b.rearm(slots[kind][i])
+                    cycles += b.run(graph=self.graph)
+                    for i in ids:
+                        st = b.status(slots[kind][i])
+                        if not st.converged:
+                            raise RuntimeError(
+                                f"realization {i} kind {kind} did not converge:"
+                                f" rel {st.rel_residual:.3e} after {st.sweeps} sweeps")
+                        b.writeback(slots[kind][i])
+                        if kind == KIND_P and self.tune_omega:
+                            self.scn[i].omega_p.report(st.sweeps)
+                        sweeps[kind] += st.sweeps
+                        cell_sweeps += st.sweeps * self.scn[i].spec.cells
+                        solves += 1
+                    for i in ids:
+                        try:
+                            pend[i] = next(gens[i])
+                        except StopIteration:
+                            del pend[i]
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+        return EnsembleStats(sec, rounds, solves, cell_sweeps, sweeps[KIND_P],
+                             sweeps[KIND_TR], cycles)
+
+    def close(self):
+        for b in (self.bp, self.btr):
+            if b is not None:
+                b.close()
+        for s in self.scn:
+            s.close()

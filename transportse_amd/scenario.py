@@ -30,6 +30,7 @@ import numpy as np
 
 from . import _lib
 from .device import HipContext
+from .omega import OmegaTuner
 from .pressure import FastPressure
 from .transport import FastTransport
 from .workloads import hydrostatic_P, lognormal_K
@@ -172,6 +173,7 @@ class DeviceScenario:
                                 None, None, hip=hip)
         # the autotuner restores the system it timed through `assemble`
         self.fp.assemble = self.setup_p
+        self.omega_p = OmegaTuner(max(spec.nx, spec.ny, spec.nz))
         self.t = 0.0
         self.n_steps = 0
         self.history: list[StepStats] = []
@@ -304,21 +306,29 @@ class DeviceScenario:
             self.history.append(st)
 
     def run(self, nsteps: int, form="mixed", tol=1e-10, plan_p=None,
-            plan_tr=None, sor_p=1.996, sor_tr=1.2, inner_p=100, inner_tr=10,
-            max_sweeps=400000):
-        """Drive ``steps`` with single-system solves; returns per-solve stats."""
+            plan_tr=None, sor_p=None, sor_tr=1.0, inner_p=100, inner_tr=10,
+            max_sweeps=400000, strict=True, progress=None):
+        """Drive ``steps`` with single-system solves; returns per-solve stats.
+        sor_p None: the pressure factor comes from the scenario's tuner.
+        strict: raise if a solve stops at max_sweeps above its tolerance.
+        progress: called with (kind, stats) after every solve."""
         out = []
         for kind, sysm in self.steps(nsteps):
             if kind == 0:
-                r = sysm.solve(form, tol=tol, sor=sor_p, inner=inner_p,
+                om = self.omega_p.omega if sor_p is None else sor_p
+                r = sysm.solve(form, tol=tol, sor=om, inner=inner_p,
                                plan=plan_p, max_sweeps=max_sweeps)
+                if sor_p is None:
+                    self.omega_p.report(r.sweeps)
             else:
                 r = sysm.solve(form, tol=tol, sor=sor_tr, inner=inner_tr,
                                plan=plan_tr, max_sweeps=max_sweeps)
-            if not r.converged:
+            if strict and not r.converged:
                 raise RuntimeError(f"{'P' if kind == 0 else 'transport'} solve did"
                                    f" not converge: rel {r.rel_residual:.3e}"
                                    f" after {r.sweeps} sweeps")
+            if progress:
+                progress(kind, r)
             out.append((kind, r))
         return out
 

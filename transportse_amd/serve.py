@@ -1,0 +1,216 @@
+"""Serving loop: scenario requests with continuous batching on one GPU.
+
+Requests arrive as a Poisson process.  Each request is a scenario (grid,
+permeability, species) that runs a number of SIA steps to a relative
+residual in mixed precision.  The server
+
+* admits waiting requests in arrival order while their HBM footprint fits
+  the budget (``scenario.field_bytes``) -- fields are uploaded once and stay
+  device-resident across the request's time steps;
+* every tick runs one cycle (``inner`` sweeps + residual + stop test) of the
+  pressure batch and one of the transport batch over all requests that are
+  in a solve right now, whichever step or Picard iteration each is in;
+* between ticks, moves every request whose solve finished on to its next
+  solve (device work of the SIA step in between) and adds it to the batch
+  of that solve's kind; requests that arrive meanwhile join at the next
+  tick.
+
+A request's arithmetic does not depend on what else is in the batches (each
+member's sweeps, residuals and stop test are its own), so its results are
+the same alone and in the mix.  Latency is arrival to completion (queueing
+included); throughput is completed requests per second of wall time.
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .batch import KIND_P, KIND_TR, Batch
+from .scenario import DeviceScenario, ScenarioSpec, field_bytes
+
+GRIDS = ((64, 64, 64), (128, 128, 128), (256, 256, 128))
+
+
+@dataclass
+class Request:
+    rid: int
+    arrival: float            # seconds after the start of the run
+    spec: ScenarioSpec
+    steps: int
+    admitted: float = -1.0
+    done: float = -1.0
+    solves: int = 0
+    sweeps: int = 0
+    fields: dict | None = None   # request payload: initial host fields
+    result: dict | None = None
+
+
+def prepare(requests) -> None:
+    """Build each request's input fields (the payload a client sends)."""
+    from .scenario import host_fields
+    for r in requests:
+        if r.fields is None:
+            r.fields = host_fields(r.spec)
+
+
+def make_requests(n: int, rate: float, steps: int = 10, species: int = 3,
+                  arrival_seed: int = 5, grid_seed: int = 6,
+                  grids=GRIDS, dt: float = 3600.0) -> list[Request]:
+    """n requests, Poisson arrivals at `rate` per second (seed 5), grids drawn
+    from `grids` (seed 6), sigma_lnK ~ U[0.5, 2.5], heat + `species`, time
+    step dt (1 h keeps the transport update a contraction through the
+    initial density-driven transient, as for the ensemble)."""
+    ra = np.random.default_rng(arrival_seed)
+    rg = np.random.default_rng(grid_seed)
+    t = np.cumsum(ra.exponential(1.0 / rate, size=n))
+    out = []
+    for i in range(n):
+        g = grids[int(rg.integers(0, len(grids)))]
+        spec = ScenarioSpec(*g, spacing=5.0, beta=0.0,
+                            sigma=float(rg.uniform(0.5, 2.5)), corr=5.0,
+                            seed=int(rg.integers(0, 2 ** 31 - 1)),
+                            species=species, heat=True, salt_patch=4, dt=dt)
+        out.append(Request(i, float(t[i]), spec, steps))
+    return out
+
+
+@dataclass
+class ServeStats:
+    rate: float
+    completed: int
+    seconds: float
+    p50: float
+    p99: float
+    mean: float
+    solves_per_s: float       # completed requests per second
+    peak_bytes: int
+    max_concurrent: int
+    latencies: list = field(default_factory=list)
+
+
+class Server:
+    def __init__(self, budget_bytes=200e9, form="mixed", tol=1e-10,
+                 plan_p=(2, 16, 1, 32), plan_tr=(2, 16, 1, 32), inner_p=50,
+                 inner_tr=10, sor_p=1.95, sor_tr=1.0, device=0,
+                 max_sweeps=400000, keep_results=False):
+        import torch
+        self.budget = float(budget_bytes)
+        self.device = device
+        self.stream = torch.cuda.Stream(device)   # default priority
+        self.keep = keep_results
+        mk = lambda kind, plan, inner, sor: Batch(
+            kind, form, plan, inner=inner, tol=tol, sor=sor,
+            max_sweeps=max_sweeps, device=device, stream=self.stream)
+        self.batches = {KIND_P: mk(KIND_P, plan_p, inner_p, sor_p),
+                        KIND_TR: mk(KIND_TR, plan_tr, inner_tr, sor_tr)}
+
+    def run(self, requests: list[Request], time_scale: float = 1.0) -> ServeStats:
+        """Serve `requests` (arrival times in seconds, scaled by
+        time_scale); returns latency / throughput statistics."""
+        import torch
+        reqs = sorted(requests, key=lambda r: r.arrival)
+        waiting: deque = deque()
+        live = {}            # rid -> [request, scenario, generator, kind, slot]
+        used = 0
+        peak = 0
+        max_conc = 0
+        nxt = 0
+        done = []
+        with torch.cuda.stream(self.stream):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            now = lambda: time.perf_counter() - t0
+            while nxt < len(reqs) or waiting or live:
+                t = now()
+                while nxt < len(reqs) and reqs[nxt].arrival * time_scale <= t:
+                    waiting.append(reqs[nxt])
+                    nxt += 1
+                # admission by HBM footprint, in arrival order
+                while waiting:
+                    r = waiting[0]
+                    need = field_bytes(r.spec)
+                    if used + need > self.budget and live:
+                        break
+                    waiting.popleft()
+                    used += need
+                    peak = max(peak, used)
+                    r.admitted = now()
+                    scn = DeviceScenario(r.spec, self.device, fields=r.fields)
+                    r.fields = None
+                    gen = scn.steps(r.steps)
+                    kind, sysm = next(gen)
+                    slots = {KIND_P: None, KIND_TR: None}
+                    self._enter(scn, kind, sysm, slots)
+                    live[r.rid] = [r, scn, gen, kind, slots, need]
+                max_conc = max(max_conc, len(live))
+                if not live:
+                    # idle until the next arrival
+                    if nxt < len(reqs):
+                        dt = reqs[nxt].arrival * time_scale - now()
+                        if dt > 0:
+                            time.sleep(min(dt, 0.05))
+                    continue
+                # one cycle of each batch over every request in a solve
+                for b in self.batches.values():
+                    b.cycle(graph=False)
+                # advance requests whose solve finished
+                for rid in list(live):
+                    r, scn, gen, kind, slots, need = live[rid]
+                    b = self.batches[kind]
+                    st = b.status(slots[kind])
+                    if st.active:
+                        continue
+                    if not st.converged:
+                        raise RuntimeError(f"request {rid}: solve did not converge"
+                                           f" (rel {st.rel_residual:.3e})")
+                    b.writeback(slots[kind])
+                    if kind == KIND_P:
+                        scn.omega_p.report(st.sweeps)
+                    r.solves += 1
+                    r.sweeps += st.sweeps
+                    try:
+                        kind, sysm = next(gen)
+                    except StopIteration:
+                        for k, sl in slots.items():
+                            if sl is not None:
+                                self.batches[k].remove(sl)
+                        torch.cuda.synchronize()
+                        r.done = now()
+                        if self.keep:
+                            r.result = scn.download()
+                        scn.close()
+                        used -= need
+                        del live[rid]
+                        done.append(r)
+                        continue
+                    self._enter(scn, kind, sysm, slots)
+                    live[rid][3] = kind
+            sec = now()
+        lat = np.array([r.done - r.arrival * time_scale for r in done])
+        return ServeStats(
+            rate=0.0, completed=len(done), seconds=sec,
+            p50=float(np.percentile(lat, 50)) if len(lat) else float("nan"),
+            p99=float(np.percentile(lat, 99)) if len(lat) else float("nan"),
+            mean=float(lat.mean()) if len(lat) else float("nan"),
+            solves_per_s=len(done) / sec if sec > 0 else 0.0, peak_bytes=peak,
+            max_concurrent=max_conc, latencies=lat.tolist())
+
+    def _enter(self, scn, kind, sysm, slots):
+        """Put a scenario's set-up system into the batch of its kind (a slot
+        per kind for the request's lifetime; re-armed for later solves)."""
+        b = self.batches[kind]
+        if slots[kind] is None:
+            slots[kind] = b.add(sysm)
+            if kind == KIND_P:
+                b.set_sor(slots[kind], scn.omega_p.omega)
+        else:
+            if kind == KIND_P:
+                b.set_sor(slots[kind], scn.omega_p.omega)
+            b.rearm(slots[kind])
+
+    def close(self):
+        for b in self.batches.values():
+            b.close()
