@@ -26,8 +26,9 @@ Further sections of the same line (each bounded; --no-sia / --no-ensemble /
                to 1e-13, and the relative L2 / max error of P, T and each C
                of the mixed run against the FP64 run
   ensemble     64 realizations of uneven grids (nx, ny in 32..160, nz in
-               32..96, seed 4) in lockstep batched solves with hipGraph-replayed
-               persistent tile-queue cycles: cell-updates/s, s per step
+               32..96, seed 4), heat + salt, 50 SIA steps each, in lockstep
+               batched solves with hipGraph-replayed persistent tile-queue
+               cycles and autotuned plans: cell-updates/s, s per step
   serving      Poisson request arrivals (grids 64^3 / 128^3 / 256x256x128,
                heat + 3 species, 10 SIA steps each, mixed to 1e-10) under a
                200 GB HBM admission budget: p50 / p99 time to solution and
@@ -80,7 +81,7 @@ def main(argv=None):
     ap.add_argument("--sia-steps", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-members", type=int, default=64)
-    ap.add_argument("--ens-steps", type=int, default=2)
+    ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+", default=[5.0])
     ap.add_argument("--serve-requests", type=int, default=12)
