@@ -52,10 +52,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 BYTES_FP64_SWEEP = 56
-# HBM bytes per launch of rb_zmarch_kernel<float, 2, 16, 1> at 512x512x256,
-# z-chunk 64: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, calibrated on a 1 GiB
-# copy (profiles/rb_zmarch_fp32_pressure_512x512x256.md)
-TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64): 3.337e9}
+# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plans
+# the autotuner picks there: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, each
+# calibrated on a 1 GiB copy (profiles/rb_zmarch*_fp32_pressure_512x512x256.md)
+TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64, 0): 3.337e9,
+                      (512, 512, 256, 2, 16, 1, 128, 1): 3.167e9,
+                      (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9}
 
 
 def log(*a):
@@ -74,9 +76,9 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=8)
     ap.add_argument("--profile-kernel", type=int, default=0)
-    ap.add_argument("--plan", type=int, nargs=4, default=None,
-                    metavar=("NS", "BY", "RY", "ZCHUNK"),
-                    help="FP32 sweep plan instead of autotuning")
+    ap.add_argument("--plan", type=int, nargs="+", default=None,
+                    metavar="N", help="FP32 sweep plan NS BY RY ZCHUNK [VARIANT]"
+                    " instead of autotuning")
     ap.add_argument("--no-sia", action="store_true")
     ap.add_argument("--sia-steps", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")
@@ -159,8 +161,8 @@ def main(argv=None):
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     ns = plan32[0]
-    launches = 40
-    fp.sweeps(ns, plan32, sor=args.omega, fp32=True)
+    launches = 200
+    fp.sweeps(50 * ns, plan32, sor=args.omega, fp32=True)  # steady clocks
     ev0.record(stream)
     fp.sweeps(launches * ns, plan32, sor=args.omega, fp32=True)
     ev1.record(stream)
@@ -218,7 +220,7 @@ def main(argv=None):
     ens = None if args.no_ensemble else ensemble_section(args)
     serving = None if args.no_serve else serve_section(args)
 
-    traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32))
+    traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32) + (0,) * (5 - len(plan32)))
     line = {
         "metric": "SOR cell-updates/s",
         "value": value,
@@ -236,7 +238,7 @@ def main(argv=None):
                                "sigma_lnK 2.5, corr 16 cells, mixed precision "
                                f"to rel. residual {args.tol:g}",
                    "grid": [nx, ny, nz], "form": "mixed", "omega": args.omega,
-                   "plan_ns_by_ry_zchunk": list(plan32)},
+                   "plan_ns_by_ry_zchunk_variant": list(plan32)},
         "time_to_solution_s": tts,
         "solves_per_s": 1.0 / tts,
         "sweeps_per_solve": sweeps // args.steps,
@@ -246,10 +248,12 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "traffic_source": "rocprofv3 --pmc, profiles/"
-                                       "rb_zmarch_fp32_pressure_512x512x256.md"
+                     "traffic_source": "rocprofv3 --pmc of the same plan, profiles/"
+                                       + ("rb_zmarch_xl" if plan32[-1] == 1 else "rb_zmarch")
+                                       + "_fp32_pressure_512x512x256.md"
                                        if traffic else None,
-                     "kernel": "rb_zmarch_kernel<float>",
+                     "kernel": ("rb_zmarch_xl_kernel<float>" if len(plan32) > 4
+                                and plan32[4] == 1 else "rb_zmarch_kernel<float>"),
                      "bytes_per_update": BYTES_FP32_SWEEP,
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,

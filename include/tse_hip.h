@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TSE_ABI_VERSION 1
+#define TSE_ABI_VERSION 2
 
 enum tse_status {
     TSE_OK = 0,
@@ -179,11 +179,16 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *                     FP64, p += e in FP64 (defect correction). */
 /* Launch plan of the fused sweep kernel: ns sweeps per launch, tile of
  * 64 x by columns with ry tile rows per thread (blocks of 64 x by/ry
- * threads), zchunk z planes per block (<= 0: whole column).  Supported
- * (ns, by, ry): (1,8,1) (1,16,1) (2,16,1) (1,24,2), and for the FP32
- * sweeps also (1,32,2) (2,32,2) (2,32,4) (3,16,1). */
+ * threads), zchunk z planes per block (<= 0: whole column), variant 0
+ * (x neighbours through LDS) or 1 (x neighbours exchanged across the lanes
+ * of a wavefront; smaller LDS ring).  Supported (ns, by, ry):
+ *   variant 0: (1,8,1) (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also
+ *              (1,32,2) (2,32,2) (2,32,4) (3,16,1)
+ *   variant 1: (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also (2,24,2)
+ *              (2,32,2) (3,16,1) (3,24,2)
+ * Transport systems: variant 0, (1,8,1) (1,16,1) (2,16,1), FP32 (3,16,1). */
 typedef struct {
-    int32_t ns, by, ry, zchunk;
+    int32_t ns, by, ry, zchunk, variant;
 } tse_sweep_plan;
 
 typedef struct {

@@ -19,11 +19,14 @@ from helpers import Fields
 
 pytestmark = pytest.mark.gpu
 
-# (ns, by, ry, zchunk): every kernel variant, with and without z-chunks
-PLANS = [(1, 8, 1, 0), (1, 16, 1, 5), (2, 16, 1, 0), (2, 16, 1, 6),
-         (1, 24, 2, 0), (1, 24, 2, 7)]
-PLANS32 = [(2, 32, 2, 6), (2, 32, 4, 0), (1, 32, 2, 5),
-           (3, 16, 1, 9)]
+# (ns, by, ry, zchunk, variant): every kernel instance, with and without
+# z-chunks; variant 1 exchanges x neighbours across lanes instead of LDS
+PLANS = [(1, 8, 1, 0, 0), (1, 16, 1, 5, 0), (2, 16, 1, 0, 0), (2, 16, 1, 6, 0),
+         (1, 24, 2, 0, 0), (1, 24, 2, 7, 0),
+         (1, 16, 1, 0, 1), (2, 16, 1, 5, 1), (1, 24, 2, 6, 1)]
+PLANS32 = [(2, 32, 2, 6, 0), (2, 32, 4, 0, 0), (1, 32, 2, 5, 0),
+           (3, 16, 1, 9, 0), (2, 24, 2, 6, 1), (2, 32, 2, 0, 1),
+           (3, 16, 1, 7, 1), (3, 24, 2, 9, 1)]
 
 def _setup(f, dt):
     from transportse_amd import _lib
@@ -47,8 +50,8 @@ def _result(hip, d):
 
 
 @pytest.mark.parametrize("shape", [(70, 23, 19), (33, 20, 17), (9, 40, 12)])
-@pytest.mark.parametrize("ns,by,ry,zchunk", PLANS)
-def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, ry, zchunk):
+@pytest.mark.parametrize("ns,by,ry,zchunk,variant", PLANS)
+def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, ry, zchunk, variant):
     from oracle import Oracle
     from transportse_amd import _lib
     f = Fields(*shape, seed=21)
@@ -60,14 +63,14 @@ def test_fused_sweeps_match_red_black_sweeps(shape, ns, by, ry, zchunk):
             o.p_half_sweep(Po, f.P, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky,
                            f.Kz, f.BP, f.WP, dt, sor, colour)
     hip, d = _setup(f, dt)
-    plan = _lib.SweepPlan(ns=ns, by=by, ry=ry, zchunk=zchunk)
+    plan = _lib.SweepPlan(ns=ns, by=by, ry=ry, zchunk=zchunk, variant=variant)
     _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n, sor, 0),
                "tse_fast_p_sweeps")
     Ph = _result(hip, d)
     change = np.max(np.abs(Po - f.P))
     err = np.max(np.abs(Ph - Po))
-    print("fused", shape, ns, by, ry, zchunk, "change", change, "err", err,
-          flush=True)
+    print("fused", shape, ns, by, ry, zchunk, variant, "change", change, "err",
+          err, flush=True)
     assert change > 0
     assert err <= 1e-10 * change
 
@@ -101,14 +104,14 @@ def test_solve_reaches_oracle_fixed_point(form, tol, bound):
     assert err <= bound * change
 
 
-@pytest.mark.parametrize("ns,by,ry,zchunk", PLANS32 + PLANS)
-def test_fp32_plans_agree(ns, by, ry, zchunk):
+@pytest.mark.parametrize("ns,by,ry,zchunk,variant", PLANS32 + PLANS)
+def test_fp32_plans_agree(ns, by, ry, zchunk, variant):
     """Every FP32 sweep plan gives the same mixed-precision solve (same
     sweep count to the same residual, fields within FP32-cycle noise)."""
     from transportse_amd import _lib
     f = Fields(70, 37, 21, seed=23)
     outs = []
-    for pl in ((1, 16, 1, 0), (ns, by, ry, zchunk)):
+    for pl in ((1, 16, 1, 0, 0), (ns, by, ry, zchunk, variant)):
         hip, d = _setup(f, 3600.0)
         prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(*pl), inner=12,
                               max_sweeps=12000, sor=1.9, tol=1e-10)
@@ -119,7 +122,28 @@ def test_fp32_plans_agree(ns, by, ry, zchunk):
         outs.append((_result(hip, d), res.sweeps))
     (a, na), (b, nb) = outs
     change = np.max(np.abs(a - f.P))
-    print("fp32 plan", (ns, by, ry, zchunk), "sweeps", na, nb,
+    print("fp32 plan", (ns, by, ry, zchunk, variant), "sweeps", na, nb,
           "diff/change", np.max(np.abs(a - b)) / change, flush=True)
     assert abs(na - nb) <= 24
     assert np.max(np.abs(a - b)) <= 1e-6 * change
+
+
+@pytest.mark.parametrize("ns,by,ry", [(1, 16, 1), (2, 16, 1), (1, 24, 2)])
+def test_lane_exchange_variant_equals_lds_variant(ns, by, ry):
+    """Variant 1 reads the same operands in the same order as variant 0
+    (x neighbours from the adjacent lanes instead of LDS): FP64 sweeps agree
+    to rounding (the compiler may contract the two differently)."""
+    from transportse_amd import _lib
+    f = Fields(83, 41, 23, seed=24)
+    out = []
+    for var in (0, 1):
+        hip, d = _setup(f, 3600.0)
+        plan = _lib.SweepPlan(ns=ns, by=by, ry=ry, zchunk=9, variant=var)
+        _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), 6 * ns,
+                                             1.8, 0), "sweeps")
+        out.append(_result(hip, d))
+    change = np.max(np.abs(out[0] - f.P))
+    diff = np.max(np.abs(out[0] - out[1]))
+    print("variant 1 vs 0", (ns, by, ry), "identical", np.array_equal(*out),
+          "diff/change", diff / change, flush=True)
+    assert diff <= 1e-13 * change

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtse_hip.so")
 
-ABI_VERSION = 1
+ABI_VERSION = 2
 
 FORM_FAITHFUL64 = 0
 FORM_PLAIN64 = 1
@@ -45,7 +45,8 @@ T_FIELDS = ("T", "Tn", "l_f", "cp_f", "rho_f", "phi", "qx", "qy", "qz", "BT",
 
 class SweepPlan(ctypes.Structure):
     _fields_ = [("ns", ctypes.c_int32), ("by", ctypes.c_int32),
-                ("ry", ctypes.c_int32), ("zchunk", ctypes.c_int32)]
+                ("ry", ctypes.c_int32), ("zchunk", ctypes.c_int32),
+                ("variant", ctypes.c_int32)]
 
 
 class FastParams(ctypes.Structure):

@@ -15,11 +15,15 @@ import ctypes
 
 from . import _lib
 
-PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1), (1, 24, 2)]
-PLANS32 = PLANS64 + [(1, 32, 2), (2, 32, 2), (2, 32, 4), (3, 16, 1)]
+# (ns, by, ry, variant): variant 0 exchanges x neighbours through LDS,
+# variant 1 across the lanes of a wavefront (two LDS plane rings, not three)
+PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0), (1, 24, 2, 0),
+           (1, 16, 1, 1), (2, 16, 1, 1), (1, 24, 2, 1)]
+PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
+                     (2, 24, 2, 1), (2, 32, 2, 1), (3, 16, 1, 1), (3, 24, 2, 1)]
 # transport: the cell's own coefficients stay in registers, one row per thread
-TR_PLANS64 = [(1, 8, 1), (1, 16, 1), (2, 16, 1)]
-TR_PLANS32 = TR_PLANS64 + [(3, 16, 1)]
+TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0)]
+TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0)]
 ZCHUNKS = (0, 128, 64, 32)
 
 KINDS = {
@@ -32,7 +36,7 @@ _cache: dict = {}
 
 def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
          verbose: bool = False, kind: str = "p"):
-    """Return ((ns, by, ry, zchunk), ms_per_sweep) of the fastest plan."""
+    """Return ((ns, by, ry, zchunk, variant), ms_per_sweep) of the fastest plan."""
     import torch
     key = (kind, hip.shape, bool(fp32))
     if key in _cache:
@@ -44,12 +48,12 @@ def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
     ev1 = torch.cuda.Event(enable_timing=True)
     stream = torch.cuda.current_stream()
     nz = hip.shape[2]
-    for ns, by, ry in (p32 if fp32 else p64):
+    for ns, by, ry, var in (p32 if fp32 else p64):
         for zc in zchunks:
             if zc and zc >= nz:
                 continue
             n = ns * max(1, sweeps // ns)
-            plan = _lib.SweepPlan(ns, by, ry, zc)
+            plan = _lib.SweepPlan(ns, by, ry, zc, var)
             _lib.check(fn(hip.ctx, ctypes.byref(plan), ns, sor, int(fp32)), "warmup")
             ev0.record(stream)
             _lib.check(fn(hip.ctx, ctypes.byref(plan), n, sor, int(fp32)), "sweeps")
@@ -57,9 +61,9 @@ def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
             torch.cuda.synchronize()
             ms = ev0.elapsed_time(ev1) / n
             if verbose:
-                print(f"  {kind} plan ns={ns} by={by} ry={ry} zchunk={zc}: "
-                      f"{ms * 1e3:.1f} us/sweep", flush=True)
+                print(f"  {kind} plan ns={ns} by={by} ry={ry} zchunk={zc} "
+                      f"variant={var}: {ms * 1e3:.1f} us/sweep", flush=True)
             if best is None or ms < best[1]:
-                best = ((ns, by, ry, zc), ms)
+                best = ((ns, by, ry, zc, var), ms)
     _cache[key] = best
     return best

@@ -66,7 +66,7 @@ __device__ inline int fetch_item(unsigned *queue, unsigned base) {
 
 // Persistent pressure z-march over (member, tile) items.  correction != 0:
 // FP32 sweeps on e (launch 0 starts from zero), else FP64 sweeps on p.
-template <typename T, int NS, int BYT, int RY>
+template <typename T, int NS, int BYT, int RY, int VAR>
 __global__ void __launch_bounds__(64 * BYT)
 batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
                        const BStat *__restrict__ st,
@@ -79,16 +79,36 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
         const BMember &m = mem[w.x];
         if (!st[w.x].active) continue;  // uniform per block
         const T omega = (T)m.sor;
+        const T *pin;
+        T *pout;
+        const T *wx, *wy, *wz, *inv, *rhs;
+        int zero;
         if constexpr (sizeof(T) == 4) {
-            rb_zmarch_tile<T, NS, BYT, RY>(
-                m.g, w.y, m.e[launch & 1], m.e[(launch + 1) & 1], m.fop.Wx32,
-                m.fop.Wy32, m.fop.Wz32, m.fop.invD32, m.r32, omega, launch == 0);
+            pin = m.e[launch & 1];
+            pout = m.e[(launch + 1) & 1];
+            wx = m.fop.Wx32;
+            wy = m.fop.Wy32;
+            wz = m.fop.Wz32;
+            inv = m.fop.invD32;
+            rhs = m.r32;
+            zero = launch == 0;
         } else {
             const int c = (st[w.x].cur + launch) & 1;
-            rb_zmarch_tile<T, NS, BYT, RY>(m.g, w.y, m.p[c], m.p[c ^ 1],
-                                           m.fop.Wx, m.fop.Wy, m.fop.Wz,
-                                           m.fop.invD, m.fop.b, omega, 0);
+            pin = m.p[c];
+            pout = m.p[c ^ 1];
+            wx = m.fop.Wx;
+            wy = m.fop.Wy;
+            wz = m.fop.Wz;
+            inv = m.fop.invD;
+            rhs = m.fop.b;
+            zero = 0;
         }
+        if constexpr (VAR == 0)
+            rb_zmarch_tile<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
+                                           rhs, omega, zero);
+        else
+            rb_zmarch_tile_xl<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
+                                              rhs, omega, zero);
         __syncthreads();  // LDS reuse by the next tile
     }
 }
@@ -337,10 +357,10 @@ int occupancy_grid(K kernel, int threads, int n_items, int *grid) {
 // Launch one persistent sweep launch (index `launch` within the cycle).
 template <typename T>
 int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
-    const int ns = b->plan.ns, by = b->plan.by, ry = b->plan.ry;
-#define TSE_BP(NS_, BYT_, RY_)                                                         \
-    if (b->kind == SYS_P && ns == NS_ && by == BYT_ * RY_ && ry == RY_) {              \
-        auto k = batch_rb_zmarch_kernel<T, NS_, BYT_, RY_>;                            \
+    const int ns = b->plan.ns, by = b->plan.by, ry = b->plan.ry, v = b->plan.variant;
+#define TSE_BP(V_, NS_, BYT_, RY_)                                                     \
+    if (b->kind == SYS_P && v == V_ && ns == NS_ && by == BYT_ * RY_ && ry == RY_) {   \
+        auto k = batch_rb_zmarch_kernel<T, NS_, BYT_, RY_, V_>;                        \
         if (size_only) return occupancy_grid(k, 64 * BYT_, b->n_items, &b->grid);     \
         hipLaunchKernelGGL(k, dim3(b->grid), dim3(64, BYT_), 0, b->stream, b->dmem,    \
                            b->dst, b->ditems, b->n_items, b->dqueue, base, launch);    \
@@ -348,7 +368,7 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         return 0;                                                                      \
     }
 #define TSE_BT(NS_, BY_)                                                               \
-    if (b->kind == SYS_TR && ns == NS_ && by == BY_ && ry == 1) {                      \
+    if (b->kind == SYS_TR && v == 0 && ns == NS_ && by == BY_ && ry == 1) {            \
         auto k = batch_tr_zmarch_kernel<T, NS_, BY_>;                                  \
         if (size_only) return occupancy_grid(k, 64 * BY_, b->n_items, &b->grid);      \
         hipLaunchKernelGGL(k, dim3(b->grid), dim3(64, BY_), 0, b->stream, b->dmem,     \
@@ -356,18 +376,25 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_B_TRY(hipGetLastError());                                                  \
         return 0;                                                                      \
     }
-    TSE_BP(1, 8, 1)
-    TSE_BP(1, 16, 1)
-    TSE_BP(2, 16, 1)
-    TSE_BP(1, 12, 2)
+    TSE_BP(0, 1, 8, 1)
+    TSE_BP(0, 1, 16, 1)
+    TSE_BP(0, 2, 16, 1)
+    TSE_BP(0, 1, 12, 2)
+    TSE_BP(1, 1, 16, 1)
+    TSE_BP(1, 2, 16, 1)
+    TSE_BP(1, 1, 12, 2)
     TSE_BT(1, 8)
     TSE_BT(1, 16)
     TSE_BT(2, 16)
     if constexpr (sizeof(T) == 4) {
-        TSE_BP(1, 16, 2)
-        TSE_BP(2, 16, 2)
-        TSE_BP(2, 8, 4)
-        TSE_BP(3, 16, 1)
+        TSE_BP(0, 1, 16, 2)
+        TSE_BP(0, 2, 16, 2)
+        TSE_BP(0, 2, 8, 4)
+        TSE_BP(0, 3, 16, 1)
+        TSE_BP(1, 2, 12, 2)
+        TSE_BP(1, 2, 16, 2)
+        TSE_BP(1, 3, 16, 1)
+        TSE_BP(1, 3, 12, 2)
         TSE_BT(3, 16)
     }
 #undef TSE_BP

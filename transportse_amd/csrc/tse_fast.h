@@ -171,6 +171,185 @@ rb_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
     }
 }
 
+// ---- cross-lane x neighbours -------------------------------------------------------------------
+// A tile row is one 64-lane wavefront (lane = x offset), so the cells left
+// and right of lane i sit in lanes i-1 and i+1: DPP wave shift right / left
+// by one lane moves a register between them (lanes 0 and 63 receive 0; the
+// tile's outer columns are halo and never updated).
+__device__ __forceinline__ int lane_shr1(int v) {  // lane i <- lane i-1
+    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int lane_shl1(int v) {  // lane i <- lane i+1
+    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+}
+__device__ __forceinline__ float from_left(float v) {
+    return __int_as_float(lane_shr1(__float_as_int(v)));
+}
+__device__ __forceinline__ float from_right(float v) {
+    return __int_as_float(lane_shl1(__float_as_int(v)));
+}
+__device__ __forceinline__ double from_left(double v) {
+    const long long b = __double_as_longlong(v);
+    const long long lo = (unsigned)lane_shr1((int)(b & 0xffffffffll));
+    const long long hi = lane_shr1((int)(b >> 32));
+    return __longlong_as_double((hi << 32) | lo);
+}
+__device__ __forceinline__ double from_right(double v) {
+    const long long b = __double_as_longlong(v);
+    const long long lo = (unsigned)lane_shl1((int)(b & 0xffffffffll));
+    const long long hi = lane_shl1((int)(b >> 32));
+    return __longlong_as_double((hi << 32) | lo);
+}
+
+// Variant of rb_zmarch_tile with the x neighbours exchanged across lanes:
+// only P and Wy go through LDS (two plane rings instead of three), each
+// thread keeps its own Wx and Wy of the in-flight planes in registers, and
+// P(x+-1) and Wx(x-1) come from the adjacent lanes.  Same operands in the
+// same order as rb_zmarch_tile; the smaller LDS ring allows taller tiles at
+// the same occupancy.
+template <typename T, int NS, int BYT, int RY>
+__device__ __forceinline__ void
+rb_zmarch_tile_xl(const ZGeo &g, int tile, const T *__restrict__ Pin,
+                  T *__restrict__ Pout, const T *__restrict__ Wx,
+                  const T *__restrict__ Wy, const T *__restrict__ Wz,
+                  const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
+                  int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int BY = BYT * RY;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int NSLOT = R + 2;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[NSLOT][BY][BX];
+    __shared__ T sWy[NSLOT][BY][BX];
+
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x;
+    const int i = tx * TX - R + lx;
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const long plane = (long)g.nx * g.ny;
+    const int dxe = min(lx, BX - 1 - lx);
+
+    int ly[RY], d[RY], par[RY];
+    bool in_dom[RY];
+    long col[RY];
+    T p[RY][R + 2], rwz[RY][R + 2], rinv[RY][R + 1], rb[RY][R + 1];
+    T rwx[RY][R + 1], rwy[RY][R + 1];
+    T np[RY], ni[RY], nb[RY], nwz[RY], nwx[RY], nwy[RY];
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        ly[q] = threadIdx.y + q * BYT;
+        const int j = ty * TY - R + ly[q];
+        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+        d[q] = min(dxe, min(ly[q], BY - 1 - ly[q]));
+        col[q] = in_dom[q] ? (long)i + (long)g.nx * j : 0;
+        par[q] = (i + j) & 1;
+#pragma unroll
+        for (int r = 0; r < R + 2; ++r) {
+            p[q][r] = T(0);
+            rwz[q][r] = T(0);
+        }
+#pragma unroll
+        for (int r = 0; r < R + 1; ++r) {
+            rinv[q][r] = rb[q][r] = rwx[q][r] = rwy[q][r] = T(0);
+        }
+        np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
+        if (in_dom[q] && kl0 <= kl1) {
+            const long o = col[q] + plane * kl0;
+            np[q] = zero_init ? T(0) : Pin[o];
+            ni[q] = invD[o];
+            nb[q] = rhs[o];
+            nwz[q] = Wz[o];
+            nwx[q] = Wx[o];
+            nwy[q] = Wy[o];
+        }
+    }
+
+    int s0 = kl0 % NSLOT;
+    for (int k = kl0; k <= k1 + R - 1; ++k) {
+#pragma unroll
+        for (int q = 0; q < RY; ++q) {
+            const T p0 = np[q], i0 = ni[q], b0 = nb[q], wz0 = nwz[q];
+            const T wx0 = nwx[q], wy0 = nwy[q];
+            np[q] = ni[q] = nb[q] = nwz[q] = nwx[q] = nwy[q] = T(0);
+            if (in_dom[q] && k + 1 <= kl1) {
+                const long o = col[q] + plane * (k + 1);
+                np[q] = zero_init ? T(0) : Pin[o];
+                ni[q] = invD[o];
+                nb[q] = rhs[o];
+                nwz[q] = Wz[o];
+                nwx[q] = Wx[o];
+                nwy[q] = Wy[o];
+            }
+#pragma unroll
+            for (int r = R + 1; r > 0; --r) {
+                p[q][r] = p[q][r - 1];
+                rwz[q][r] = rwz[q][r - 1];
+            }
+#pragma unroll
+            for (int r = R; r > 0; --r) {
+                rinv[q][r] = rinv[q][r - 1];
+                rb[q][r] = rb[q][r - 1];
+                rwx[q][r] = rwx[q][r - 1];
+                rwy[q][r] = rwy[q][r - 1];
+            }
+            p[q][0] = p0;
+            rwz[q][0] = wz0;
+            rinv[q][0] = i0;
+            rb[q][0] = b0;
+            rwx[q][0] = wx0;
+            rwy[q][0] = wy0;
+            sP[s0][ly[q]][lx] = p0;
+            sWy[s0][ly[q]][lx] = wy0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < R; ++h) {
+            const int m = k - 1 - h;
+            int s = s0 - 1 - h;
+            if (s < 0) s += NSLOT;
+            const bool mok = m >= kl0 && m <= kl1;
+#pragma unroll
+            for (int q = 0; q < RY; ++q) {
+                // all lanes take part in the exchange; x neighbours have the
+                // other colour, so phase h does not change what is read
+                const T pl = from_left(p[q][1 + h]);
+                const T pr = from_right(p[q][1 + h]);
+                const T wxl = from_left(rwx[q][1 + h]);
+                if (mok && d[q] >= h + 1 && in_dom[q] &&
+                    ((par[q] + m) & 1) == (h & 1) && rinv[q][1 + h] > T(0)) {
+                    const int y = ly[q];
+                    T acc = rb[q][1 + h];
+                    acc += rwx[q][1 + h] * pr;
+                    acc += wxl * pl;
+                    acc += rwy[q][1 + h] * sP[s][y + 1][lx];
+                    acc += sWy[s][y - 1][lx] * sP[s][y - 1][lx];
+                    acc += rwz[q][1 + h] * p[q][h];
+                    acc += rwz[q][2 + h] * p[q][h + 2];
+                    const T pc = p[q][1 + h];
+                    const T pn = pc + omega * (acc * rinv[q][1 + h] - pc);
+                    p[q][1 + h] = pn;
+                    sP[s][y][lx] = pn;
+                }
+            }
+        }
+        const int mo = k - R;
+        if (mo >= k0 && mo < k1) {
+#pragma unroll
+            for (int q = 0; q < RY; ++q)
+                if (d[q] >= R && in_dom[q]) Pout[col[q] + plane * mo] = p[q][R];
+        }
+        s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
+    }
+}
+
 struct FastOp {  // natural-layout operator arrays
     double *Wx, *Wy, *Wz, *D, *invD, *b;
     float *Wx32, *Wy32, *Wz32, *invD32;

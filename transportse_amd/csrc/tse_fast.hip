@@ -44,6 +44,18 @@ rb_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                    omega, zero_init);
 }
 
+template <typename T, int NS, int BYT, int RY>
+__global__ void __launch_bounds__(64 * BYT)
+rb_zmarch_xl_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                    const T *__restrict__ Wx, const T *__restrict__ Wy,
+                    const T *__restrict__ Wz, const T *__restrict__ invD,
+                    const T *__restrict__ rhs, T omega, int zero_init) {
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    rb_zmarch_tile_xl<T, NS, BYT, RY>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs,
+                                      omega, zero_init);
+}
+
 // ---- operator assembly (from split-layout fields) ---------------------------------------
 
 __device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
@@ -367,31 +379,39 @@ static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
 }
 
 template <typename T>
-static int launch_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by, int ry,
+static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
                          const T *Pin, T *Pout, const T *Wx, const T *Wy,
                          const T *Wz, const T *invD, const T *rhs, T omega,
                          int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
-#define TSE_ZM(NS_, BYT_, RY_)                                                    \
-    if (ns == NS_ && by == BYT_ * RY_ && ry == RY_) {                             \
-        hipLaunchKernelGGL((rb_zmarch_kernel<T, NS_, BYT_, RY_>), grid,           \
-                           dim3(64, BYT_), 0, st, zg, Pin, Pout, Wx, Wy, Wz,       \
-                           invD, rhs, omega, zero_init);                          \
+    const int ns = pl.ns, by = pl.by, ry = pl.ry, v = pl.variant;
+#define TSE_ZM(V_, K_, NS_, BYT_, RY_)                                            \
+    if (v == V_ && ns == NS_ && by == BYT_ * RY_ && ry == RY_) {                  \
+        hipLaunchKernelGGL((K_<T, NS_, BYT_, RY_>), grid, dim3(64, BYT_), 0, st,   \
+                           zg, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,           \
+                           zero_init);                                            \
         TSE_HIP_TRY(hipGetLastError());                                           \
         return 0;                                                                 \
     }
-    TSE_ZM(1, 8, 1)
-    TSE_ZM(1, 16, 1)
-    TSE_ZM(2, 16, 1)
-    TSE_ZM(1, 12, 2)
+    TSE_ZM(0, rb_zmarch_kernel, 1, 8, 1)
+    TSE_ZM(0, rb_zmarch_kernel, 1, 16, 1)
+    TSE_ZM(0, rb_zmarch_kernel, 2, 16, 1)
+    TSE_ZM(0, rb_zmarch_kernel, 1, 12, 2)
+    TSE_ZM(1, rb_zmarch_xl_kernel, 1, 16, 1)
+    TSE_ZM(1, rb_zmarch_xl_kernel, 2, 16, 1)
+    TSE_ZM(1, rb_zmarch_xl_kernel, 1, 12, 2)
     if constexpr (sizeof(T) == 4) {
-        TSE_ZM(1, 16, 2)
-        TSE_ZM(2, 16, 2)
-        TSE_ZM(2, 8, 4)
-        TSE_ZM(3, 16, 1)
+        TSE_ZM(0, rb_zmarch_kernel, 1, 16, 2)
+        TSE_ZM(0, rb_zmarch_kernel, 2, 16, 2)
+        TSE_ZM(0, rb_zmarch_kernel, 2, 8, 4)
+        TSE_ZM(0, rb_zmarch_kernel, 3, 16, 1)
+        TSE_ZM(1, rb_zmarch_xl_kernel, 2, 12, 2)
+        TSE_ZM(1, rb_zmarch_xl_kernel, 2, 16, 2)
+        TSE_ZM(1, rb_zmarch_xl_kernel, 3, 16, 1)
+        TSE_ZM(1, rb_zmarch_xl_kernel, 3, 12, 2)
     }
 #undef TSE_ZM
-    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry)");
+    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
 }
 
 template <typename T>
@@ -419,16 +439,24 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     if (kind == SYS_TR)
-        return pl.ry == 1 && ((pl.ns == 1 && (pl.by == 8 || pl.by == 16)) ||
-                              (pl.ns == 2 && pl.by == 16) ||
-                              (fp32 && pl.ns == 3 && pl.by == 16));
-    static const int both[][3] = {{1, 8, 1}, {1, 16, 1}, {2, 16, 1}, {1, 24, 2}};
-    static const int f32[][3] = {{1, 32, 2}, {2, 32, 2}, {2, 32, 4}, {3, 16, 1}};
+        return pl.variant == 0 && pl.ry == 1 &&
+               ((pl.ns == 1 && (pl.by == 8 || pl.by == 16)) ||
+                (pl.ns == 2 && pl.by == 16) || (fp32 && pl.ns == 3 && pl.by == 16));
+    // {variant, ns, by, ry}
+    static const int both[][4] = {{0, 1, 8, 1},  {0, 1, 16, 1}, {0, 2, 16, 1},
+                                  {0, 1, 24, 2}, {1, 1, 16, 1}, {1, 2, 16, 1},
+                                  {1, 1, 24, 2}};
+    static const int f32[][4] = {{0, 1, 32, 2}, {0, 2, 32, 2}, {0, 2, 32, 4},
+                                 {0, 3, 16, 1}, {1, 2, 24, 2}, {1, 2, 32, 2},
+                                 {1, 3, 16, 1}, {1, 3, 24, 2}};
+    auto hit = [&](const int c[4]) {
+        return pl.variant == c[0] && pl.ns == c[1] && pl.by == c[2] && pl.ry == c[3];
+    };
     for (auto &c : both)
-        if (pl.ns == c[0] && pl.by == c[1] && pl.ry == c[2]) return true;
+        if (hit(c)) return true;
     if (fp32)
         for (auto &c : f32)
-            if (pl.ns == c[0] && pl.by == c[1] && pl.ry == c[2]) return true;
+            if (hit(c)) return true;
     return false;
 }
 
@@ -456,7 +484,7 @@ static int plain_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
         int rc;
         if (s->kind == SYS_P) {
             FastOp op = s->pop();
-            rc = launch_zmarch<double>(ctx->stream, zg, pl.ns, pl.by, pl.ry,
+            rc = launch_zmarch<double>(ctx->stream, zg, pl,
                                        s->p(s->cur), s->p(1 - s->cur), op.Wx,
                                        op.Wy, op.Wz, op.invD, op.b, sor, 0);
         } else {
@@ -481,7 +509,7 @@ static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
         int rc;
         if (s->kind == SYS_P) {
             FastOp op = s->pop();
-            rc = launch_zmarch<float>(ctx->stream, zg, pl.ns, pl.by, pl.ry,
+            rc = launch_zmarch<float>(ctx->stream, zg, pl,
                                       s->e(e), s->e(1 - e), op.Wx32, op.Wy32,
                                       op.Wz32, op.invD32, s->r32(), sor, q == 0);
         } else {

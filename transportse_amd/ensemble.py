@@ -105,9 +105,9 @@ class Ensemble:
                      lambda s: s.setup_c(0), lambda s: s.ft, self.inner_tr,
                      self.sor_tr)):
                 best = None
-                for ns, by, ry in plans:
+                for ns, by, ry, var in plans:
                     for zc in zchunks:
-                        plan = (ns, by, ry, zc)
+                        plan = (ns, by, ry, zc, var)
                         for s in self.scn:
                             setup(s)
                         # tolerance 0: every timed cycle does its full sweeps

@@ -65,9 +65,8 @@ class FastSystem:
                 if form == "mixed":
                     # the FP32 timing path needs a residual in place
                     self.solve("mixed", max_sweeps=0, plan=(1, 16, 1, 0))
-                (ns, by, ry, zc), _ = _tune(self.hip, form == "mixed",
-                                            kind=self.KIND)
-                self.plans[form] = (ns, by, ry, zc)
+                plan, _ = _tune(self.hip, form == "mixed", kind=self.KIND)
+                self.plans[form] = tuple(plan)
                 self.assemble()  # restore the iterate the tuning advanced
             else:
                 self.plans[form] = self.DEFAULT_PLAN
