@@ -157,6 +157,14 @@ int tse_compute_c(tse_ctx *ctx, const tse_c_fields *f,
 int tse_compute_t(tse_ctx *ctx, const tse_t_fields *f,
                   const tse_sor_params *prm, tse_result *out);
 
+/* smolarkiewicz (tse.py:1083-1116): anti-diffusive velocities of the
+ * Smolarkiewicz correction from the field C (a species slice or T) and the
+ * fluxes; vx, vy, vz are updated in place (the last index of an axis keeps
+ * its previous value, as in the reference). */
+int tse_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
+                      const double *qy, const double *qz, double *vx,
+                      double *vy, double *vz, double dt, double eps, double Sc);
+
 /* l2_norm (tse.py:273-281) of two device fields: a parallel FP64
  * reduction (rounding differs from Numba's sequential sum by O(1e-16)). */
 int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,

@@ -85,6 +85,8 @@ def _bind(L):
         L.tse_or_l2_norm.argtypes = [_dp, _dp, ctypes.c_size_t, ctypes.POINTER(I)]
         L.tse_or_dispersivity.restype = None
         L.tse_or_dispersivity.argtypes = [G, _dp, _dp, _dp, _dp, _dp, _dp, I, I, I, _dp]
+        L.tse_or_smolarkiewicz.restype = None
+        L.tse_or_smolarkiewicz.argtypes = [G] + [_dp] * 7 + [D, D, D]
         L.tse_or_compute_p.restype = I
         L.tse_or_compute_p.argtypes = [G] + [_dp] * 12 + [D, D, I, D, D, D, D, R]
         L.tse_or_p_half_sweep.restype = None
@@ -197,6 +199,18 @@ class Oracle:
         self._L.tse_or_dispersivity(ctypes.byref(self.grid), *[_p(a) for a in arrs],
                                   i, j, k, _p(out))
         return tuple(float(v) for v in out)
+
+    def smolarkiewicz(self, C, qx, qy, qz, vx, vy, vz, dt, eps=1e-15, Sc=1.08):
+        """tse.py:1083-1116; updates vx, vy, vz in place and returns them."""
+        ins = [_c(a) for a in (C, qx, qy, qz)]
+        outs = [_c(a) for a in (vx, vy, vz)]
+        self._L.tse_or_smolarkiewicz(ctypes.byref(self.grid), *[_p(a) for a in ins],
+                                     *[_p(a) for a in outs], float(dt), float(eps),
+                                     float(Sc))
+        for dst, src in zip((vx, vy, vz), outs):
+            if dst is not src:
+                dst[...] = src
+        return vx, vy, vz
 
     # solvers -------------------------------------------------------------
     def compute_P(self, P, c_f, nue, rho_f, phi, Kx, Ky, Kz, qx, qy, qz, dt_P,

@@ -349,6 +349,52 @@ int tse_or_compute_p(const tse_or_grid *g, double *P, double *Pn,
 }
 
 /* ---- compute_q (tse.py:744-780) ----------------------------------------- */
+/* smolarkiewicz (tse.py:1083-1116): anti-diffusive velocities of the
+ * Smolarkiewicz (1983) correction.  Cells with C > 0 get, per axis with an
+ * upper neighbour, Sc*(|q| dd - dt q^2)*(C_nb - C) / ((C + C_nb + eps) dd)
+ * with dd the centre distance, evaluated left to right; the last index of
+ * an axis keeps its previous value.  Cells with C <= 0 get 0 on all axes.
+ * q^2 is q*q (Numba's integer power). */
+void tse_or_smolarkiewicz(const tse_or_grid *g, const double *C,
+                          const double *qx, const double *qy, const double *qz,
+                          double *vx, double *vy, double *vz, double dt,
+                          double eps, double Sc) {
+    const int nx = g->nx, ny = g->ny, nz = g->nz;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const size_t c = IDX(g, i, j, k);
+                if (C[c] > 0) {
+                    if (i < nx - 1) {
+                        const size_t e = IDX(g, i + 1, j, k);
+                        const double dd = 0.5 * (g->dx[i + 1] + g->dx[i]);
+                        const double q = qx[c];
+                        vx[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[e] - C[c]) /
+                                ((C[c] + C[e] + eps) * dd);
+                    }
+                    if (j < ny - 1) {
+                        const size_t e = IDX(g, i, j + 1, k);
+                        const double dd = 0.5 * (g->dy[j + 1] + g->dy[j]);
+                        const double q = qy[c];
+                        vy[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[e] - C[c]) /
+                                ((C[c] + C[e] + eps) * dd);
+                    }
+                    if (k < nz - 1) {
+                        const size_t e = IDX(g, i, j, k + 1);
+                        const double dd = 0.5 * (g->dz[k + 1] + g->dz[k]);
+                        const double q = qz[c];
+                        vz[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[e] - C[c]) /
+                                ((C[c] + C[e] + eps) * dd);
+                    }
+                } else {
+                    vx[c] = 0.0;
+                    vy[c] = 0.0;
+                    vz[c] = 0.0;
+                }
+            }
+}
+
 void tse_or_compute_q(const tse_or_grid *g, const double *P, const double *BP,
                       const double *Kx, const double *Ky, const double *Kz,
                       double *qx, double *qy, double *qz, const double *nue,

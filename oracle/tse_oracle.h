@@ -125,6 +125,11 @@ int tse_or_compute_t(const tse_or_grid *g, double *T, double *Tn_out,
                      tse_or_result *out);
 
 /* Number of sweep threads (OpenMP); 0 leaves the OpenMP default. */
+/* smolarkiewicz (tse.py:1083-1116); vx, vy, vz updated in place. */
+void tse_or_smolarkiewicz(const tse_or_grid *g, const double *C,
+                          const double *qx, const double *qy, const double *qz,
+                          double *vx, double *vy, double *vz, double dt,
+                          double eps, double Sc);
 void tse_or_set_threads(int n);
 int tse_or_get_threads(void);
 

@@ -25,7 +25,8 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
           sor_t=1.0, conv_p=1e-8, conv_c=1e-8, conv_t=1e-8, nsteps=4,
           dt0=86400.0, dt_max=86400.0, beta=5.0, stretch=1.05,
           maxit_p=0, maxit_c=0, maxit_t=0, disp=(1.0, 0.1), wp_cell=True,
-          plot_steps=0, c_f=4.5e-10, salt_patch=True, t_end=None):
+          plot_steps=0, c_f=4.5e-10, salt_patch=True, t_end=None,
+          smolar_c=False, smolar_t=False, nlte=False):
     rng = numpy.random.default_rng(seed)
     S = species
     ns.update(dict(
@@ -37,8 +38,9 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
         FLUIDFLOW=True, SPECTRANS=True, HEATTRANS=heat, FLUIDPROP=True,
         CHEMREACT=False, DYNAMICBC=True, MASSBALAN=True, DTCHEM=1,
         C_IMBLFLO=c_imblflo, T_IMBLFLO=t_imblflo, SIA=sia,
-        SIA_maxit=sia_maxit, SIA_conv=sia_conv, SMOLAR_C=False,
-        SMOLAR_T=False, SPECIES=S, SPTRANS=numpy.array(sptrans),
+        SIA_maxit=sia_maxit, SIA_conv=sia_conv, SMOLAR_C=smolar_c,
+        SMOLAR_T=smolar_t, NLTE_SOLV=nlte, SPECIES=S,
+        SPTRANS=numpy.array(sptrans),
         dt=dt0, dt_max=dt_max, OMEGA_C=omega_c, OMEGA_T=omega_t,
         SOR_P=sor_p, SOR_C=sor_c, SOR_T=sor_t, CONV_P=conv_p,
         CONV_C=conv_c, CONV_T=conv_t, PRINTOUT=1, PLOTSAVE=False,
@@ -82,7 +84,8 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
     if S > 1:
         C[:, :, :, 1] = rng.uniform(0.0, 0.1, shp)
     ns['C'] = C
-    ns['T'] = numpy.full(shp, 10.0)
+    # NLTE: layer 0 fluid, layer 1 rock (tse.py:1343-1354)
+    ns['T'] = numpy.full(shp + (2,), 10.0) if nlte else numpy.full(shp, 10.0)
     ns['BP'] = numpy.zeros(shp)
     ns['BC'] = numpy.zeros(shp)
     ns['BT'] = numpy.zeros(shp)
@@ -91,6 +94,8 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
     if heat:
         ns['BT'][:, :, 0] = 1
         ns['T'][:, :, 0] = 60.0
+        if nlte:  # rock layer starts warmer than the fluid in the middle
+            ns['T'][nx // 2, :, nz // 2, 1] = 30.0
     ns['WP'] = numpy.zeros(shp)
     ns['WT'] = numpy.zeros(shp)
     ns['WC'] = numpy.zeros(shp + (S,))

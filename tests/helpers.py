@@ -6,7 +6,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ("g1_salt", "g2_sia_heat", "g3_halving")
+CASES = ("g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte")
 
 
 def oracle_backend(sim):

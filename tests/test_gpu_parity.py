@@ -159,6 +159,29 @@ def test_compute_t(shape, omega, sor, imbl):
     print("compute_T ok", shape, omega, "iters", ro[4], flush=True)
 
 
+@pytest.mark.parametrize("shape", GRIDS)
+def test_smolarkiewicz(shape):
+    """Anti-diffusive velocities from a species view with C <= 0 cells; the
+    last index of each axis keeps the previous vx / vy / vz."""
+    f = Fields(*shape, seed=9)
+    o, h = _pair(f)
+    rng = np.random.default_rng(10)
+    C4 = f.C.copy()
+    C4[..., 1] -= 0.2          # some cells <= 0
+    q = [1e-6 * rng.standard_normal(f.shape) for _ in range(3)]
+    v0 = [rng.standard_normal(f.shape) for _ in range(3)]
+    vo = [a.copy() for a in v0]
+    vh = [a.copy() for a in v0]
+    o.smolarkiewicz(C4[..., 1], *q, *vo, 3600.0)
+    h.smolarkiewicz(C4[..., 1], *q, *vh, 3600.0)
+    for a, b in zip(vo, vh):
+        assert np.array_equal(a, b)
+    keep = (C4[-1, :, :, 1] > 0)
+    assert np.array_equal(vo[0][-1][keep], v0[0][-1][keep])
+    assert np.all(vo[0][C4[..., 1] <= 0] == 0)
+    print("smolarkiewicz ok", shape, flush=True)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_driver_matches_reference_trajectory(case, tmp_path):
     """Full SIA driver on the GPU against the reference's golden vectors."""

@@ -38,3 +38,10 @@ def test_golden_cases_exercise_the_control_paths():
     assert log3.count("Reducing dt") >= 3
     log2 = open(f"{GOLDEN}/g2_sia_heat.log.txt").read()
     assert "[SIA]    1" in log2 or "[SIA]    2" in log2
+    # Smolarkiewicz passes double the transport solves; NLTE keeps two layers
+    g4 = np.load(f"{GOLDEN}/g4_smolar.npz")
+    assert "SMOLAR_C [on]" in open(f"{GOLDEN}/g4_smolar.log.txt").read()
+    assert g4["numitC"][-1] > 0 and g4["numitT"][-1] > 0
+    g5 = np.load(f"{GOLDEN}/g5_nlte.npz")
+    assert g5["T"].ndim == 5 and g5["T"].shape[-1] == 2
+    assert not np.array_equal(g5["T"][-1, ..., 0], g5["T"][-1, ..., 1])

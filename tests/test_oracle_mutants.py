@@ -40,6 +40,14 @@ MUTANTS = [
      "                halvings += 1;"),
     ("C retardation storage", "R = RC[c] * phi[c] / dt_C;", "R = RC[c] / dt_C;"),
     ("l2 norm squares", "s1 += d * d;", "s1 += fabs(d);"),
+    ("smolarkiewicz numerical-dispersion term dropped",
+     "vy[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[e] - C[c]) /",
+     "vy[c] = Sc * (fabs(q) * dd) * (C[e] - C[c]) /"),
+    ("smolarkiewicz gradient sign",
+     "vz[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[e] - C[c]) /",
+     "vz[c] = Sc * (fabs(q) * dd - dt * (q * q)) * (C[c] - C[e]) /"),
+    ("smolarkiewicz x flux component", "const double q = qx[c];",
+     "const double q = qy[c];"),
 ]
 
 

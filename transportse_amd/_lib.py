@@ -104,6 +104,7 @@ SIGNATURES = {
     "tse_compute_t": (_I, [_V, ctypes.POINTER(TFields),
                            ctypes.POINTER(SorParams), ctypes.POINTER(Result)]),
     "tse_l2_norm": (_I, [_V, _dp, _dp, _DPH]),
+    "tse_smolarkiewicz": (_I, [_V] + [_dp] * 7 + [_D, _D, _D]),
     "tse_host_l2_norm_seq": (_D, [_DPH, _DPH, _L, ctypes.POINTER(ctypes.c_int32)]),
     "tse_p_sweeps": (_I, [_V, ctypes.POINTER(PFields), _D, _D, ctypes.c_int32,
                            ctypes.c_int32]),

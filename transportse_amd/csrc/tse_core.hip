@@ -380,6 +380,16 @@ int tse_compute_t(tse_ctx *ctx, const tse_t_fields *f,
     return sor_solve(ctx, hk, prm, out);
 }
 
+int tse_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
+                      const double *qy, const double *qz, double *vx, double *vy,
+                      double *vz, double dt, double eps, double Sc) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!C || !qx || !qy || !qz || !vx || !vy || !vz)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    return launch_smolarkiewicz(ctx, C, qx, qy, qz, vx, vy, vz, dt, eps, Sc);
+}
+
 int tse_compute_q(tse_ctx *ctx, const double *P, const double *BP,
                   const double *Kx, const double *Ky, const double *Kz,
                   double *qx, double *qy, double *qz, const double *nue,

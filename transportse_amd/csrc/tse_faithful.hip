@@ -431,6 +431,47 @@ q_kernel(Geo G, const double *__restrict__ P, const double *__restrict__ BP,
     }
 }
 
+// ---- smolarkiewicz (tse.py:1083-1116) -------------------------------------------------
+// Anti-diffusive velocities: for C > 0, per axis with an upper neighbour,
+// Sc*(|q| dd - dt q*q)*(C_nb - C) / ((C + C_nb + eps) dd), left to right; the
+// last index of an axis keeps its value; C <= 0 zeroes all three.
+__device__ inline double smol_v(double Sc, double q, double dd, double dt,
+                                double c0, double c1, double eps) {
+    return Sc * (fabs(q) * dd - dt * (q * q)) * (c1 - c0) / ((c0 + c1 + eps) * dd);
+}
+
+__global__ void __launch_bounds__(kBlock)
+smolarkiewicz_kernel(Geo G, const double *__restrict__ C,
+                     const double *__restrict__ qx, const double *__restrict__ qy,
+                     const double *__restrict__ qz, double *__restrict__ vx,
+                     double *__restrict__ vy, double *__restrict__ vz, double dt,
+                     double eps, double Sc) {
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        if (!q.valid) continue;
+        const int i = q.i, j = q.j, k = q.k;
+        const double c0 = C[t];
+        if (c0 > 0) {
+            if (i < G.nx - 1)
+                vx[t] = smol_v(Sc, qx[t], 0.5 * (G.dx[i + 1] + G.dx[i]), dt, c0,
+                               C[slot(G, i + 1, j, k)], eps);
+            if (j < G.ny - 1)
+                vy[t] = smol_v(Sc, qy[t], 0.5 * (G.dy[j + 1] + G.dy[j]), dt, c0,
+                               C[slot(G, i, j + 1, k)], eps);
+            if (k < G.nz - 1)
+                vz[t] = smol_v(Sc, qz[t], 0.5 * (G.dz[k + 1] + G.dz[k]), dt, c0,
+                               C[slot(G, i, j, k + 1)], eps);
+        } else {
+            vx[t] = 0.0;
+            vy[t] = 0.0;
+            vz[t] = 0.0;
+        }
+    }
+}
+
 // ---- flow_error (tse.py:409-446) -------------------------------------------------------
 __global__ void __launch_bounds__(kBlock)
 flow_error_kernel(Geo G, const double *__restrict__ P,
@@ -884,6 +925,18 @@ int launch_q(tse_ctx *ctx, const double *P, const double *BP,
     hipLaunchKernelGGL(q_kernel, dim3(grid_for(2L * G.S)), dim3(kBlock), 0,
                        ctx->stream, G, P, BP, Kx, Ky, Kz, qx, qy, qz, nue,
                        rho_f);
+    TSE_HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int launch_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
+                         const double *qy, const double *qz, double *vx,
+                         double *vy, double *vz, double dt, double eps,
+                         double Sc) {
+    const Geo &G = ctx->geo;
+    hipLaunchKernelGGL(smolarkiewicz_kernel, dim3(grid_for(2L * G.S)),
+                       dim3(kBlock), 0, ctx->stream, G, C, qx, qy, qz, vx, vy,
+                       vz, dt, eps, Sc);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }

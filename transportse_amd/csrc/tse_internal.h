@@ -168,6 +168,10 @@ int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
 int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
                     double omega);
 int launch_l2(tse_ctx *ctx, const double *New, const double *Old);
+int launch_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
+                         const double *qy, const double *qz, double *vx,
+                         double *vy, double *vz, double dt, double eps,
+                         double Sc);
 // implemented in tse_fast.hip
 void fast_free(tse_ctx *ctx);
 }  // namespace tse

@@ -16,8 +16,9 @@ order of operations and the state handling follow the reference exactly:
 
 The solver backend is any object with compute_P, compute_q, flow_error,
 compute_C, compute_T and l2_norm; ``make_hip_backend`` builds the GPU one.
-Not covered yet (raise NotImplementedError): SMOLAR_C / SMOLAR_T
-(tse.py:1318-1336, 1368-1381) and NLTE two-phase heat (1343-1354).
+The Smolarkiewicz correction passes (SMOLAR_C / SMOLAR_T, tse.py:1318-1336,
+1368-1381) and NLTE two-layer heat (NLTE_SOLV, tse.py:1343-1354; the fluid
+properties then act on layer 0, tse.py:207-226) follow the reference.
 """
 from __future__ import annotations
 
@@ -88,10 +89,7 @@ class Simulation:
         self.MAXIT_P = c.MAXIT_P if getattr(c, "MAXIT_P", 0) else 20000
         self.MAXIT_C = c.MAXIT_C if getattr(c, "MAXIT_C", 0) else 100
         self.MAXIT_T = c.MAXIT_T if getattr(c, "MAXIT_T", 0) else 100
-        if getattr(c, "SMOLAR_C", False) or getattr(c, "SMOLAR_T", False):
-            raise NotImplementedError("Smolarkiewicz correction pass")
-        if getattr(c, "NLTE_SOLV", False):
-            raise NotImplementedError("NLTE two-phase heat transport")
+        self.NLTE = bool(getattr(c, "NLTE_SOLV", False))   # 'NLTE_SOLV' in dir(config)
         self._eos_vectorized = None
         if c.FLUIDPROP:
             self.update_fluid_properties()
@@ -100,14 +98,20 @@ class Simulation:
         self.trajectory = []
 
     # ---- EOS (tse.py:178-229) -------------------------------------------------
-    def _eos_cell(self, i, j, k):
+    def _fp_arrays(self):
+        """The fluid_properties operands; a 4-D (NLTE) T contributes its
+        fluid layer 0 (tse.py:207-226)."""
         st = self.st
-        args = [getattr(st, n)[i, j, k] for n in _FP_NAMES]
+        return [st.T[:, :, :, 0] if n == "T" and st.T.ndim == 4 else getattr(st, n)
+                for n in _FP_NAMES]
+
+    def _eos_cell(self, i, j, k):
+        args = [a[i, j, k] for a in self._fp_arrays()]
         return self.config.fluid_properties(*args)
 
     def _eos_loop(self):
         st = self.st
-        arrs = [getattr(st, n) for n in _FP_NAMES]
+        arrs = self._fp_arrays()
         fp = self.config.fluid_properties
         for k in range(st.nz):
             for j in range(st.ny):
@@ -118,15 +122,13 @@ class Simulation:
 
     def _eos_vector(self):
         """Call fluid_properties once on whole fields (species axis first)."""
-        st = self.st
+        arrs = self._fp_arrays()
         args = []
-        for n in _FP_NAMES:
-            a = getattr(st, n)
+        for a in arrs:
             args.append(numpy.moveaxis(a, -1, 0) if a.ndim == 4 else a)
         out = self.config.fluid_properties(*args)
         res = []
-        for n, v in zip(_FP_NAMES, out):
-            a = getattr(st, n)
+        for a, v in zip(arrs, out):
             v = numpy.asarray(v, dtype=numpy.float64)
             if a.ndim == 4 and v.ndim == 4:
                 v = numpy.moveaxis(v, 0, -1)  # back to species-last
@@ -134,9 +136,6 @@ class Simulation:
         return res
 
     def update_fluid_properties(self):
-        st = self.st
-        if st.T.ndim != 3:
-            raise NotImplementedError("4-D (NLTE) temperature arrays")
         if self.eos_mode == "loop" or self._eos_vectorized is False:
             self._eos_loop()
             return
@@ -150,8 +149,8 @@ class Simulation:
             self._eos_loop()
             return
         self._eos_vectorized = True
-        for n, v in zip(_FP_NAMES, res):
-            getattr(st, n)[...] = v
+        for a, v in zip(self._fp_arrays(), res):
+            a[...] = v
 
     def _validate_eos(self, res) -> bool:
         """Check the whole-field call against the per-cell contract on a sample
@@ -350,21 +349,72 @@ class Simulation:
                             if not converged_C:
                                 raise ConvergenceError(
                                     '* C CONVERGENCE ERROR- check dt and C_conv in compute_C')
+                            if c.SMOLAR_C:  # tse.py:1318-1336
+                                t0 = time.perf_counter()
+                                st.vx, st.vy, st.vz = sv.smolarkiewicz(
+                                    st.C[:, :, :, n], st.qx, st.qy, st.qz, st.vx,
+                                    st.vy, st.vz, self.dt, eps=1e-15, Sc=1.08)
+                                self.dt, dt_C, st.C[:, :, :, n], converged_C, num_itC = \
+                                    sv.compute_C(st.C[:, :, :, n], st.D_fx[:, :, :, n],
+                                                 st.D_fy[:, :, :, n], st.D_fz[:, :, :, n],
+                                                 st.phi, st.rho_f, st.vx, st.vy, st.vz,
+                                                 self.dt, st.BC, st.WC[:, :, :, n], st.R,
+                                                 st.ZEROS, omega=c.OMEGA_C,
+                                                 max_it=self.MAXIT_C, SOR=c.SOR_C,
+                                                 C_conv=c.CONV_C)
+                                solve_time += time.perf_counter() - t0
+                                num_it_C += num_itC
+                                if not converged_C:
+                                    raise ConvergenceError(
+                                        '* C2 CONVERGENCE ERROR- check dt and C_conv in compute_C')
 
                 num_it_T = 0
                 if c.HEATTRANS:
                     Tn = st.T.copy()
                     t0 = time.perf_counter()
-                    self.dt, dt_T, st.T, converged_T, num_itT = \
-                        sv.compute_T(st.T, st.l_f, st.cp_f, st.rho_f, st.phi,
-                                     st.qx, st.qy, st.qz, self.dt, st.BT, st.WT,
-                                     st.e, omega=c.OMEGA_T, max_it=self.MAXIT_T,
-                                     SOR=c.SOR_T, T_conv=c.CONV_T)
+                    if self.NLTE:  # tse.py:1343-1354
+                        # fluid layer: unit porosity, no source
+                        self.dt, dt_T, st.T[:, :, :, 0], converged_T, num_itT1 = \
+                            sv.compute_T(st.T[:, :, :, 0], st.l_f, st.cp_f, st.rho_f,
+                                         st.ONES, st.qx, st.qy, st.qz, self.dt,
+                                         st.BT, st.ZEROS, st.e, omega=c.OMEGA_T,
+                                         max_it=self.MAXIT_T, SOR=c.SOR_T,
+                                         T_conv=c.CONV_T)
+                        # rock layer: zero fluid properties
+                        self.dt, dt_T, st.T[:, :, :, 1], converged_T, num_itT2 = \
+                            sv.compute_T(st.T[:, :, :, 1], st.ZEROS, st.ZEROS,
+                                         st.ZEROS, st.ZEROS, st.qx, st.qy, st.qz,
+                                         self.dt, st.BT, st.WT, st.e,
+                                         omega=c.OMEGA_T, max_it=self.MAXIT_T,
+                                         SOR=c.SOR_T, T_conv=c.CONV_T)
+                        num_itT = num_itT1 + num_itT2
+                    else:
+                        self.dt, dt_T, st.T, converged_T, num_itT = \
+                            sv.compute_T(st.T, st.l_f, st.cp_f, st.rho_f, st.phi,
+                                         st.qx, st.qy, st.qz, self.dt, st.BT, st.WT,
+                                         st.e, omega=c.OMEGA_T, max_it=self.MAXIT_T,
+                                         SOR=c.SOR_T, T_conv=c.CONV_T)
                     solve_time += time.perf_counter() - t0
                     num_it_T += num_itT
                     if not converged_T:
                         raise ConvergenceError(
                             '* T CONVERGENCE ERROR - check dt and T_conv in compute_T')
+                    if c.SMOLAR_T:  # tse.py:1368-1381
+                        t0 = time.perf_counter()
+                        st.vx, st.vy, st.vz = sv.smolarkiewicz(
+                            st.T, st.qx, st.qy, st.qz, st.vx, st.vy, st.vz,
+                            self.dt, eps=1e-15, Sc=1.08)
+                        self.dt, dt_T, st.T, converged_T, num_itT = \
+                            sv.compute_T(st.T, st.l_f, st.cp_f, st.rho_f, st.phi,
+                                         st.vx, st.vy, st.vz, self.dt, st.BT, st.WT,
+                                         st.ZEROS, omega=c.OMEGA_T,
+                                         max_it=self.MAXIT_T, SOR=c.SOR_T,
+                                         T_conv=c.CONV_T)
+                        solve_time += time.perf_counter() - t0
+                        num_it_T += num_itT
+                        if not converged_T:
+                            raise ConvergenceError(
+                                '* T CONVERGENCE ERROR - check dt and T_conv in compute_T')
 
                 if c.CHEMREACT:
                     if self.t_sim > 0:

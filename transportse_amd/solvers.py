@@ -238,6 +238,23 @@ class HipSolvers:
         self.last = res
         return res.t, res.dt, T, bool(res.converged), int(res.num_it)
 
+    def smolarkiewicz(self, C, qx, qy, qz, vx, vy, vz, dt, eps=1e-15, Sc=1.08):
+        """smolarkiewicz (tse.py:1083-1116): updates vx, vy, vz in place and
+        returns them."""
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (
+            ("smol_C", C), ("qx", qx), ("qy", qy), ("qz", qz),
+            ("vx", vx), ("vy", vy), ("vz", vz))}
+        _lib.check(self.lib.tse_smolarkiewicz(
+            self.hip.ctx, *[d[n].data_ptr() for n in
+                            ("smol_C", "qx", "qy", "qz", "vx", "vy", "vz")],
+            float(dt), float(eps), float(Sc)), "tse_smolarkiewicz")
+        for n, a in (("vx", vx), ("vy", vy), ("vz", vz)):
+            self.hip.download(d[n], out=a)
+            self._mark(n, a, d[n])
+        self.stats["downloads"] += 3
+        return vx, vy, vz
+
     def l2_norm(self, New, Old, Process=""):
         """l2_norm (tse.py:273-281) on host arrays, Numba's sequential sums."""
         a = np.ascontiguousarray(New, dtype=np.float64)
