@@ -19,8 +19,11 @@ Reported (one JSON line on stdout):
                to solution on the same problem and the speedup over it, and
                the mixed solution's deviation from the FP64 solution.
 
-Further sections of the same line (each bounded; --no-sia / --no-ensemble /
---no-serve skip them):
+Further sections of the same line (each bounded; --no-configs / --no-sia /
+--no-ensemble / --no-serve skip them):
+  configs_faithful64  configs A (64x32x32, 100 steps) and B (256x256x128,
+               2 steps) through the SIA driver with the faithful FP64
+               solvers; A also with the oracle on the host cores (10 steps)
   sia          device-resident SIA steps (heat + salt + tracer) on the
                512x512x256 problem in the mixed form and in plain FP64 solved
                to 1e-13, and the relative L2 / max error of P, T and each C
@@ -79,13 +82,18 @@ def main(argv=None):
     ap.add_argument("--plan", type=int, nargs="+", default=None,
                     metavar="N", help="FP32 sweep plan NS BY RY ZCHUNK [VARIANT]"
                     " instead of autotuning")
+    ap.add_argument("--no-configs", action="store_true")
+    ap.add_argument("--a-steps", type=int, default=100)
+    ap.add_argument("--a-cpu-steps", type=int, default=10)
+    ap.add_argument("--b-steps", type=int, default=2)
     ap.add_argument("--no-sia", action="store_true")
     ap.add_argument("--sia-steps", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-rates", type=float, nargs="+", default=[5.0])
+    ap.add_argument("--serve-rates", type=float, nargs="+",
+                    default=[5.0, 50.0, 200.0])
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-steps", type=int, default=10)
     args = ap.parse_args(argv)
@@ -216,6 +224,7 @@ def main(argv=None):
 
     del fp
     torch.cuda.empty_cache()
+    cfgs = None if args.no_configs else configs_section(args)
     sia = None if args.no_sia else sia_section(args, nx, ny, nz)
     ens = None if args.no_ensemble else ensemble_section(args)
     serving = None if args.no_serve else serve_section(args)
@@ -257,12 +266,71 @@ def main(argv=None):
                      "bytes_per_update": BYTES_FP32_SWEEP,
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,
+        "configs_faithful64": cfgs,
         "sia": sia,
         "ensemble": ens,
         "serving": serving,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def configs_section(args):
+    """Configs A (64x32x32) and B (256x256x128) through the SIA driver with the
+    faithful FP64 solvers (bit-identical to the reference arithmetic); A also
+    with the oracle on the host cores for a bounded number of steps."""
+    import torch
+
+    from transportse_amd.driver import Simulation, make_hip_backend
+    cfg_dir = os.path.join(ROOT, "transportse_amd", "configs")
+    out = {}
+    for name, cfg, steps in (("A", "baseline_a.py", args.a_steps),
+                             ("B", "baseline_b.py", args.b_steps)):
+        sim = Simulation(os.path.join(cfg_dir, cfg), make_hip_backend,
+                         verbose=False, max_steps=steps)
+        n = sim.st.nx * sim.st.ny * sim.st.nz
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run()
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        sm = sim.summary
+        rec = {"grid": [sim.st.nx, sim.st.ny, sim.st.nz], "steps": sm["n_tsteps"],
+               "seconds": sec, "steps_per_s": sm["n_tsteps"] / sec,
+               "p_sweeps": sm["numitP"], "c_sweeps": sm["numitC"],
+               "t_sweeps": sm["numitT"], "solver_seconds": sim.solve_time,
+               "p_cell_updates_per_s_end_to_end": sm["numitP"] * n / sec}
+        log(f"config {name} faithful FP64: {sm['n_tsteps']} steps in {sec:.2f}s, "
+            f"P sweeps {sm['numitP']}")
+        out[name] = rec
+        del sim
+    if args.a_cpu_steps and not args.no_cpu_baseline:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as orc
+        orc.build()
+        threads = len(os.sched_getaffinity(0))
+        orc.set_threads(threads)
+
+        def oracle_backend(sim):
+            st, c = sim.st, sim.config
+            return orc.Oracle(st.nx, st.ny, st.nz, st.dx, st.dy, st.dz, st.g,
+                              st.beta_deg, frozen_source=st, maxit_p=sim.MAXIT_P,
+                              maxit_c=sim.MAXIT_C, maxit_t=sim.MAXIT_T,
+                              dt_max=c.dt_max, c_imblflo=c.C_IMBLFLO,
+                              t_imblflo=c.T_IMBLFLO)
+        sim = Simulation(os.path.join(cfg_dir, "baseline_a.py"), oracle_backend,
+                         verbose=False, max_steps=args.a_cpu_steps)
+        t0 = time.perf_counter()
+        sim.run()
+        sec = time.perf_counter() - t0
+        out["A"]["cpu_oracle"] = {"steps": sim.summary["n_tsteps"], "seconds": sec,
+                                  "steps_per_s": sim.summary["n_tsteps"] / sec,
+                                  "cores": threads,
+                                  "gpu_speedup": out["A"]["steps_per_s"]
+                                  / (sim.summary["n_tsteps"] / sec)}
+        log(f"config A oracle on {threads} host threads: "
+            f"{sim.summary['n_tsteps']} steps in {sec:.2f}s")
+    return out
 
 
 def sia_section(args, nx, ny, nz):
@@ -343,6 +411,7 @@ def ensemble_section(args):
         f"{st.cell_updates_per_s:.3e} cell-updates/s")
     return {"realizations": len(specs), "cells": cells, "steps": args.ens_steps,
             "seconds": st.seconds, "seconds_per_step": st.seconds / args.ens_steps,
+            "realization_steps_per_s": len(specs) * args.ens_steps / st.seconds,
             "cell_updates_per_s": st.cell_updates_per_s, "solves": st.solves,
             "sweeps_p": st.sweeps_p, "sweeps_tr": st.sweeps_tr,
             "cycles": st.cycles, "graph": True,

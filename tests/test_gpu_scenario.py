@@ -117,3 +117,36 @@ def test_scenario_steps_match_oracle(form, tol, bp, bx):
     for a, b in zip(m, exp):
         assert abs(a - b) <= 1e-12 * max(abs(b), 1e-300)
     scn.close()
+
+
+def test_scenario_snapshot_pickle(tmp_path):
+    """The device-side snapshot writes the 34-tuple of the state at the time
+    of the call while the scenario goes on stepping."""
+    from joblib import load
+
+    from transportse_amd.scenario import DeviceScenario
+    from transportse_amd.snapshot import PickleWriter
+    spec = _spec()
+    scn = DeviceScenario(spec)
+    kw = dict(form="mixed", tol=1e-10, plan_p=(2, 16, 1, 0), plan_tr=(2, 16, 1, 0),
+              sor_p=1.9, sor_tr=1.0, inner_p=20, inner_tr=4)
+    scn.run(1, **kw)
+    ref = scn.download()
+    qx = scn.hip.download(scn.qx)
+    e = scn.hip.download(scn.e)
+    w = PickleWriter()
+    path = str(tmp_path / "scn-00000.p")
+    scn.snapshot(path, w)
+    scn.run(1, **kw)            # the next step overwrites the device fields
+    w.close()
+    tup = load(path)
+    assert len(tup) == 34
+    assert tup[9] == spec.dt and (tup[3], tup[4], tup[5]) == (spec.nx, spec.ny, spec.nz)
+    assert np.array_equal(tup[11], ref["P"]) and np.array_equal(tup[13], ref["T"])
+    assert tup[12].shape == (spec.nx, spec.ny, spec.nz, spec.species)
+    for s_ in range(spec.species):
+        assert np.array_equal(tup[12][..., s_], ref["C"][s_])
+    assert np.array_equal(tup[14], qx) and np.array_equal(tup[33], e)
+    assert np.array_equal(tup[21], ref["rho_f"])
+    assert not np.array_equal(scn.download()["P"], ref["P"])
+    scn.close()

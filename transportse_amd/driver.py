@@ -30,6 +30,7 @@ import types
 import numpy
 
 from .config_loader import load_config
+from .snapshot import PickleWriter
 
 # config attributes the reference binds at start-up (tse.py:81-151)
 _STATE_ARRAYS = ("phi", "Kx", "Ky", "Kz", "c_r", "cp_r", "l_r", "rho_r",
@@ -96,6 +97,7 @@ class Simulation:
         # the backend snapshots frozen globals at first call from `st`
         self.solvers = backend_factory(self)
         self.trajectory = []
+        self.writer = PickleWriter()
 
     # ---- EOS (tse.py:178-229) -------------------------------------------------
     def _fp_arrays(self):
@@ -217,16 +219,19 @@ class Simulation:
                 self.dt_prev = self.dt
                 self.dt = min(self.dt, t_plot - self.t_sim)
             elif self.t_sim == t_plot:
-                from joblib import dump
                 path = os.path.join(self.out_dir, "{:s}-{:05d}.p".format(
                     self.cfile, self.PLOTINDEX))
-                with open(path, "wb") as pfile:
-                    dump((st.lx, st.ly, st.lz, st.nx, st.ny, st.nz, st.dx,
-                          st.dy, st.dz, self.t_sim, self.dt, st.P, st.C, st.T,
-                          st.qx, st.qy, st.qz, st.phi, st.Kx, st.Ky, st.Kz,
-                          st.rho_f, st.nue, st.cp_f, st.l_f, st.BP, st.BC,
-                          st.BT, st.L, st.mass, st.WP, st.WT, st.WC, st.e),
-                         pfile, compress=3)
+                # the 34-tuple of tse.py:322-327; arrays are snapshot-copied so
+                # the compressed dump runs on the writer thread while the time
+                # loop continues
+                snap = tuple(numpy.array(v, copy=True) if isinstance(v, numpy.ndarray)
+                             else v for v in (
+                    st.lx, st.ly, st.lz, st.nx, st.ny, st.nz, st.dx, st.dy, st.dz,
+                    self.t_sim, self.dt, st.P, st.C, st.T, st.qx, st.qy, st.qz,
+                    st.phi, st.Kx, st.Ky, st.Kz, st.rho_f, st.nue, st.cp_f,
+                    st.l_f, st.BP, st.BC, st.BT, st.L, st.mass, st.WP, st.WT,
+                    st.WC, st.e))
+                self.writer.submit(path, snap)
                 self.dt = self.dt_prev
                 self.PLOTINDEX += 1
 
@@ -241,6 +246,14 @@ class Simulation:
 
     # ---- main loop (tse.py:1118-1494) --------------------------------------------------
     def run(self):
+        """Run the time loop; pickles still being written are finished before
+        this returns (or re-raises)."""
+        try:
+            return self._run()
+        finally:
+            self.writer.flush()
+
+    def _run(self):
         c, st, sv = self.config, self.st, self.solvers
         self.dt = c.dt
         self.dt_prev = self.dt

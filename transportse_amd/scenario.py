@@ -332,6 +332,65 @@ class DeviceScenario:
             out.append((kind, r))
         return out
 
+    def snapshot(self, path: str, writer) -> None:
+        """Queue the 34-tuple pickle of tse.py:322-327 for the current state
+        on `writer` (a snapshot.PickleWriter).  The fields are converted to
+        the reference layout on the compute stream; the copy to pinned host
+        memory runs on a side stream and the writer thread waits for it, so
+        the caller may go on with the next step at once."""
+        import torch
+        sp, hip, L = self.spec, self.hip, self.lib
+        nx, ny, nz, S = sp.nx, sp.ny, sp.nz, sp.species
+        N = nx * ny * nz
+        own = (("P", self.P), ("T", self.T), ("qx", self.qx), ("qy", self.qy),
+               ("qz", self.qz), ("K", self.K), ("rho_f", self.rho),
+               ("l_f", self.l_f), ("BP", self.BP), ("BC", self.BC),
+               ("BT", self.BT), ("e", self.e))
+        stage = torch.empty((len(own) + S) * N, dtype=torch.float64,
+                            device=hip.device)
+        base = stage.data_ptr()
+        hip.sync_stream()
+        for n, (_, t) in enumerate(own):
+            _lib.check(L.tse_from_split(hip.ctx, base + 8 * n * N, ny * nz, nz, 1,
+                                        t.data_ptr()), "tse_from_split")
+        c0 = len(own) * N
+        for s_ in range(S):   # species last: C[i, j, k, s]
+            _lib.check(L.tse_from_split(hip.ctx, base + 8 * (c0 + s_),
+                                        ny * nz * S, nz * S, S,
+                                        self.C[s_].data_ptr()), "tse_from_split")
+        host = torch.empty(stage.numel(), dtype=torch.float64, pin_memory=True)
+        if not hasattr(self, "_copy_stream"):
+            self._copy_stream = torch.cuda.Stream(hip.device)   # default priority
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(hip.device))
+        self._copy_stream.wait_event(ready)
+        with torch.cuda.stream(self._copy_stream):
+            host.copy_(stage, non_blocking=True)
+        done = torch.cuda.Event()
+        done.record(self._copy_stream)
+        stage.record_stream(self._copy_stream)
+        t_sim, dt = self.t, sp.dt
+        mass = np.array(self.history[-1].mass if self.history else (0.0, 0.0, 0.0))
+        dxs = np.full(nx, float(sp.spacing))
+        dys = np.full(ny, float(sp.spacing))
+        dzs = np.full(nz, float(sp.spacing))
+
+        def build():
+            h = host.numpy()
+            f = {name: h[n * N:(n + 1) * N].reshape(nx, ny, nz)
+                 for n, (name, _) in enumerate(own)}
+            C = h[c0:c0 + S * N].reshape(nx, ny, nz, S)
+            shp = (nx, ny, nz)
+            full = lambda name: np.full(shp, UNIFORM[name])
+            K = f["K"]
+            return (float(dxs.sum()), float(dys.sum()), float(dzs.sum()), nx, ny,
+                    nz, dxs, dys, dzs, t_sim, dt, f["P"], C, f["T"], f["qx"],
+                    f["qy"], f["qz"], full("phi"), K, K, K, f["rho_f"],
+                    full("nue"), full("cp_f"), f["l_f"], f["BP"], f["BC"],
+                    f["BT"], np.zeros(shp), mass, np.zeros(shp), np.zeros(shp),
+                    np.zeros(shp + (S,)), f["e"])
+        writer.submit(path, build, wait=done.synchronize)
+
     def download(self) -> dict:
         h = self.hip
         return {"P": h.download(self.P), "T": h.download(self.T),
