@@ -4,7 +4,7 @@ Hand-written HIP kernels behind a C-ABI (include/tse_hip.h), called from
 Python through ctypes, with the reference's solver call signatures:
 
 * ``HipSolvers``   -- compute_P / compute_q / flow_error / compute_C /
-  compute_T / l2_norm for one grid (solvers.py)
+  compute_T / l2_norm / smolarkiewicz for one grid (solvers.py)
 * ``Simulation``   -- the time-step / SIA driver that calls them
   (driver.py), ``make_hip_backend`` builds its GPU backend
 * ``load_config``  -- imports a TRANSPORTSE config module (config_loader.py)
@@ -57,6 +57,10 @@ def compute_T(*a, **k):
 
 def l2_norm(*a, **k):
     return _solvers().l2_norm(*a, **k)
+
+
+def smolarkiewicz(*a, **k):
+    return _solvers().smolarkiewicz(*a, **k)
 
 
 def __getattr__(name):
