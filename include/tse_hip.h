@@ -278,6 +278,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm);
 int tse_batch_add(tse_batch *b, tse_ctx *member);
 /* Restarts a member after its fast system was set up again (next solve). */
 int tse_batch_rearm(tse_batch *b, int32_t slot);
+/* Restarts n members at once (one synchronisation). */
+int tse_batch_rearm_many(tse_batch *b, const int32_t *slots, int32_t n);
 /* Hands the member's iterate back to its context (tse_fast_*_result). */
 int tse_batch_writeback(tse_batch *b, int32_t slot);
 /* Over-relaxation factor of one member (default: the batch's sor). */

@@ -2,7 +2,8 @@
 serving loop (continuous batching), on small grids.
 
 * Ensemble: every realization must end bit-identical whether the batched
-  cycles are replayed from a captured hipGraph or launched directly, and
+  cycles are replayed from a captured hipGraph or launched directly,
+  whether the realizations advance asynchronously or stage by stage, and
   whether the realization runs inside the ensemble or in an ensemble of
   its own; its fields must also match the single-system scenario driver
   (same sweeps and residual tests, residual norms summed in another order)
@@ -69,6 +70,12 @@ def test_ensemble_graph_alone_single():
     e3.run(2)
     assert _same(_fields(e3)[0], f1[4])
     e3.close()
+    e4 = Ensemble(specs, graph=True, **KW)     # stage-by-stage driver
+    st4 = e4.run(2, mode="lockstep")
+    assert st4.solves == st.solves
+    for a, b in zip(f1, _fields(e4)):
+        assert _same(a, b)
+    e4.close()
     scn = DeviceScenario(specs[4])
     # the pressure factor comes from the scenario's own tuner, as in the ensemble
     scn.run(2, form="mixed", tol=1e-10, plan_p=KW["plan_p"], plan_tr=KW["plan_tr"],

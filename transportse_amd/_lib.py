@@ -128,6 +128,7 @@ SIGNATURES = {
     "tse_batch_begin": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastParams)]),
     "tse_batch_add": (_I, [_V, _V]),
     "tse_batch_rearm": (_I, [_V, ctypes.c_int32]),
+    "tse_batch_rearm_many": (_I, [_V, ctypes.POINTER(ctypes.c_int32), ctypes.c_int32]),
     "tse_batch_writeback": (_I, [_V, ctypes.c_int32]),
     "tse_batch_remove": (_I, [_V, ctypes.c_int32]),
     "tse_batch_set_sor": (_I, [_V, ctypes.c_int32, _D]),

@@ -71,6 +71,13 @@ class Batch:
     def rearm(self, slot: int):
         _lib.check(self.lib.tse_batch_rearm(self.h, slot), "tse_batch_rearm")
 
+    def rearm_many(self, slots):
+        if not slots:
+            return
+        arr = (ctypes.c_int32 * len(slots))(*slots)
+        _lib.check(self.lib.tse_batch_rearm_many(self.h, arr, len(slots)),
+                   "tse_batch_rearm_many")
+
     def writeback(self, slot: int):
         _lib.check(self.lib.tse_batch_writeback(self.h, slot), "tse_batch_writeback")
 

@@ -629,6 +629,54 @@ int tse_batch_rearm(tse_batch *b, int32_t slot) {
     return init_member(b, slot);
 }
 
+// Re-arms several members at once: one synchronisation for all of them
+// (each member's initial residual still runs over its own chunks).
+int tse_batch_rearm_many(tse_batch *b, const int32_t *slots, int32_t n) {
+    int rc = check_batch(b);
+    if (rc) return rc;
+    if (n < 0 || (n > 0 && !slots)) return fail(TSE_ERR_ARG, "bad slot list");
+    for (int q = 0; q < n; ++q) {
+        const int slot = slots[q];
+        if (slot < 0 || slot >= b->cap || !b->hused[slot])
+            return fail(TSE_ERR_ARG, "no member in that slot");
+        tse_ctx *c = b->ctx[slot];
+        FastSys *s = member_sys(b, c);
+        if (!s || !s->ready) return fail(TSE_ERR_STATE, "member's fast system not set up");
+        if (c->stream != b->stream) TSE_B_TRY(hipStreamSynchronize(c->stream));
+        BMember m = member_entry(b, c, s);
+        const BMember &o = b->hmem[slot];
+        m.sor = o.sor;
+        if (m.N != o.N || m.p[0] != o.p[0] || m.p[1] != o.p[1] ||
+            m.e[0] != o.e[0] || m.nx != o.nx || m.ny != o.ny) {
+            b->hmem[slot] = m;
+            b->dirty = true;
+        }
+    }
+    rc = prepare(b);
+    if (rc) return rc;
+    const bool fp32 = b->form == TSE_FORM_MIXED;
+    for (int q = 0; q < n; ++q) {
+        const int slot = slots[q];
+        BStat st0{};
+        st0.cur = member_sys(b, b->ctx[slot])->cur;
+        b->hst[slot] = st0;
+        TSE_B_TRY(hipMemcpyAsync(b->dst + slot, &b->hst[slot], sizeof(BStat),
+                                 hipMemcpyHostToDevice, b->stream));
+        const BMember &mm = b->hmem[slot];
+        const int nch = (int)((mm.N + kChunk - 1) / kChunk);
+        rc = residual_any(b, fp32 ? 2 : 0, mm.rchunk0, nch, 0, 0, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_reduce_kernel, dim3(1), dim3(kBlock), 0, b->stream,
+                           b->dmem, b->dst, b->dused, slot, b->dpartials, 1, 0,
+                           b->inner, b->max_sweeps, b->tol);
+        TSE_B_TRY(hipGetLastError());
+    }
+    TSE_B_TRY(hipMemcpyAsync(b->hst, b->dst, sizeof(BStat) * b->cap,
+                             hipMemcpyDeviceToHost, b->stream));
+    TSE_B_TRY(hipStreamSynchronize(b->stream));
+    return 0;
+}
+
 // Hands the member's current iterate back to its context (the fast-form
 // result / residual calls then see it) without removing it.
 int tse_batch_writeback(tse_batch *b, int32_t slot) {
