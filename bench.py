@@ -230,8 +230,15 @@ def main(argv=None):
     serving = None if args.no_serve else serve_section(args)
 
     traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32) + (0,) * (5 - len(plan32)))
+    srv0 = serving["runs"][0] if serving and serving.get("runs") else None
     line = {
-        "metric": "SOR cell-updates/s",
+        "metric": "SOR cell-updates/s & HBM roofline %; solves/s + p99 at rel. "
+                  "residual 1e-10",
+        "value_is": "SOR cell-updates/s of the 512x512x256 mixed solve",
+        "hbm_roofline_pct": 100.0 * achieved / HBM_PEAK_GBS,
+        "serving_solves_per_s": srv0["solves_per_s"] if srv0 else None,
+        "serving_p99_s": srv0["p99_s"] if srv0 else None,
+        "serving_rate_per_s": srv0["rate_per_s"] if srv0 else None,
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
