@@ -59,6 +59,9 @@ class HipContext:
         self.F = int(self.lib.tse_field_elems(self.ctx))
         self.nxh = (self.nx + 1) // 2
         self._staging = None
+        self._up = None        # device staging of uploads
+        self._pin = None       # pinned host staging (both directions)
+        self._pin_free = None  # event: last copy reading / writing _pin done
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -97,13 +100,32 @@ class HipContext:
         # keep the element strides of the host view when its memory is one
         # contiguous block (e.g. a species slice of a C-order 4-D array)
         base, strides = _block_and_strides(a)
-        src = torch.from_numpy(base).to(self.device)
+        n = base.size
+        pin = self._pinned(n)
+        np.copyto(pin.numpy()[:n], base)
+        if self._up is None or self._up.numel() < n:
+            self._up = torch.empty(n, dtype=torch.float64, device=self.device)
         if out is None:
             out = self.empty()
         self.sync_stream()
-        _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), src.data_ptr(),
+        stream = torch.cuda.current_stream(self.device)
+        self._up[:n].copy_(pin[:n], non_blocking=True)   # DMA from pinned memory
+        _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), self._up.data_ptr(),
                                           *strides), "tse_to_split")
+        self._pin_free = torch.cuda.Event()
+        self._pin_free.record(stream)
         return out
+
+    def _pinned(self, n: int):
+        """The pinned host staging buffer (one per context, reused), once the
+        last copy that read it has finished."""
+        torch = _torch()
+        if self._pin_free is not None:
+            self._pin_free.synchronize()
+            self._pin_free = None
+        if self._pin is None or self._pin.numel() < n:
+            self._pin = torch.empty(n, dtype=torch.float64, pin_memory=True)
+        return self._pin
 
     def download(self, field, out: np.ndarray | None = None) -> np.ndarray:
         """Copy a split field into a host (nx,ny,nz) array (new, or `out`)."""
@@ -116,9 +138,12 @@ class HipContext:
         _lib.check(self.lib.tse_from_split(self.ctx, st.data_ptr(),
                                             self.ny * self.nz, self.nz, 1,
                                             field.data_ptr()), "tse_from_split")
-        host = st.cpu().numpy().reshape(self.shape)
+        pin = self._pinned(n)
+        pin[:n].copy_(st, non_blocking=True)            # DMA into pinned memory
+        torch.cuda.current_stream(self.device).synchronize()
+        host = pin[:n].numpy().reshape(self.shape)
         if out is None:
-            return host
+            return host.copy()
         out[...] = host
         return out
 

@@ -127,8 +127,13 @@ class Simulation:
         arrs = self._fp_arrays()
         args = []
         for a in arrs:
-            args.append(numpy.moveaxis(a, -1, 0) if a.ndim == 4 else a)
+            v = (numpy.moveaxis(a, -1, 0) if a.ndim == 4 else a).view()
+            # read-only: an argument returned unchanged is known to be
+            # unchanged (an in-place write raises and selects the loop)
+            v.flags.writeable = False
+            args.append(v)
         out = self.config.fluid_properties(*args)
+        self._eos_same = [v is arg for v, arg in zip(out, args)]
         res = []
         for a, v in zip(arrs, out):
             v = numpy.asarray(v, dtype=numpy.float64)
@@ -138,9 +143,10 @@ class Simulation:
         return res
 
     def update_fluid_properties(self):
+        """Returns the state arrays whose values may have changed."""
         if self.eos_mode == "loop" or self._eos_vectorized is False:
             self._eos_loop()
-            return
+            return self._fp_arrays()
         try:
             res = self._eos_vector()
             ok = self._eos_vectorized or self._validate_eos(res)
@@ -149,10 +155,14 @@ class Simulation:
         if not ok:
             self._eos_vectorized = False
             self._eos_loop()
-            return
+            return self._fp_arrays()
         self._eos_vectorized = True
-        for a, v in zip(self._fp_arrays(), res):
-            a[...] = v
+        changed = []
+        for a, v, same in zip(self._fp_arrays(), res, self._eos_same):
+            if not same:       # the callback returned its argument unchanged
+                a[...] = v
+                changed.append(a)
+        return changed
 
     def _validate_eos(self, res) -> bool:
         """Check the whole-field call against the per-cell contract on a sample
@@ -235,10 +245,11 @@ class Simulation:
                 self.dt = self.dt_prev
                 self.PLOTINDEX += 1
 
-    def _host_mutated(self):
+    def _host_mutated(self, arrays=None):
+        """Tell the backend that host code changed `arrays` (None: any)."""
         inv = getattr(self.solvers, "invalidate", None)
         if inv is not None:
-            inv()
+            inv(arrays)
 
     def _print(self, *a, **k):
         if self.verbose:
@@ -443,11 +454,11 @@ class Simulation:
                                  if st.BC[i, j, k] != 1]
                     if chem_list != [] and chem_inc == 0:
                         self._chemistry(chem_list, dt_chem)
-                        self._host_mutated()
+                        self._host_mutated([st.C, st.P, st.T, st.rho_f, st.phi,
+                                            st.Kx, st.Ky, st.Kz])
 
                 if c.FLUIDPROP:
-                    self.update_fluid_properties()
-                    self._host_mutated()
+                    self._host_mutated(self.update_fluid_properties())
 
                 if c.SIA:
                     SIA_diff = sv.l2_norm(st.rho_f, rhof_prev, 'rhof')
@@ -463,7 +474,8 @@ class Simulation:
                         if c.SPECTRANS:
                             st.C = Cn.copy()
                         SIA_iter += 1
-                        self._host_mutated()
+                        # new arrays: the backend's cache keys no longer match
+                        self._host_mutated([st.P, st.T, st.C])
                 else:
                     SIA_filled = True
                     if n_tsteps % c.PRINTOUT == 0:
@@ -480,9 +492,10 @@ class Simulation:
                         st.BT, st.BC, st.WP, st.WT, st.WC, st.Kx, st.Ky, st.Kz,
                         st.rho_f, st.nue, st.c_f, st.cp_f, st.qx, st.phi,
                         st.l_f, st.l_r)
-                self._host_mutated()
+                self._host_mutated([st.P, st.T, st.C, st.BP, st.BT, st.BC, st.WP,
+                                    st.WT, st.WC, st.Kx, st.Ky, st.Kz])
                 if bc_inc_ != bc_inc and c.FLUIDPROP:
-                    self.update_fluid_properties()
+                    self._host_mutated(self.update_fluid_properties())
 
             if c.MASSBALAN:
                 st.mass = c.mass_balance(st.C, st.qx, st.qy, st.qz, st.WP,

@@ -70,9 +70,17 @@ class HipSolvers:
         self.stats = {"uploads": 0, "downloads": 0}
 
     # residency -------------------------------------------------------------------
-    def invalidate(self):
-        """Signal that host code may have changed any input array."""
-        self._gen += 1
+    def invalidate(self, arrays=None):
+        """Signal that host code may have changed input arrays: all of them
+        (arrays None), or those sharing memory with any of `arrays`."""
+        if arrays is None:
+            self._gen += 1
+            return
+        arrays = [np.asarray(a) for a in arrays]
+        for ent in self._cache.values():
+            ref = ent[3]
+            if ref is not None and any(np.may_share_memory(ref, a) for a in arrays):
+                ent[1] = -1
 
     def _dev(self, role: str, arr):
         a = np.asarray(arr)
