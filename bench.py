@@ -71,10 +71,11 @@ def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=4)
     ap.add_argument("--grid", type=int, nargs=3, default=[512, 512, 256])
     ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--omega", type=float, default=1.996)
+    ap.add_argument("--omega", type=float, default=None,
+                    help="SOR factor (default: tuned online over the warm-up solves)")
     ap.add_argument("--no-compare", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=8)
@@ -130,7 +131,7 @@ def main(argv=None):
     if args.profile_kernel:
         fp.solve("mixed", max_sweeps=0, plan=plan32)
         n = args.profile_kernel * plan32[0]
-        fp.sweeps(n, plan32, sor=args.omega, fp32=True)
+        fp.sweeps(n, plan32, sor=args.omega or 1.996, fp32=True)
         src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)      # calibration: 1 GiB read + 1 GiB write
@@ -139,21 +140,33 @@ def main(argv=None):
                           "plan": plan32, "copy_bytes": 2 * src.numel() * 8}))
         return 0
 
-    def step(form):
+    from transportse_amd.omega import OmegaTuner
+    tuner = OmegaTuner(max(nx, ny, nz))
+    explicit_omega = args.omega is not None
+
+    def step(form, omega):
         fp.assemble()
-        st = fp.solve(form, tol=args.tol, sor=args.omega)
+        st = fp.solve(form, tol=args.tol, sor=omega)
         fp.result_device()  # solution back into a split-layout device field
         return st
 
     for _ in range(args.warmup):
-        st = step("mixed")
-        log(f"warmup solve: {st.sweeps} sweeps, rel {st.rel_residual:.2e}, "
-            f"{st.seconds:.2f}s")
+        om = args.omega if args.omega else tuner.omega
+        st = step("mixed", om)
+        if not args.omega:
+            tuner.report(st.sweeps)
+        log(f"warmup solve (omega {om:.5f}): {st.sweeps} sweeps, rel "
+            f"{st.rel_residual:.2e}, {st.seconds:.2f}s")
+    if not args.omega:      # the best factor the warm-up solves found
+        tuner.u = tuner.best_u if tuner.best_u is not None else tuner.u
+    omega = args.omega if args.omega else tuner.omega
+    args.omega = omega
+    log(f"SOR factor {omega:.5f}")
     torch.cuda.synchronize()
     stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        stats.append(step("mixed"))
+        stats.append(step("mixed", omega))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     sweeps = sum(s.sweeps for s in stats)
@@ -183,7 +196,7 @@ def main(argv=None):
 
     plain = None
     if not args.no_compare:
-        st64 = step("plain64")
+        st64 = step("plain64", omega)
         P_plain = fp.result()
         change = float(np.max(np.abs(P_plain - prob.P)))
         dev = float(np.max(np.abs(P_mixed - P_plain)))
@@ -253,7 +266,9 @@ def main(argv=None):
         "config": {"workload": f"pressure solve {nx}x{ny}x{nz}, 5 m cells, "
                                "sigma_lnK 2.5, corr 16 cells, mixed precision "
                                f"to rel. residual {args.tol:g}",
-                   "grid": [nx, ny, nz], "form": "mixed", "omega": args.omega,
+                   "grid": [nx, ny, nz], "form": "mixed", "omega": omega,
+                   "omega_source": "fixed" if explicit_omega else
+                                   "tuned online over the warm-up solves",
                    "plan_ns_by_ry_zchunk_variant": list(plan32)},
         "time_to_solution_s": tts,
         "solves_per_s": 1.0 / tts,

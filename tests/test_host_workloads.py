@@ -1,0 +1,55 @@
+"""Host logic of the ensemble and serving workloads (CPU only)."""
+import numpy as np
+
+from transportse_amd.ensemble import ensemble_specs
+from transportse_amd.scenario import ScenarioSpec, field_bytes, host_fields
+from transportse_amd.serve import GRIDS, admissible, make_requests
+
+
+def test_ensemble_specs_ranges_and_determinism():
+    a = ensemble_specs(64)
+    b = ensemble_specs(64)
+    assert len(a) == 64 and a == b
+    assert all(32 <= s.nx <= 160 and 32 <= s.ny <= 160 and 32 <= s.nz <= 96
+               for s in a)
+    assert all(0.5 <= s.sigma <= 2.0 and 0.0 <= s.beta <= 10.0 for s in a)
+    assert all(0.2 <= s.c_top <= 1.0 and s.heat and s.species == 1 for s in a)
+    assert len({(s.nx, s.ny, s.nz) for s in a}) > 50      # uneven grids
+    assert ensemble_specs(64, seed=5) != a
+
+
+def test_requests_poisson_and_grids():
+    r = make_requests(400, rate=20.0)
+    t = np.array([q.arrival for q in r])
+    assert np.all(np.diff(t) > 0)
+    gaps = np.diff(np.concatenate([[0.0], t]))
+    assert abs(gaps.mean() - 1 / 20.0) < 0.15 / 20.0     # exponential, mean 1/rate
+    assert abs(gaps.std() / gaps.mean() - 1.0) < 0.15
+    shapes = [(q.spec.nx, q.spec.ny, q.spec.nz) for q in r]
+    assert set(shapes) == set(GRIDS)
+    assert all(0.5 <= q.spec.sigma <= 2.5 and q.spec.species == 3 for q in r)
+    again = make_requests(400, rate=20.0)
+    assert [q.arrival for q in again] == list(t) and \
+        [q.spec for q in again] == [q.spec for q in r]
+
+
+def test_admission_rule():
+    assert admissible(10, 0, 100, 0)
+    assert admissible(60, 40, 100, 3)
+    assert not admissible(61, 40, 100, 3)
+    assert admissible(500, 0, 100, 0)            # alone, even if too large
+    assert not admissible(500, 0, 100, 1)
+
+
+def test_footprint_and_initial_fields():
+    s = ScenarioSpec(20, 16, 12, species=2, heat=True, salt_patch=3, seed=2)
+    assert field_bytes(s) > s.cells * (16 + 6) * 8
+    f = host_fields(s)
+    assert f["K"].shape == (20, 16, 12) and np.all(f["K"] > 0)
+    assert f["C"][0][:, :, -1].max() == 1.0 and f["C"][0][:, :, 0].max() == 0.0
+    assert np.all(f["BP"][0] == 1) and np.all(f["BP"][-1] == 1)
+    assert np.all(f["T"][:, :, 0] == s.t_bottom) and np.all(f["BT"][:, :, 0] == 1)
+    np.testing.assert_array_equal(f["rho_f"], 1000.0 + 700.0 * f["C"][0])
+    top = ScenarioSpec(20, 16, 12, salt_patch=0, c_top=0.6)
+    g = host_fields(top)
+    assert np.all(g["C"][0][:, :, -1] == 0.6) and np.all(g["BC"][:, :, -1] == 1)

@@ -77,6 +77,13 @@ def make_requests(n: int, rate: float, steps: int = 10, species: int = 3,
     return out
 
 
+def admissible(need: int, used: int, budget: float, n_live: int) -> bool:
+    """Admission rule: a request is admitted when its footprint fits the
+    remaining budget, or when nothing is running (a request larger than the
+    whole budget still runs, alone)."""
+    return used + need <= budget or n_live == 0
+
+
 @dataclass
 class ServeStats:
     rate: float
@@ -132,7 +139,7 @@ class Server:
                 while waiting:
                     r = waiting[0]
                     need = field_bytes(r.spec)
-                    if used + need > self.budget and live:
+                    if not admissible(need, used, self.budget, len(live)):
                         break
                     waiting.popleft()
                     used += need
