@@ -56,3 +56,11 @@ def test_no_cpu_fallback_without_gpu():
     import numpy as np
     with pytest.raises(RuntimeError, match="no CPU fallback"):
         HipContext(2, 2, 2, np.ones(2), np.ones(2), np.ones(2), 9.8, 0.0)
+
+
+def test_c_host_example_compiles_as_c99(tmp_path):
+    """include/tse_hip.h is plain C: a C99 host program using it compiles
+    with -Werror and links against the library."""
+    from transportse_amd.build import build_c_example
+    out = build_c_example(str(tmp_path / "c_abi_example"))
+    assert os.path.exists(out)

@@ -67,9 +67,29 @@ def build(verbose: bool = False) -> str:
     if proc.returncode != 0:
         raise RuntimeError(f"link failed:\n{proc.stderr[-6000:]}")
     os.replace(tmp, LIB)
+    build_c_example()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+EXAMPLE_SRC = os.path.join(os.path.dirname(HERE), "tests", "c_abi_example.c")
+EXAMPLE_BIN = os.path.join(HERE, "c_abi_example")
+
+
+def build_c_example(out: str = EXAMPLE_BIN) -> str:
+    """Compile the plain-C host of the C-ABI (tests/c_abi_example.c) with gcc
+    as C99 against include/tse_hip.h and libtse_hip.so."""
+    rocm = os.environ.get("ROCM_PATH", "/opt/rocm")
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror",
+           "-D__HIP_PLATFORM_AMD__", "-I", INCLUDE, "-I", os.path.join(rocm, "include"),
+           EXAMPLE_SRC, "-L", HERE, "-ltse_hip", "-L", os.path.join(rocm, "lib"),
+           "-lamdhip64", "-Wl,-rpath,$ORIGIN", f"-Wl,-rpath,{os.path.join(rocm, 'lib')}",
+           "-lm", "-o", out]
+    proc = subprocess.run(cmd, capture_output=True, text=True)
+    if proc.returncode != 0:
+        raise RuntimeError(f"C example failed to build:\n{proc.stderr[-4000:]}")
+    return out
 
 
 if __name__ == "__main__":
