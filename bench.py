@@ -21,9 +21,10 @@ Reported (one JSON line on stdout):
 
 Further sections of the same line (each bounded; --no-configs / --no-sia /
 --no-ensemble / --no-serve skip them):
-  configs_faithful64  configs A (64x32x32, 100 steps) and B (256x256x128,
-               2 steps) through the SIA driver with the faithful FP64
-               solvers; A also with the oracle on the host cores (10 steps)
+  configs_faithful64  the salt 64x32x32 config (100 steps) and the salt +
+               heat SIA 256x256x128 config (2 steps) through the SIA driver
+               with the faithful FP64 solvers; the 64x32x32 one also with the
+               oracle on the host cores (10 steps)
   sia          device-resident SIA steps (heat + salt + tracer) on the
                512x512x256 problem in the mixed form and in plain FP64 solved
                to 1e-13, and the relative L2 / max error of P, T and each C
@@ -298,16 +299,18 @@ def main(argv=None):
 
 
 def configs_section(args):
-    """Configs A (64x32x32) and B (256x256x128) through the SIA driver with the
-    faithful FP64 solvers (bit-identical to the reference arithmetic); A also
-    with the oracle on the host cores for a bounded number of steps."""
+    """The salt 64x32x32 and salt + heat SIA 256x256x128 configs through the
+    SIA driver with the faithful FP64 solvers (bit-identical to the reference
+    arithmetic); the first also with the oracle on the host cores for a
+    bounded number of steps."""
     import torch
 
     from transportse_amd.driver import Simulation, make_hip_backend
     cfg_dir = os.path.join(ROOT, "transportse_amd", "configs")
     out = {}
-    for name, cfg, steps in (("A", "baseline_a.py", args.a_steps),
-                             ("B", "baseline_b.py", args.b_steps)):
+    small, large = "salt_64x32x32", "salt_heat_sia_256x256x128"
+    for name, steps in ((small, args.a_steps), (large, args.b_steps)):
+        cfg = name + ".py"
         sim = Simulation(os.path.join(cfg_dir, cfg), make_hip_backend,
                          verbose=False, max_steps=steps)
         n = sim.st.nx * sim.st.ny * sim.st.nz
@@ -322,7 +325,7 @@ def configs_section(args):
                "p_sweeps": sm["numitP"], "c_sweeps": sm["numitC"],
                "t_sweeps": sm["numitT"], "solver_seconds": sim.solve_time,
                "p_cell_updates_per_s_end_to_end": sm["numitP"] * n / sec}
-        log(f"config {name} faithful FP64: {sm['n_tsteps']} steps in {sec:.2f}s, "
+        log(f"{name} faithful FP64: {sm['n_tsteps']} steps in {sec:.2f}s, "
             f"P sweeps {sm['numitP']}")
         out[name] = rec
         del sim
@@ -340,17 +343,17 @@ def configs_section(args):
                               maxit_c=sim.MAXIT_C, maxit_t=sim.MAXIT_T,
                               dt_max=c.dt_max, c_imblflo=c.C_IMBLFLO,
                               t_imblflo=c.T_IMBLFLO)
-        sim = Simulation(os.path.join(cfg_dir, "baseline_a.py"), oracle_backend,
+        sim = Simulation(os.path.join(cfg_dir, small + ".py"), oracle_backend,
                          verbose=False, max_steps=args.a_cpu_steps)
         t0 = time.perf_counter()
         sim.run()
         sec = time.perf_counter() - t0
-        out["A"]["cpu_oracle"] = {"steps": sim.summary["n_tsteps"], "seconds": sec,
+        out[small]["cpu_oracle"] = {"steps": sim.summary["n_tsteps"], "seconds": sec,
                                   "steps_per_s": sim.summary["n_tsteps"] / sec,
                                   "cores": threads,
-                                  "gpu_speedup": out["A"]["steps_per_s"]
+                                  "gpu_speedup": out[small]["steps_per_s"]
                                   / (sim.summary["n_tsteps"] / sec)}
-        log(f"config A oracle on {threads} host threads: "
+        log(f"{small} oracle on {threads} host threads: "
             f"{sim.summary['n_tsteps']} steps in {sec:.2f}s")
     return out
 

@@ -1,10 +1,12 @@
 """GPU: the benchmark configs through the SIA driver, against the oracle.
 
-Workload A at its full size (64x32x32) and workload B's physics at 64x64x32
+The salt config at its full size (64x32x32) and the salt + heat SIA
+physics at 64x64x32
 (stretched z, slope, salt + heat, dispersion, SIA) run through the same
 driver twice -- GPU backend and oracle backend -- and every field of every
 step must be bit-identical, with equal iteration counts and times.
-Workload B at full size (256x256x128) runs on the GPU alone: each step
+The salt + heat SIA config at full size (256x256x128) runs on the GPU
+alone: each step
 must converge, and the mass-balance output must be consistent.
 """
 import os
@@ -26,8 +28,8 @@ def _run(cfg, backend, steps, tmp):
     return sim.run()
 
 
-@pytest.mark.parametrize("cfg,steps", [("baseline_a.py", 25),
-                                       ("baseline_b_small.py", 2)])
+@pytest.mark.parametrize("cfg,steps", [("salt_64x32x32.py", 25),
+                                       ("salt_heat_sia_64x64x32.py", 2)])
 def test_config_matches_oracle(cfg, steps, tmp_path):
     from transportse_amd.driver import make_hip_backend
     th = _run(cfg, make_hip_backend, steps, tmp_path)
@@ -44,7 +46,7 @@ def test_config_matches_oracle(cfg, steps, tmp_path):
 
 def test_config_b_full_size(tmp_path):
     from transportse_amd.driver import make_hip_backend
-    th = _run("baseline_b.py", make_hip_backend, 2, tmp_path)
+    th = _run("salt_heat_sia_256x256x128.py", make_hip_backend, 2, tmp_path)
     assert len(th) == 2
     m0 = None
     for r in th:
@@ -63,5 +65,5 @@ def test_config_b_full_size(tmp_path):
             m0 = m[0]
         # salt mass changes only by what leaves through the open faces
         assert abs(m[0] - m0) <= 0.05 * m0
-    print("baseline_b full size: P iterations", th[-1]["numitP"], "C",
+    print("salt_heat_sia_256x256x128: P iterations", th[-1]["numitP"], "C",
           th[-1]["numitC"], "T", th[-1]["numitT"], flush=True)

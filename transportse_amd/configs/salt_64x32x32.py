@@ -1,4 +1,4 @@
-"""Workload A: 64x32x32 uniform 10 m cells; isothermal variable-density
+"""Salt transport on 64x32x32 uniform 10 m cells: isothermal variable-density
 Darcy flow plus one salt species; lognormal K (geometric mean 1e-12 m^2,
 sigma_lnK 1.0, exponential correlation 5 cells, seed 1); porosity 0.2;
 rho = 1000 + 700 C; C = 1 in a 4-cell-thick top patch; hydrostatic

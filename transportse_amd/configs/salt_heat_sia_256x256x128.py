@@ -1,5 +1,5 @@
-"""Workload B: 256x256x128, dx = dy = 10 m, dz = 2 * 1.05^(nz-1-k) m
-(2 m at the top), slope 5 deg; salt as in A (sigma_lnK 1.0, correlation 5,
+"""Salt + heat under SIA on 256x256x128: dx = dy = 10 m, dz = 2 * 1.05^(nz-1-k) m
+(2 m at the top), slope 5 deg; salt as in salt_64x32x32 (sigma_lnK 1.0, correlation 5,
 seed 2) plus heat: T0 = 10 C, 60 C Dirichlet bottom face, fluid conductivity
 2.5 W/m/K scaled linearly with T; dispersivity 1.0 / 0.1 m; OMEGA 1; SIA
 Picard iteration; 20 time steps of 1 day."""

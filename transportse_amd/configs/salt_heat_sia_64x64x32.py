@@ -1,4 +1,4 @@
-"""Workload B at 64x64x32 (same physics and settings as baseline_b, sized
+"""salt_heat_sia_256x256x128 at 64x64x32 (same physics and settings, sized
 for the CPU oracle): stretched dz, slope 5 deg, salt + heat, dispersion,
 SIA, 20 time steps."""
 import numpy
