@@ -1,13 +1,11 @@
 """GPU: the benchmark configs through the SIA driver, against the oracle.
 
-The salt config at its full size (64x32x32) and the salt + heat SIA
-physics at 64x64x32
-(stretched z, slope, salt + heat, dispersion, SIA) run through the same
-driver twice -- GPU backend and oracle backend -- and every field of every
-step must be bit-identical, with equal iteration counts and times.
-The salt + heat SIA config at full size (256x256x128) runs on the GPU
-alone: each step
-must converge, and the mass-balance output must be consistent.
+The salt config at its full size (64x32x32) and the salt + heat SIA physics
+at 64x64x32 (stretched z, slope, salt + heat, dispersion, SIA) run through
+the same driver twice -- GPU backend and oracle backend -- and every field of
+every step must be bit-identical, with equal iteration counts and times.
+The salt + heat SIA config at full size (256x256x128) runs on the GPU alone:
+each step must converge, and the mass-balance output must be consistent.
 """
 import os
 
