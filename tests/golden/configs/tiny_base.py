@@ -26,7 +26,7 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
           dt0=86400.0, dt_max=86400.0, beta=5.0, stretch=1.05,
           maxit_p=0, maxit_c=0, maxit_t=0, disp=(1.0, 0.1), wp_cell=True,
           plot_steps=0, c_f=4.5e-10, salt_patch=True, t_end=None,
-          smolar_c=False, smolar_t=False, nlte=False):
+          smolar_c=False, smolar_t=False, nlte=False, bp_faces="x"):
     rng = numpy.random.default_rng(seed)
     S = species
     ns.update(dict(
@@ -89,8 +89,12 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
     ns['BP'] = numpy.zeros(shp)
     ns['BC'] = numpy.zeros(shp)
     ns['BT'] = numpy.zeros(shp)
-    ns['BP'][0, :, :] = 1
-    ns['BP'][-1, :, :] = 1
+    if bp_faces == "x":
+        ns['BP'][0, :, :] = 1
+        ns['BP'][-1, :, :] = 1
+    else:  # columns: Dirichlet top and bottom
+        ns['BP'][:, :, 0] = 1
+        ns['BP'][:, :, -1] = 1
     if heat:
         ns['BT'][:, :, 0] = 1
         ns['T'][:, :, 0] = 60.0
@@ -100,10 +104,11 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
     ns['WT'] = numpy.zeros(shp)
     ns['WC'] = numpy.zeros(shp + (S,))
     if wp_cell:
-        ns['WP'][nx // 2, ny // 2, 1] = 2e-6
-        ns['WC'][nx // 2, ny // 2, 1, 0] = 1e-7
+        kw = min(1, nz - 1)
+        ns['WP'][nx // 2, ny // 2, kw] = 2e-6
+        ns['WC'][nx // 2, ny // 2, kw, 0] = 1e-7
         if heat:
-            ns['WT'][nx // 2, ny // 2, 1] = 5.0
+            ns['WT'][nx // 2, ny // 2, kw] = 5.0
     ns['qx'] = numpy.zeros(shp)
     ns['qy'] = numpy.zeros(shp)
     ns['qz'] = numpy.zeros(shp)
@@ -136,7 +141,7 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
     def mass_balance(C, qx, qy, qz, WP, WC, WT, D_fx, D_fy, D_fz, dt, mass):
         mass = mass.copy()
         mass[0] = numpy.sum(C[:, :, :, 0] * ns['phi'] * vol)
-        mass[1] += dt * numpy.sum(numpy.abs(qx[-2, :, :]) * ns['dy'][:, None]
+        mass[1] += dt * numpy.sum(numpy.abs(qx[max(nx - 2, 0), :, :]) * ns['dy'][:, None]
                                   * ns['dz'][None, :])
         mass[2] += dt * numpy.sum(WC[:, :, :, 0] * vol)
         return mass

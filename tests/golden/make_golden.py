@@ -31,7 +31,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 REPO = os.path.dirname(os.path.dirname(HERE))
 REFERENCE = "/root/reference/tse.py"
 STUB = os.path.join(REPO, "transportse_amd", "numba_stub")
-CASES = ["g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte"]
+CASES = ["g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte",
+         "g6_xz", "g7_column", "g8_xy"]
 
 SITECUSTOMIZE = '''
 import numpy

@@ -6,7 +6,8 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
-CASES = ("g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte")
+CASES = ("g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte",
+         "g6_xz", "g7_column", "g8_xy")
 
 
 def oracle_backend(sim):

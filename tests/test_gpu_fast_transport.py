@@ -139,3 +139,37 @@ def test_explicit_omega_zero():
     err = np.max(np.abs(x - ref)) / np.max(np.abs(ref))
     print("omega=0 sweeps", st.sweeps, "err", err, flush=True)
     assert st.converged and err <= 1e-14
+
+
+@pytest.mark.parametrize("shape", [(51, 51, 1), (200, 1, 60), (1, 1, 145),
+                                   (1, 20, 30)])
+def test_unit_dimension_grids_fast_forms(shape):
+    """Pressure and transport fast forms on grids with a unit dimension reach
+    the oracle's fixed points (plain FP64 and mixed)."""
+    from oracle import Oracle
+    from transportse_amd.pressure import FastPressure
+    f = Fields(*shape, seed=45, dirichlet_p=shape[0] > 2)
+    dt = 86400.0
+    o = Oracle(*f.grid_args(), frozen_source=_frozen(f), dt_max=dt)
+    Po = f.P.copy()
+    r = o.compute_P(Po, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky, f.Kz, f.qx,
+                    f.qy, f.qz, dt, f.BP, f.WP, SOR=1.8, max_it=10 ** 6,
+                    inc_it=1.0, max_dt=dt, P_conv=1e-15)
+    assert r[4]
+    fp = FastPressure(*f.grid_args())
+    fp.load(f.P, dt, **{n: getattr(f, n) for n in ("c_f", "nue", "rho_f", "phi",
+                                                    "Kx", "Ky", "Kz", "BP", "WP",
+                                                    "c_r")})
+    change = np.max(np.abs(Po - f.P))
+    for form, tol, bound, plan in (("plain64", 1e-13, 1e-8, (2, 16, 1, 0, 1)),
+                                   ("mixed", 1e-11, 1e-5, (2, 16, 1, 0, 1))):
+        fp.assemble()
+        st = fp.solve(form, tol=tol, sor=1.8, inner=20, plan=plan,
+                      max_sweeps=200000)
+        err = np.max(np.abs(fp.result() - Po))
+        print("P", shape, form, "sweeps", st.sweeps, "err/change", err / change,
+              flush=True)
+        assert st.converged and err <= bound * change
+    ref = _oracle_c(f, dt, 1.0, True)
+    _check(_fast_c(f, dt, 1.0, True), ref, "plain64", 1e-13, 1e-10,
+           (2, 16, 1, 0), "C")

@@ -182,6 +182,48 @@ def test_smolarkiewicz(shape):
     print("smolarkiewicz ok", shape, flush=True)
 
 
+# unit dimensions, as in the reference's course examples (51x51x1 planes,
+# 200x1x60 sections, 1x1x145 columns)
+DEGENERATE = [(51, 51, 1), (1, 1, 145), (200, 1, 60), (1, 20, 30), (3, 2, 1)]
+
+
+@pytest.mark.parametrize("shape", DEGENERATE)
+def test_unit_dimension_grids(shape):
+    f = Fields(*shape, seed=11, dirichlet_p=shape[0] > 2)
+    o, h = _pair(f, dt_max=1e6)
+    Po, Ph = f.P.copy(), f.P.copy()
+    args = (f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky, f.Kz, f.qx, f.qy, f.qz,
+            3600.0, f.BP, f.WP)
+    kw = dict(SOR=1.5, max_it=20000, inc_it=1.25, max_dt=7200.0, P_conv=1e-10)
+    ro = o.compute_P(Po, *args, **kw)
+    rh = h.compute_P(Ph, *args, **kw)
+    assert (ro[0], ro[1], ro[4], ro[5]) == (rh[0], rh[1], rh[4], rh[5])
+    assert np.array_equal(ro[2], rh[2])
+    qo = [f.qx.copy(), f.qy.copy(), f.qz.copy()]
+    qh = [f.qx.copy(), f.qy.copy(), f.qz.copy()]
+    o.compute_q(Po, f.BP, f.Kx, f.Ky, f.Kz, *qo, f.nue, f.rho_f)
+    h.compute_q(Ph, f.BP, f.Kx, f.Ky, f.Kz, *qh, f.nue, f.rho_f)
+    for a, b in zip(qo, qh):
+        assert np.array_equal(a, b)
+    eo = o.flow_error(Po, ro[3], *qo, f.WP, f.phi, f.c_f, f.c_r, f.rho_f, 3600.0)
+    eh = h.flow_error(Ph, rh[3], *qh, f.WP, f.phi, f.c_f, f.c_r, f.rho_f, 3600.0)
+    assert np.array_equal(eo, eh)
+    Co, Ch = f.C[..., 0].copy(), f.C[..., 0].copy()
+    cargs = (f.D_fx[..., 0], f.D_fy[..., 0], f.D_fz[..., 0], f.phi, f.rho_f,
+             *qo, 3600.0, f.BC, f.WC[..., 0], f.R, eo)
+    ckw = dict(omega=1.0, SOR=1.0, max_it=100, C_conv=1e-10)
+    rco, rch = o.compute_C(Co, *cargs, **ckw), h.compute_C(Ch, *cargs, **ckw)
+    assert (rco[0], rco[1], rco[3], rco[4]) == (rch[0], rch[1], rch[3], rch[4])
+    assert np.array_equal(Co, Ch)
+    To, Th = f.T.copy(), f.T.copy()
+    targs = (f.l_f, f.cp_f, f.rho_f, f.phi, *qo, 3600.0, f.BT, f.WT, eo)
+    tkw = dict(omega=0.8, SOR=1.0, max_it=100, T_conv=1e-10)
+    rto, rth = o.compute_T(To, *targs, **tkw), h.compute_T(Th, *targs, **tkw)
+    assert (rto[0], rto[1], rto[3], rto[4]) == (rth[0], rth[1], rth[3], rth[4])
+    assert np.array_equal(rto[2], rth[2])
+    print("unit-dimension grid ok", shape, "P iterations", ro[5], flush=True)
+
+
 @pytest.mark.parametrize("case", CASES)
 def test_driver_matches_reference_trajectory(case, tmp_path):
     """Full SIA driver on the GPU against the reference's golden vectors."""
