@@ -235,6 +235,15 @@ def main(argv=None):
                          f"compute_P's sweep, OpenMP {threads} threads, FP64), "
                          f"{cpu_s:.1f}s"}
         log(f"cpu baseline {cpu['value']:.3e} cell-updates/s ({threads} threads)")
+        cpu["one_thread"] = cpu_sweep_rate(orc, prob, dt, args.omega, 1, 1)
+        cpu["other_grids"] = {}
+        for g, sweeps in (((64, 32, 32), 40), ((256, 256, 128), 6)):
+            small = PressureProblem(*g, spacing=10.0, sigma=1.0, corr_len=5, seed=1)
+            cpu["other_grids"]["x".join(map(str, g))] = cpu_sweep_rate(
+                orc, small, dt, 1.8, threads, sweeps)
+        orc.set_threads(threads)
+        log(f"cpu 1 thread {cpu['one_thread']['value']:.3e}; other grids "
+            + ", ".join(f"{k} {v['value']:.3e}" for k, v in cpu["other_grids"].items()))
 
     del fp
     torch.cuda.empty_cache()
@@ -296,6 +305,24 @@ def main(argv=None):
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def cpu_sweep_rate(orc, prob, dt, omega, threads, sweeps):
+    """Red+black SOR sweeps of the oracle (reference arithmetic, FP64) on a
+    pressure problem: cell-updates/s with `threads` OpenMP threads."""
+    orc.set_threads(threads)
+    o = orc.Oracle(*prob.grid_args(), frozen_source=prob)
+    P = prob.P.copy()
+    c0 = time.perf_counter()
+    for _ in range(sweeps):
+        for colour in (0, 1):
+            o.p_half_sweep(P, prob.P, prob.c_f, prob.nue, prob.rho_f, prob.phi,
+                           prob.Kx, prob.Ky, prob.Kz, prob.BP, prob.WP, dt, omega,
+                           colour)
+    sec = time.perf_counter() - c0
+    n = prob.nx * prob.ny * prob.nz
+    return {"value": n * sweeps / sec, "unit": "cell-updates/s", "cores": threads,
+            "sweeps": sweeps, "seconds": sec}
 
 
 def configs_section(args):
