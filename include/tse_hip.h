@@ -313,6 +313,28 @@ int tse_linear_eos(tse_ctx *ctx, const double *C, const double *T,
 int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
                   const double *phi, const double *qx, double *out3);
 
+/* ---- graphs of a context's device work ------------------------------------ */
+/* Captures the work that calls issue on the context's stream between
+ * tse_graph_begin and tse_graph_end (no call that synchronises may come in
+ * between) and replays it with tse_graph_launch.  Host-side effects of the
+ * captured calls happen at capture time only: after replaying a fast-form
+ * setup, call tse_fast_mark_loaded to restore what the setup sets on the
+ * host (the iterate is in the first buffer and ready), and a captured
+ * tse_fast_*_result reads the iterate buffer that was current at capture,
+ * so key such graphs by tse_fast_current.  No counterpart in the reference
+ * (its step is Python calls); these serve callers that replay a time step's
+ * device work, like the batched ensemble. */
+typedef struct tse_graph tse_graph;
+int tse_graph_begin(tse_ctx *ctx);
+int tse_graph_end(tse_ctx *ctx, tse_graph **out);
+int tse_graph_launch(tse_graph *g, tse_ctx *ctx);
+int tse_graph_destroy(tse_graph *g);
+/* Host-side state of a fast system (kind 0 pressure, 1 transport) after its
+ * setup: iterate in buffer 0, ready. */
+int tse_fast_mark_loaded(tse_ctx *ctx, int32_t kind);
+/* Which of the two iterate buffers (0 or 1) holds the current iterate. */
+int tse_fast_current(tse_ctx *ctx, int32_t kind, int32_t *cur);
+
 /* ---- measurement hooks -------------------------------------------------- */
 /* Runs n_iter full red+black iterations of the faithful compute_P sweep at
  * fixed dt (no norm, no control), for timing and sweep-level parity.

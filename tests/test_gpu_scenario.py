@@ -150,3 +150,39 @@ def test_scenario_snapshot_pickle(tmp_path):
     assert np.array_equal(tup[21], ref["rho_f"])
     assert not np.array_equal(scn.download()["P"], ref["P"])
     scn.close()
+
+
+def test_scenario_segment_graphs_bit_identical():
+    """The device work between solves (setups, result conversions, fluxes,
+    EOS), replayed from captured hipGraphs, gives the same bits as the same
+    calls issued directly, over steps whose solves end in either iterate
+    buffer."""
+    import torch
+
+    from transportse_amd.scenario import DeviceScenario
+    spec = _spec()
+    kw = dict(form="mixed", tol=1e-10, plan_p=(2, 16, 1, 0), plan_tr=(2, 16, 1, 0),
+              sor_p=1.9, sor_tr=1.0, inner_p=20, inner_tr=4)
+    out = []
+    for graphs in (False, True):
+        stream = torch.cuda.Stream()          # capture needs a non-null stream
+        with torch.cuda.stream(stream):
+            scn = DeviceScenario(spec, use_graphs=graphs)
+            scn.hip.sync_stream()
+            res = scn.run(3, **kw)
+            got = scn.download()
+            got["qx"] = scn.hip.download(scn.qx)
+            got["e"] = scn.hip.download(scn.e)
+            got["graphs"] = sorted(scn._graphs)
+            got["sweeps"] = [r.sweeps for _, r in res]
+            scn.close()
+        out.append(got)
+    eager, replay = out
+    print("captured segments:", replay["graphs"], flush=True)
+    assert not eager["graphs"]
+    assert {"setup_p", "setup_c0", "setup_t", "eos"} <= set(replay["graphs"])
+    assert replay["sweeps"] == eager["sweeps"]
+    for k in ("P", "T", "rho_f", "qx", "e"):
+        assert np.array_equal(replay[k], eager[k]), k
+    for a, b in zip(replay["C"], eager["C"]):
+        assert np.array_equal(a, b)

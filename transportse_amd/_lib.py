@@ -136,6 +136,12 @@ SIGNATURES = {
     "tse_batch_run": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_batch_status": (_I, [_V, ctypes.c_int32, ctypes.POINTER(FastResult)]),
     "tse_linear_eos": (_I, [_V, _dp, _dp, _dp, _dp, ctypes.POINTER(EosParams)]),
+    "tse_graph_begin": (_I, [_V]),
+    "tse_graph_end": (_I, [_V, ctypes.POINTER(_V)]),
+    "tse_graph_launch": (_I, [_V, _V]),
+    "tse_graph_destroy": (_I, [_V]),
+    "tse_fast_mark_loaded": (_I, [_V, ctypes.c_int32]),
+    "tse_fast_current": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_mass_sums": (_I, [_V, _dp, _dp, _dp, _dp, _DPH]),
 }
 

@@ -259,6 +259,53 @@ int tse_ctx_set_stream(tse_ctx *ctx, void *stream) {
     return 0;
 }
 
+// ---- graphs of a context's work ------------------------------------------------
+struct tse_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+int tse_graph_begin(tse_ctx *ctx) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!ctx->stream) return fail(TSE_ERR_STATE, "graph capture needs a non-null stream");
+    TSE_HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    return 0;
+}
+
+int tse_graph_end(tse_ctx *ctx, tse_graph **out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!out) return fail(TSE_ERR_ARG, "null out");
+    hipGraph_t g = nullptr;
+    TSE_HIP_TRY(hipStreamEndCapture(ctx->stream, &g));
+    tse_graph *h = new tse_graph();
+    h->graph = g;
+    if (hipGraphInstantiate(&h->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        delete h;
+        return fail(TSE_ERR_HIP, "hipGraphInstantiate failed");
+    }
+    *out = h;
+    return 0;
+}
+
+int tse_graph_launch(tse_graph *g, tse_ctx *ctx) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!g || !g->exec) return fail(TSE_ERR_ARG, "null graph");
+    TSE_HIP_TRY(hipGraphLaunch(g->exec, ctx->stream));
+    return 0;
+}
+
+int tse_graph_destroy(tse_graph *g) {
+    if (!g) return 0;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+    return 0;
+}
+
 int tse_ctx_synchronize(tse_ctx *ctx) {
     if (!ctx) return fail(TSE_ERR_ARG, "null context");
     TSE_HIP_TRY(hipSetDevice(ctx->device));

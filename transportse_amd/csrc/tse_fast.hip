@@ -708,6 +708,28 @@ int tse_fast_t_setup(tse_ctx *ctx, const tse_t_fields *f, double dt,
     return tr_setup_common(ctx, f->T, omega);
 }
 
+int tse_fast_mark_loaded(tse_ctx *ctx, int32_t kind) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (kind != SYS_P && kind != SYS_TR) return fail(TSE_ERR_ARG, "kind must be 0 or 1");
+    FastSys *s = (FastSys *)(kind == SYS_P ? ctx->fast : ctx->fast_tr);
+    if (!s || !s->buf64) return fail(TSE_ERR_STATE, "fast system never set up");
+    s->cur = 0;
+    s->ready = true;
+    return 0;
+}
+
+int tse_fast_current(tse_ctx *ctx, int32_t kind, int32_t *cur) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (kind != SYS_P && kind != SYS_TR) return fail(TSE_ERR_ARG, "kind must be 0 or 1");
+    if (!cur) return fail(TSE_ERR_ARG, "null pointer");
+    FastSys *s = (FastSys *)(kind == SYS_P ? ctx->fast : ctx->fast_tr);
+    if (!s || !s->ready) return fail(TSE_ERR_STATE, "fast form not set up");
+    *cur = s->cur;
+    return 0;
+}
+
 int tse_fast_tr_solve(tse_ctx *ctx, const tse_fast_params *prm,
                       tse_fast_result *out) {
     int rc = check_ctx(ctx);

@@ -9,7 +9,10 @@ advance stage by stage; ``"async"`` runs one cycle of both batches per tick
 and moves every realization whose solve finished on to its next solve.  A realization keeps its slot in both
 batches for the whole run and is re-armed for each new solve, so each
 batch's cycle is captured as a hipGraph once the member set is complete and
-replayed for every later cycle.
+replayed for every later cycle, and each realization's device work between
+solves (setups, fluxes, conversions, EOS) is replayed from its own captured
+graphs -- the whole SIA step apart from the density-norm and mass-sum
+readbacks that decide the next move.
 
 Launch plans are chosen by ``tune_plans``: each candidate plan is timed over
 a few batched cycles of the actual ensemble, and the fastest per sweep
@@ -82,7 +85,7 @@ class Ensemble:
         self.inner_p, self.inner_tr = inner_p, inner_tr
         self.max_sweeps = max_sweeps
         with torch.cuda.stream(self.stream):
-            self.scn = [DeviceScenario(s, device) for s in specs]
+            self.scn = [DeviceScenario(s, device, use_graphs=graph) for s in specs]
         self.plan_p, self.plan_tr = tuple(plan_p), tuple(plan_tr)
         self.bp = self.btr = None
 

@@ -139,9 +139,16 @@ class StepStats:
 class DeviceScenario:
     """One scenario's fields in HBM and its SIA step as a generator."""
 
-    def __init__(self, spec: ScenarioSpec, device: int = 0, fields=None):
+    def __init__(self, spec: ScenarioSpec, device: int = 0, fields=None,
+                 use_graphs: bool = False):
         import torch
         self.spec = spec
+        # device work between solves as hipGraphs: each segment runs eagerly
+        # once (allocations happen there), is captured on its second use and
+        # replayed from then on (the fields never move)
+        self.use_graphs = use_graphs
+        self._seen = {}
+        self._graphs = {}
         d = np.full(spec.nx, float(spec.spacing))
         self.hip = HipContext(spec.nx, spec.ny, spec.nz, d,
                               np.full(spec.ny, float(spec.spacing)),
@@ -264,6 +271,43 @@ class DeviceScenario:
                                           self._p(self.qx), out), "tse_mass_sums")
         return tuple(out)
 
+    def _current(self, kind):
+        """The iterate buffer a fast system's result is read from: graphs of
+        the result conversions are kept per buffer."""
+        if not self.use_graphs:
+            return 0
+        c = ctypes.c_int32()
+        _lib.check(self.lib.tse_fast_current(self.hip.ctx, kind, ctypes.byref(c)),
+                   "tse_fast_current")
+        return c.value
+
+    def _segment(self, name, fn, kinds=()):
+        """Run a segment of device work (no synchronisation inside), from a
+        captured graph once it has run eagerly before."""
+        if not self.use_graphs:
+            fn()
+            return
+        n = self._seen.get(name, 0)
+        self._seen[name] = n + 1
+        if n == 0:
+            fn()
+            return
+        L, ctx = self.lib, self.hip.ctx
+        g = self._graphs.get(name)
+        if g is None:
+            self.hip.sync_stream()
+            _lib.check(L.tse_graph_begin(ctx), "tse_graph_begin")
+            try:
+                fn()
+            finally:
+                h = ctypes.c_void_p()
+                rc = L.tse_graph_end(ctx, ctypes.byref(h))
+            _lib.check(rc, "tse_graph_end")
+            g = self._graphs[name] = h
+        _lib.check(L.tse_graph_launch(g, ctx), "tse_graph_launch")
+        for k in kinds:   # what the captured setup set on the host
+            _lib.check(L.tse_fast_mark_loaded(ctx, k), "tse_fast_mark_loaded")
+
     # the SIA step ----------------------------------------------------------------
     def steps(self, nsteps: int):
         """Generator over ``nsteps`` SIA steps; yields (kind, system) for each
@@ -277,21 +321,23 @@ class DeviceScenario:
             self.Tn.copy_(self.T)
             for it in range(sp.sia_maxit):
                 st.sia_iters = it + 1
-                self.setup_p()
+                self._segment("setup_p", self.setup_p, (0,))
                 yield 0, self.fp
-                self.finish_p()
-                self.flow()
+                self._segment(f"post_p{self._current(0)}",
+                              lambda: (self.finish_p(), self.flow()))
                 for s in range(sp.species):
-                    self.setup_c(s)
+                    self._segment(f"setup_c{s}", lambda s=s: self.setup_c(s), (1,))
                     self.ft.assemble = lambda s=s: self.setup_c(s)
                     yield 1, self.ft
-                    self.finish_tr(self.C[s])
+                    self._segment(f"finish_c{s}.{self._current(1)}",
+                                  lambda s=s: self.finish_tr(self.C[s]))
                 if sp.heat:
-                    self.setup_t()
+                    self._segment("setup_t", self.setup_t, (1,))
                     self.ft.assemble = self.setup_t
                     yield 1, self.ft
-                    self.finish_tr(self.T)
-                self.eos_update()
+                    self._segment(f"finish_t{self._current(1)}",
+                                  lambda: self.finish_tr(self.T))
+                self._segment("eos", self.eos_update)
                 diff = self.density_norm()
                 if diff < sp.sia_conv or it == sp.sia_maxit - 1:
                     break
@@ -397,4 +443,7 @@ class DeviceScenario:
                 "C": [h.download(c) for c in self.C], "rho_f": h.download(self.rho)}
 
     def close(self):
+        for g in self._graphs.values():
+            self.lib.tse_graph_destroy(g)
+        self._graphs.clear()
         self.hip.close()
