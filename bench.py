@@ -62,6 +62,10 @@ BYTES_FP64_SWEEP = 56
 TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64, 0): 3.337e9,
                       (512, 512, 256, 2, 16, 1, 128, 1): 3.167e9,
                       (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9}
+# sweep kernel and its profile summary per plan variant
+VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
+                  1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),
+                  2: ("rb_zmarch_xr_kernel<float>", "rb_zmarch_xr")}
 
 
 def log(*a):
@@ -253,6 +257,7 @@ def main(argv=None):
     serving = None if args.no_serve else serve_section(args)
 
     traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32) + (0,) * (5 - len(plan32)))
+    variant = plan32[4] if len(plan32) > 4 else 0
     srv0 = serving["runs"][0] if serving and serving.get("runs") else None
     line = {
         "metric": "SOR cell-updates/s & HBM roofline %; solves/s + p99 at rel. "
@@ -290,11 +295,10 @@ def main(argv=None):
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
                      "traffic_source": "rocprofv3 --pmc of the same plan, profiles/"
-                                       + ("rb_zmarch_xl" if plan32[-1] == 1 else "rb_zmarch")
+                                       + VARIANT_KERNEL[variant][1]
                                        + "_fp32_pressure_512x512x256.md"
                                        if traffic else None,
-                     "kernel": ("rb_zmarch_xl_kernel<float>" if len(plan32) > 4
-                                and plan32[4] == 1 else "rb_zmarch_kernel<float>"),
+                     "kernel": VARIANT_KERNEL[variant][0],
                      "bytes_per_update": BYTES_FP32_SWEEP,
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6},
         "cpu_baseline": cpu,

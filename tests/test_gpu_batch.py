@@ -63,6 +63,8 @@ def _systems(kind):
 SOR = {0: 1.9, 1: 1.2}
 CASES = [(0, "mixed", (2, 16, 1, 0), 1e-10),
          (0, "mixed", (2, 24, 2, 6, 1), 1e-10),
+         (0, "mixed", (2, 16, 1, 0, 2), 1e-10),
+         (0, "plain64", (2, 16, 1, 9, 2), 1e-11),
          (0, "plain64", (1, 16, 1, 7), 1e-11),
          (1, "mixed", (3, 16, 1, 8), 1e-11),
          (1, "plain64", (2, 16, 1, 0), 1e-12)]

@@ -20,13 +20,17 @@ from helpers import Fields
 pytestmark = pytest.mark.gpu
 
 # (ns, by, ry, zchunk, variant): every kernel instance, with and without
-# z-chunks; variant 1 exchanges x neighbours across lanes instead of LDS
+# z-chunks; variant 1 exchanges x neighbours across lanes instead of LDS,
+# variant 2 is variant 1 with unrolled register rings and buffer loads
 PLANS = [(1, 8, 1, 0, 0), (1, 16, 1, 5, 0), (2, 16, 1, 0, 0), (2, 16, 1, 6, 0),
          (1, 24, 2, 0, 0), (1, 24, 2, 7, 0),
-         (1, 16, 1, 0, 1), (2, 16, 1, 5, 1), (1, 24, 2, 6, 1)]
+         (1, 16, 1, 0, 1), (2, 16, 1, 5, 1), (1, 24, 2, 6, 1),
+         (1, 16, 1, 0, 2), (2, 16, 1, 5, 2), (2, 16, 1, 0, 2), (1, 24, 2, 6, 2)]
 PLANS32 = [(2, 32, 2, 6, 0), (2, 32, 4, 0, 0), (1, 32, 2, 5, 0),
            (3, 16, 1, 9, 0), (2, 24, 2, 6, 1), (2, 32, 2, 0, 1),
-           (3, 16, 1, 7, 1), (3, 24, 2, 9, 1)]
+           (3, 16, 1, 7, 1), (3, 24, 2, 9, 1),
+           (2, 24, 2, 6, 2), (2, 32, 2, 0, 2), (3, 16, 1, 7, 2), (3, 24, 2, 9, 2),
+           (3, 32, 2, 0, 2), (2, 32, 2, 5, 2)]
 
 def _setup(f, dt):
     from transportse_amd import _lib
@@ -147,3 +151,34 @@ def test_lane_exchange_variant_equals_lds_variant(ns, by, ry):
     print("variant 1 vs 0", (ns, by, ry), "identical", np.array_equal(*out),
           "diff/change", diff / change, flush=True)
     assert diff <= 1e-13 * change
+
+
+@pytest.mark.parametrize("ns,by,ry,zchunk", [(1, 16, 1, 0), (2, 16, 1, 9),
+                                             (2, 16, 1, 0), (1, 24, 2, 7)])
+def test_unrolled_ring_variant_equals_lane_exchange_variant(ns, by, ry, zchunk):
+    """Variant 2 reads the same operands in the same order as variant 1
+    (rings rotated by unrolling, buffer loads that read 0 outside the grid):
+    FP64 sweeps and whole mixed solves are bit-identical, including on a grid
+    whose x and y extents leave partial tiles."""
+    from transportse_amd import _lib
+    f = Fields(83, 41, 23, seed=25)
+    sw, sol = [], []
+    for var in (1, 2):
+        hip, d = _setup(f, 3600.0)
+        plan = _lib.SweepPlan(ns=ns, by=by, ry=ry, zchunk=zchunk, variant=var)
+        _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), 6 * ns,
+                                             1.8, 0), "sweeps")
+        sw.append(_result(hip, d))
+        hip, d = _setup(f, 3600.0)
+        prm = _lib.FastParams(form=2, plan=plan, inner=6 * ns, max_sweeps=12000,
+                              sor=1.9, tol=1e-10)
+        res = _lib.FastResult()
+        _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                            ctypes.byref(res)), "solve")
+        assert res.converged
+        sol.append((_result(hip, d), res.sweeps))
+    print("variant 2 vs 1", (ns, by, ry, zchunk), "sweeps identical",
+          np.array_equal(*sw), "solve identical", np.array_equal(sol[0][0], sol[1][0]),
+          "sweeps", sol[0][1], sol[1][1], flush=True)
+    assert np.array_equal(sw[0], sw[1])
+    assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])

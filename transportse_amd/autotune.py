@@ -16,11 +16,16 @@ import ctypes
 from . import _lib
 
 # (ns, by, ry, variant): variant 0 exchanges x neighbours through LDS,
-# variant 1 across the lanes of a wavefront (two LDS plane rings, not three)
+# variant 1 across the lanes of a wavefront (two LDS plane rings, not three),
+# variant 2 as 1 with register rings rotated by unrolling and buffer loads
+# (fields up to 1 GiB)
 PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0), (1, 24, 2, 0),
-           (1, 16, 1, 1), (2, 16, 1, 1), (1, 24, 2, 1)]
+           (1, 16, 1, 1), (2, 16, 1, 1), (1, 24, 2, 1),
+           (1, 16, 1, 2), (2, 16, 1, 2), (1, 24, 2, 2)]
 PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
-                     (2, 24, 2, 1), (2, 32, 2, 1), (3, 16, 1, 1), (3, 24, 2, 1)]
+                     (2, 24, 2, 1), (2, 32, 2, 1), (3, 16, 1, 1), (3, 24, 2, 1),
+                     (2, 24, 2, 2), (2, 32, 2, 2), (3, 16, 1, 2), (3, 24, 2, 2),
+                     (3, 32, 2, 2)]
 # transport: the cell's own coefficients stay in registers, one row per thread
 TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0)]
 TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0)]

@@ -106,8 +106,11 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
         if constexpr (VAR == 0)
             rb_zmarch_tile<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
                                            rhs, omega, zero);
-        else
+        else if constexpr (VAR == 1)
             rb_zmarch_tile_xl<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
+                                              rhs, omega, zero);
+        else
+            rb_zmarch_tile_xr<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
                                               rhs, omega, zero);
         __syncthreads();  // LDS reuse by the next tile
     }
@@ -383,6 +386,9 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
     TSE_BP(1, 1, 16, 1)
     TSE_BP(1, 2, 16, 1)
     TSE_BP(1, 1, 12, 2)
+    TSE_BP(2, 1, 16, 1)
+    TSE_BP(2, 2, 16, 1)
+    TSE_BP(2, 1, 12, 2)
     TSE_BT(1, 8)
     TSE_BT(1, 16)
     TSE_BT(2, 16)
@@ -395,6 +401,11 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_BP(1, 2, 16, 2)
         TSE_BP(1, 3, 16, 1)
         TSE_BP(1, 3, 12, 2)
+        TSE_BP(2, 2, 12, 2)
+        TSE_BP(2, 2, 16, 2)
+        TSE_BP(2, 3, 16, 1)
+        TSE_BP(2, 3, 12, 2)
+        TSE_BP(2, 3, 16, 2)
         TSE_BT(3, 16)
     }
 #undef TSE_BP
@@ -597,6 +608,8 @@ int tse_batch_add(tse_batch *b, tse_ctx *c) {
     if (c->device != b->device) return fail(TSE_ERR_ARG, "member on another device");
     FastSys *s = member_sys(b, c);
     if (!s || !s->ready) return fail(TSE_ERR_STATE, "member's fast system not set up");
+    if (!fast_plan_fits(b->plan, s->N, b->form == TSE_FORM_MIXED))
+        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
     for (int q = 0; q < b->cap; ++q)
         if (b->hused[q] && b->ctx[q] == c)
             return fail(TSE_ERR_ARG, "context already in the batch");

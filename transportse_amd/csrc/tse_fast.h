@@ -174,13 +174,13 @@ rb_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
 // ---- cross-lane x neighbours -------------------------------------------------------------------
 // A tile row is one 64-lane wavefront (lane = x offset), so the cells left
 // and right of lane i sit in lanes i-1 and i+1: DPP wave shift right / left
-// by one lane moves a register between them (lanes 0 and 63 receive 0; the
-// tile's outer columns are halo and never updated).
+// by one lane moves a register between them (lanes 0 and 63 receive 0 through
+// bound_ctrl; the tile's outer columns are halo and never updated).
 __device__ __forceinline__ int lane_shr1(int v) {  // lane i <- lane i-1
-    return __builtin_amdgcn_update_dpp(0, v, 0x138, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true);
 }
 __device__ __forceinline__ int lane_shl1(int v) {  // lane i <- lane i+1
-    return __builtin_amdgcn_update_dpp(0, v, 0x130, 0xf, 0xf, false);
+    return __builtin_amdgcn_mov_dpp(v, 0x130, 0xf, 0xf, true);
 }
 __device__ __forceinline__ float from_left(float v) {
     return __int_as_float(lane_shr1(__float_as_int(v)));
@@ -347,6 +347,176 @@ rb_zmarch_tile_xl(const ZGeo &g, int tile, const T *__restrict__ Pin,
                 if (d[q] >= R && in_dom[q]) Pout[col[q] + plane * mo] = p[q][R];
         }
         s0 = (s0 + 1 == NSLOT) ? 0 : s0 + 1;
+    }
+}
+
+// Raw buffer loads (gfx9 descriptor: base, stride 0, size in bytes, word 3
+// 0x00020000).  An offset at or past the size returns 0 without a memory
+// access, which zero-fills out-of-domain lanes and planes without branches.
+constexpr uint32_t kBufMax = 1u << 30;   // field size limit of the buffer forms
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const T *p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)p, (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, float) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ double buf_ld(__amdgpu_buffer_rsrc_t r, uint32_t off, double) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+// A store at an offset beyond the field is discarded.
+__device__ __forceinline__ void buf_st(float v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)off, 0, 0);
+}
+typedef unsigned int tse_u32x2 __attribute__((__vector_size__(8)));
+__device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tse_u32x2, v), r, (int)off, 0, 0);
+}
+
+// Variant of rb_zmarch_tile_xl with the z-loop unrolled by the ring length
+// L = 2*NS + 2: plane m sits in slot (m - kl0) mod L of every per-thread ring
+// and of the LDS rings, so the rings rotate by renaming instead of register
+// moves.  The coefficient planes are loaded straight into their slots one
+// step ahead (the slot's previous plane, k+1-L, is no longer read), P and Wz
+// through one prefetch register each.  Loads are raw buffer loads at one
+// shared 32-bit byte offset per step; lanes outside the grid and planes
+// past the tile's last plane use offsets beyond the field and read 0 (fields
+// up to kBufMax bytes: in-grid offsets < 2^30, outside-lane column offset
+// 2^30, past-the-end plane offset 2^31, so no sum wraps); planes outside
+// [kl0, kl1] therefore hold invD = 0 and are skipped by the Dirichlet test,
+// which makes a separate plane-range test unnecessary.  The output store
+// goes through a buffer store whose offset is beyond the field for lanes and
+// planes that must not be written.  Phase h at step k updates plane k-1-h in
+// colour h&1, i.e. the cells with (i+j+k-1) even whatever h is, so the
+// colour test is a lane constant that alternates between steps.  Same
+// operands in the same order as rb_zmarch_tile_xl: bit-identical results.
+template <typename T, int NS, int BYT, int RY>
+__device__ __forceinline__ void
+rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
+                  T *__restrict__ Pout, const T *__restrict__ Wx,
+                  const T *__restrict__ Wy, const T *__restrict__ Wz,
+                  const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
+                  int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int BY = BYT * RY;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int L = R + 2;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[L][BY][BX];
+    __shared__ T sWy[L][BY][BX];
+
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x;
+    const int i = tx * TX - R + lx;
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const int klast = k1 + R - 1;
+    const uint32_t pbytes = (uint32_t)g.nx * (uint32_t)g.ny * (uint32_t)sizeof(T);
+    const uint32_t fbytes = pbytes * (uint32_t)g.nz;
+    const auto rP = buf_rsrc(Pin, zero_init ? 0u : fbytes);
+    const auto rWx = buf_rsrc(Wx, fbytes), rWy = buf_rsrc(Wy, fbytes);
+    const auto rWz = buf_rsrc(Wz, fbytes), rI = buf_rsrc(invD, fbytes);
+    const auto rB = buf_rsrc(rhs, fbytes);
+    const auto rOut = buf_rsrc(Pout, fbytes);
+    const int dxe = min(lx, BX - 1 - lx);
+
+    int ly[RY];
+    bool in_dom[RY], cm[RY], dm[RY][R];
+    uint32_t ocol[RY], oout[RY];
+    T p[RY][L], rwz[RY][L], rinv[RY][L], rb[RY][L], rwx[RY][L], rwy[RY][L];
+    T np[RY], nwz[RY];
+    const uint32_t ok0 = kl0 <= kl1 ? (uint32_t)kl0 * pbytes : 0x80000000u;
+#pragma unroll
+    for (int q = 0; q < RY; ++q) {
+        ly[q] = threadIdx.y + q * BYT;
+        const int j = ty * TY - R + ly[q];
+        in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+        const int d = min(dxe, min(ly[q], BY - 1 - ly[q]));
+#pragma unroll
+        for (int h = 0; h < R; ++h) dm[q][h] = in_dom[q] && d >= h + 1;
+        // colour test of the steps with (k - kl0) even
+        cm[q] = ((i + j + kl0 - 1) & 1) == 0;
+        ocol[q] = in_dom[q] ? (uint32_t)(i + g.nx * j) * (uint32_t)sizeof(T) : kBufMax;
+        oout[q] = dm[q][R - 1] ? ocol[q] : kBufMax;   // tile interior only
+#pragma unroll
+        for (int r = 1; r < L; ++r)
+            p[q][r] = rwz[q][r] = rinv[q][r] = rb[q][r] = rwx[q][r] = rwy[q][r] = T(0);
+        const uint32_t o = ocol[q] + ok0;
+        np[q] = buf_ld(rP, o, T());
+        nwz[q] = buf_ld(rWz, o, T());
+        rinv[q][0] = buf_ld(rI, o, T());
+        rb[q][0] = buf_ld(rB, o, T());
+        rwx[q][0] = buf_ld(rWx, o, T());
+        rwy[q][0] = buf_ld(rWy, o, T());
+        p[q][0] = rwz[q][0] = T(0);
+    }
+
+    for (int kb = kl0; kb <= klast; kb += L) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            const int k = kb + u;
+            if (k > klast) break;
+            const int un = (u + 1) % L;
+#pragma unroll
+            for (int q = 0; q < RY; ++q) {
+                p[q][u] = np[q];
+                rwz[q][u] = nwz[q];
+                sP[u][ly[q]][lx] = np[q];
+                sWy[u][ly[q]][lx] = rwy[q][u];
+            }
+            // plane k+1 (reads 0 past kl1 and outside the grid)
+            const uint32_t ok = k + 1 <= kl1 ? (uint32_t)(k + 1) * pbytes : 0x80000000u;
+#pragma unroll
+            for (int q = 0; q < RY; ++q) {
+                const uint32_t o = ocol[q] + ok;
+                np[q] = buf_ld(rP, o, T());
+                nwz[q] = buf_ld(rWz, o, T());
+                rinv[q][un] = buf_ld(rI, o, T());
+                rb[q][un] = buf_ld(rB, o, T());
+                rwx[q][un] = buf_ld(rWx, o, T());
+                rwy[q][un] = buf_ld(rWy, o, T());
+            }
+            __syncthreads();
+#pragma unroll
+            for (int h = 0; h < R; ++h) {
+                const int sm = (u - 1 - h + 2 * L) % L;     // plane m = k-1-h
+                const int sa = (u - h + 2 * L) % L;         // plane m + 1
+                const int sb = (u - 2 - h + 2 * L) % L;     // plane m - 1
+#pragma unroll
+                for (int q = 0; q < RY; ++q) {
+                    const T pl = from_left(p[q][sm]);
+                    const T pr = from_right(p[q][sm]);
+                    const T wxl = from_left(rwx[q][sm]);
+                    const bool col_ok = (u & 1) ? !cm[q] : cm[q];
+                    if (dm[q][h] && col_ok && rinv[q][sm] > T(0)) {
+                        const int y = ly[q];
+                        T acc = rb[q][sm];
+                        acc += rwx[q][sm] * pr;
+                        acc += wxl * pl;
+                        acc += rwy[q][sm] * sP[sm][y + 1][lx];
+                        acc += sWy[sm][y - 1][lx] * sP[sm][y - 1][lx];
+                        acc += rwz[q][sm] * p[q][sa];
+                        acc += rwz[q][sb] * p[q][sb];
+                        const T pc = p[q][sm];
+                        const T pn = pc + omega * (acc * rinv[q][sm] - pc);
+                        p[q][sm] = pn;
+                        sP[sm][y][lx] = pn;
+                    }
+                }
+            }
+            const int mo = k - R;
+            const uint32_t oo = mo >= k0 && mo < k1 ? (uint32_t)mo * pbytes : 0x80000000u;
+#pragma unroll
+            for (int q = 0; q < RY; ++q)
+                buf_st(p[q][(u - R + 2 * L) % L], rOut, oout[q] + oo);
+        }
     }
 }
 
@@ -622,6 +792,7 @@ struct FastSys {
 FastSys *system_of(tse_ctx *ctx, int kind);
 int ensure_sys(tse_ctx *ctx, FastSys *s);
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind);
+bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32);
 ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk);
 
 }  // namespace tse

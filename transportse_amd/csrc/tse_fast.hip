@@ -56,6 +56,18 @@ rb_zmarch_xl_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                       omega, zero_init);
 }
 
+template <typename T, int NS, int BYT, int RY>
+__global__ void __launch_bounds__(64 * BYT)
+rb_zmarch_xr_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                    const T *__restrict__ Wx, const T *__restrict__ Wy,
+                    const T *__restrict__ Wz, const T *__restrict__ invD,
+                    const T *__restrict__ rhs, T omega, int zero_init) {
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    rb_zmarch_tile_xr<T, NS, BYT, RY>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs,
+                                      omega, zero_init);
+}
+
 // ---- operator assembly (from split-layout fields) ---------------------------------------
 
 __device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
@@ -400,6 +412,9 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
     TSE_ZM(1, rb_zmarch_xl_kernel, 1, 16, 1)
     TSE_ZM(1, rb_zmarch_xl_kernel, 2, 16, 1)
     TSE_ZM(1, rb_zmarch_xl_kernel, 1, 12, 2)
+    TSE_ZM(2, rb_zmarch_xr_kernel, 1, 16, 1)
+    TSE_ZM(2, rb_zmarch_xr_kernel, 2, 16, 1)
+    TSE_ZM(2, rb_zmarch_xr_kernel, 1, 12, 2)
     if constexpr (sizeof(T) == 4) {
         TSE_ZM(0, rb_zmarch_kernel, 1, 16, 2)
         TSE_ZM(0, rb_zmarch_kernel, 2, 16, 2)
@@ -409,6 +424,11 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
         TSE_ZM(1, rb_zmarch_xl_kernel, 2, 16, 2)
         TSE_ZM(1, rb_zmarch_xl_kernel, 3, 16, 1)
         TSE_ZM(1, rb_zmarch_xl_kernel, 3, 12, 2)
+        TSE_ZM(2, rb_zmarch_xr_kernel, 2, 12, 2)
+        TSE_ZM(2, rb_zmarch_xr_kernel, 2, 16, 2)
+        TSE_ZM(2, rb_zmarch_xr_kernel, 3, 16, 1)
+        TSE_ZM(2, rb_zmarch_xr_kernel, 3, 12, 2)
+        TSE_ZM(2, rb_zmarch_xr_kernel, 3, 16, 2)
     }
 #undef TSE_ZM
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
@@ -437,6 +457,11 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
     return fail(TSE_ERR_ARG, "unsupported transport sweep plan (ns, by)");
 }
 
+// Variant 2 addresses a field through a buffer descriptor (up to 1 GiB).
+bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
+    return pl.variant != 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
+}
+
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     if (kind == SYS_TR)
         return pl.variant == 0 && pl.ry == 1 &&
@@ -445,10 +470,13 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     // {variant, ns, by, ry}
     static const int both[][4] = {{0, 1, 8, 1},  {0, 1, 16, 1}, {0, 2, 16, 1},
                                   {0, 1, 24, 2}, {1, 1, 16, 1}, {1, 2, 16, 1},
-                                  {1, 1, 24, 2}};
+                                  {1, 1, 24, 2}, {2, 1, 16, 1}, {2, 2, 16, 1},
+                                  {2, 1, 24, 2}};
     static const int f32[][4] = {{0, 1, 32, 2}, {0, 2, 32, 2}, {0, 2, 32, 4},
                                  {0, 3, 16, 1}, {1, 2, 24, 2}, {1, 2, 32, 2},
-                                 {1, 3, 16, 1}, {1, 3, 24, 2}};
+                                 {1, 3, 16, 1}, {1, 3, 24, 2}, {2, 2, 24, 2},
+                                 {2, 2, 32, 2}, {2, 3, 16, 1}, {2, 3, 24, 2},
+                                 {2, 3, 32, 2}};
     auto hit = [&](const int c[4]) {
         return pl.variant == c[0] && pl.ns == c[1] && pl.by == c[2] && pl.ry == c[3];
     };
@@ -560,6 +588,8 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     if (!s->ready || !plan) return fail(TSE_ERR_STATE, "fast form not set up");
     if (!fast_plan_ok(*plan, fp32, s->kind) || n_sweeps < 0 || n_sweeps % plan->ns)
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
+    if (!fast_plan_fits(*plan, s->N, fp32))
+        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
     const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
     int e;  // FP32 timing path: correction sweeps on the last residual
@@ -577,6 +607,8 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (!fast_plan_ok(pl, fp32, s->kind) || chunk < pl.ns || chunk % pl.ns ||
         prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    if (!fast_plan_fits(pl, s->N, fp32))
+        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
     const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk);
     double rn, bn;
     int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
