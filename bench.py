@@ -61,7 +61,8 @@ BYTES_FP64_SWEEP = 56
 # calibrated on a 1 GiB copy (profiles/rb_zmarch*_fp32_pressure_512x512x256.md)
 TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64, 0): 3.337e9,
                       (512, 512, 256, 2, 16, 1, 128, 1): 3.167e9,
-                      (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9}
+                      (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9,
+                      (512, 512, 256, 3, 32, 2, 32, 2): 3.776e9}
 # sweep kernel and its profile summary per plan variant
 VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
                   1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),
@@ -174,12 +175,13 @@ def main(argv=None):
         stats.append(step("mixed", omega))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    sweeps = sum(s.sweeps for s in stats)
+    solve_sweeps = sum(s.sweeps for s in stats)
     conv = all(s.converged for s in stats)
-    value = float(N) * sweeps / elapsed
+    value = float(N) * solve_sweeps / elapsed
     tts = elapsed / args.steps
-    log(f"timed {args.steps} mixed solves: {tts:.3f}s each, {sweeps // args.steps}"
-        f" sweeps, converged={conv}, {value:.3e} cell-updates/s")
+    log(f"timed {args.steps} mixed solves: {tts:.3f}s each, "
+        f"{solve_sweeps // args.steps} sweeps, converged={conv}, "
+        f"{value:.3e} cell-updates/s")
     P_mixed = fp.result()
 
     # dominant kernel: the FP32 fused sweep, HIP events on the library stream
@@ -287,7 +289,7 @@ def main(argv=None):
                    "plan_ns_by_ry_zchunk_variant": list(plan32)},
         "time_to_solution_s": tts,
         "solves_per_s": 1.0 / tts,
-        "sweeps_per_solve": sweeps // args.steps,
+        "sweeps_per_solve": solve_sweeps // args.steps,
         "rel_residual": stats[-1].rel_residual,
         "converged": conv,
         "plain64": plain,

@@ -188,13 +188,17 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
 /* Launch plan of the fused sweep kernel: ns sweeps per launch, tile of
  * 64 x by columns with ry tile rows per thread (blocks of 64 x by/ry
  * threads), zchunk z planes per block (<= 0: whole column), variant 0
- * (x neighbours through LDS) or 1 (x neighbours exchanged across the lanes
- * of a wavefront; smaller LDS ring).  Supported (ns, by, ry):
+ * (x neighbours through LDS), 1 (x neighbours exchanged across the lanes
+ * of a wavefront; smaller LDS ring) or 2 (as 1, with the register rings
+ * rotated by unrolling and buffer loads; fields of at most 1 GiB, else
+ * TSE_ERR_ARG).  Supported (ns, by, ry):
  *   variant 0: (1,8,1) (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also
  *              (1,32,2) (2,32,2) (2,32,4) (3,16,1)
  *   variant 1: (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also (2,24,2)
  *              (2,32,2) (3,16,1) (3,24,2)
- * Transport systems: variant 0, (1,8,1) (1,16,1) (2,16,1), FP32 (3,16,1). */
+ *   variant 2: as variant 1, FP32 sweeps also (3,32,2)
+ * Transport systems: variant 0 or 2, (1,8,1) (1,16,1) (2,16,1), FP32
+ * (3,16,1). */
 typedef struct {
     int32_t ns, by, ry, zchunk, variant;
 } tse_sweep_plan;

@@ -67,6 +67,7 @@ CASES = [(0, "mixed", (2, 16, 1, 0), 1e-10),
          (0, "plain64", (2, 16, 1, 9, 2), 1e-11),
          (0, "plain64", (1, 16, 1, 7), 1e-11),
          (1, "mixed", (3, 16, 1, 8), 1e-11),
+         (1, "mixed", (2, 16, 1, 5, 2), 1e-11),
          (1, "plain64", (2, 16, 1, 0), 1e-12)]
 
 

@@ -20,9 +20,13 @@ from helpers import Fields
 
 pytestmark = pytest.mark.gpu
 
-# (ns, by, ry, zchunk): every transport kernel variant, with and without z-chunks
-PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 5), (2, 16, 1, 0), (2, 16, 1, 6)]
-PLANS32 = [(1, 16, 1, 0), (2, 16, 1, 7), (3, 16, 1, 0), (3, 16, 1, 9)]
+# (ns, by, ry, zchunk, variant): every transport kernel variant, with and
+# without z-chunks; variant 2 rotates its rings by unrolling, reads through
+# buffer loads and takes x neighbours from the adjacent lanes
+PLANS64 = [(1, 8, 1, 0, 0), (1, 16, 1, 5, 0), (2, 16, 1, 0, 0), (2, 16, 1, 6, 0),
+           (1, 8, 1, 0, 2), (2, 16, 1, 6, 2)]
+PLANS32 = [(1, 16, 1, 0, 0), (2, 16, 1, 7, 0), (3, 16, 1, 0, 0), (3, 16, 1, 9, 0),
+           (1, 16, 1, 5, 2), (2, 16, 1, 0, 2), (3, 16, 1, 9, 2)]
 SHAPES = [(70, 23, 19), (33, 20, 17), (9, 40, 12)]
 
 
@@ -173,3 +177,23 @@ def test_unit_dimension_grids_fast_forms(shape):
     ref = _oracle_c(f, dt, 1.0, True)
     _check(_fast_c(f, dt, 1.0, True), ref, "plain64", 1e-13, 1e-10,
            (2, 16, 1, 0), "C")
+
+
+@pytest.mark.parametrize("form,plan", [("plain64", (1, 8, 1, 0)), ("plain64", (2, 16, 1, 6)),
+                                       ("mixed", (2, 16, 1, 7)), ("mixed", (3, 16, 1, 0))])
+def test_transport_variant2_equals_variant0(form, plan):
+    """The transport variant 2 reads the same operands in the same order as
+    variant 0 (x neighbours from the adjacent lanes instead of LDS): whole
+    solves are bit-identical, with the same sweep count."""
+    f = Fields(67, 37, 19, seed=31)
+    out = []
+    for var in (0, 2):
+        ft = _fast_c(f, 43200.0, 1.0, True)
+        st = ft.solve(form, tol=1e-12 if form == "plain64" else 1e-11, sor=1.2,
+                      max_sweeps=4000, inner=6 * plan[0], plan=plan + (var,))
+        assert st.converged
+        out.append((ft.result(), st.sweeps))
+    print("transport variant 2 vs 0", form, plan, "identical",
+          np.array_equal(out[0][0], out[1][0]), "sweeps", out[0][1], out[1][1],
+          flush=True)
+    assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])

@@ -27,8 +27,9 @@ PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
                      (2, 24, 2, 2), (2, 32, 2, 2), (3, 16, 1, 2), (3, 24, 2, 2),
                      (3, 32, 2, 2)]
 # transport: the cell's own coefficients stay in registers, one row per thread
-TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0)]
-TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0)]
+TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
+              (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]
+TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0), (3, 16, 1, 2)]
 ZCHUNKS = (0, 128, 64, 32)
 
 KINDS = {

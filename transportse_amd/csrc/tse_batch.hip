@@ -116,7 +116,7 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
     }
 }
 
-template <typename T, int NS, int BY>
+template <typename T, int NS, int BY, int VAR>
 __global__ void __launch_bounds__(64 * BY)
 batch_tr_zmarch_kernel(const BMember *__restrict__ mem,
                        const BStat *__restrict__ st,
@@ -129,15 +129,31 @@ batch_tr_zmarch_kernel(const BMember *__restrict__ mem,
         const BMember &m = mem[w.x];
         if (!st[w.x].active) continue;
         const T omega = (T)m.sor;
+        const T *pin, *coef, *inv, *rhs;
+        T *pout;
+        int zero;
         if constexpr (sizeof(T) == 4) {
-            tr_zmarch_tile<T, NS, BY>(m.g, w.y, m.e[launch & 1],
-                                      m.e[(launch + 1) & 1], m.top.c32, m.N,
-                                      m.top.invD32, m.r32, omega, launch == 0);
+            pin = m.e[launch & 1];
+            pout = m.e[(launch + 1) & 1];
+            coef = m.top.c32;
+            inv = m.top.invD32;
+            rhs = m.r32;
+            zero = launch == 0;
         } else {
             const int c = (st[w.x].cur + launch) & 1;
-            tr_zmarch_tile<T, NS, BY>(m.g, w.y, m.p[c], m.p[c ^ 1], m.top.c,
-                                      m.N, m.top.invD, m.top.b, omega, 0);
+            pin = m.p[c];
+            pout = m.p[c ^ 1];
+            coef = m.top.c;
+            inv = m.top.invD;
+            rhs = m.top.b;
+            zero = 0;
         }
+        if constexpr (VAR == 0)
+            tr_zmarch_tile<T, NS, BY>(m.g, w.y, pin, pout, coef, m.N, inv, rhs, omega,
+                                      zero);
+        else
+            tr_zmarch_tile_xr<T, NS, BY>(m.g, w.y, pin, pout, coef, m.N, inv, rhs,
+                                         omega, zero);
         __syncthreads();
     }
 }
@@ -370,9 +386,9 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_B_TRY(hipGetLastError());                                                  \
         return 0;                                                                      \
     }
-#define TSE_BT(NS_, BY_)                                                               \
-    if (b->kind == SYS_TR && v == 0 && ns == NS_ && by == BY_ && ry == 1) {            \
-        auto k = batch_tr_zmarch_kernel<T, NS_, BY_>;                                  \
+#define TSE_BT(V_, NS_, BY_)                                                           \
+    if (b->kind == SYS_TR && v == V_ && ns == NS_ && by == BY_ && ry == 1) {           \
+        auto k = batch_tr_zmarch_kernel<T, NS_, BY_, V_>;                              \
         if (size_only) return occupancy_grid(k, 64 * BY_, b->n_items, &b->grid);      \
         hipLaunchKernelGGL(k, dim3(b->grid), dim3(64, BY_), 0, b->stream, b->dmem,     \
                            b->dst, b->ditems, b->n_items, b->dqueue, base, launch);    \
@@ -389,9 +405,12 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
     TSE_BP(2, 1, 16, 1)
     TSE_BP(2, 2, 16, 1)
     TSE_BP(2, 1, 12, 2)
-    TSE_BT(1, 8)
-    TSE_BT(1, 16)
-    TSE_BT(2, 16)
+    TSE_BT(0, 1, 8)
+    TSE_BT(0, 1, 16)
+    TSE_BT(0, 2, 16)
+    TSE_BT(2, 1, 8)
+    TSE_BT(2, 1, 16)
+    TSE_BT(2, 2, 16)
     if constexpr (sizeof(T) == 4) {
         TSE_BP(0, 1, 16, 2)
         TSE_BP(0, 2, 16, 2)
@@ -406,7 +425,8 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_BP(2, 3, 16, 1)
         TSE_BP(2, 3, 12, 2)
         TSE_BP(2, 3, 16, 2)
-        TSE_BT(3, 16)
+        TSE_BT(0, 3, 16)
+        TSE_BT(2, 3, 16)
     }
 #undef TSE_BP
 #undef TSE_BT

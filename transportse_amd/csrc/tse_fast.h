@@ -651,6 +651,117 @@ tr_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
 }
 
 
+// Transport z-march with the structure of rb_zmarch_tile_xr: rings of
+// length L = 2*NS + 2 rotated by unrolling, the eight coefficient planes
+// (six faces, invD, rhs) loaded by buffer loads straight into their slots
+// one step ahead, out-of-grid lanes and planes past the tile reading 0, x
+// neighbours from the adjacent lanes (DPP), y neighbours through LDS.  Same
+// operands in the same order as tr_zmarch_tile: bit-identical results.
+template <typename T, int NS, int BY>
+__device__ __forceinline__ void
+tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
+                  T *__restrict__ Pout, const T *__restrict__ coef, long N,
+                  const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
+                  int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int TX = BX - 2 * R;
+    constexpr int TY = BY - 2 * R;
+    constexpr int L = R + 2;
+    static_assert(TY >= 1, "tile too small");
+    __shared__ T sP[L][BY][BX];
+
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const int lx = threadIdx.x, ly = threadIdx.y;
+    const int i = tx * TX - R + lx;
+    const int j = ty * TY - R + ly;
+    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const int klast = k1 + R - 1;
+    const uint32_t pbytes = (uint32_t)g.nx * (uint32_t)g.ny * (uint32_t)sizeof(T);
+    const uint32_t fbytes = pbytes * (uint32_t)g.nz;
+    const auto rP = buf_rsrc(Pin, zero_init ? 0u : fbytes);
+    const auto rI = buf_rsrc(invD, fbytes), rB = buf_rsrc(rhs, fbytes);
+    const auto rOut = buf_rsrc(Pout, fbytes);
+    __amdgpu_buffer_rsrc_t rC[6];
+#pragma unroll
+    for (int f = 0; f < 6; ++f) rC[f] = buf_rsrc(coef + f * N, fbytes);
+    bool dm[R];
+#pragma unroll
+    for (int h = 0; h < R; ++h) dm[h] = in_dom && d >= h + 1;
+    const bool cm = ((i + j + kl0 - 1) & 1) == 0;   // colour test, (k-kl0) even
+    const uint32_t ocol = in_dom ? (uint32_t)(i + g.nx * j) * (uint32_t)sizeof(T) : kBufMax;
+    const uint32_t oout = dm[R - 1] ? ocol : kBufMax;
+
+    T p[L], rinv[L], rb[L], rc[6][L];
+#pragma unroll
+    for (int r = 1; r < L; ++r) {
+        p[r] = rinv[r] = rb[r] = T(0);
+#pragma unroll
+        for (int f = 0; f < 6; ++f) rc[f][r] = T(0);
+    }
+    p[0] = T(0);
+    T np;
+    {
+        const uint32_t o = ocol + (kl0 <= kl1 ? (uint32_t)kl0 * pbytes : 0x80000000u);
+        np = buf_ld(rP, o, T());
+        rinv[0] = buf_ld(rI, o, T());
+        rb[0] = buf_ld(rB, o, T());
+#pragma unroll
+        for (int f = 0; f < 6; ++f) rc[f][0] = buf_ld(rC[f], o, T());
+    }
+    for (int kb = kl0; kb <= klast; kb += L) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            const int k = kb + u;
+            if (k > klast) break;
+            const int un = (u + 1) % L;
+            p[u] = np;
+            sP[u][ly][lx] = np;
+            const uint32_t o =
+                ocol + (k + 1 <= kl1 ? (uint32_t)(k + 1) * pbytes : 0x80000000u);
+            np = buf_ld(rP, o, T());
+            rinv[un] = buf_ld(rI, o, T());
+            rb[un] = buf_ld(rB, o, T());
+#pragma unroll
+            for (int f = 0; f < 6; ++f) rc[f][un] = buf_ld(rC[f], o, T());
+            __syncthreads();
+            const bool col_ok = (u & 1) ? !cm : cm;
+#pragma unroll
+            for (int h = 0; h < R; ++h) {
+                const int sm = (u - 1 - h + 2 * L) % L;     // plane m = k-1-h
+                const int sa = (u - h + 2 * L) % L;         // plane m + 1
+                const int sb = (u - 2 - h + 2 * L) % L;     // plane m - 1
+                const T pl = from_left(p[sm]);
+                const T pr = from_right(p[sm]);
+                if (dm[h] && col_ok && rinv[sm] != T(0)) {
+                    T acc = rb[sm];
+                    acc += rc[0][sm] * pr;
+                    acc += rc[1][sm] * pl;
+                    acc += rc[2][sm] * sP[sm][ly + 1][lx];
+                    acc += rc[3][sm] * sP[sm][ly - 1][lx];
+                    acc += rc[4][sm] * p[sa];
+                    acc += rc[5][sm] * p[sb];
+                    const T pc = p[sm];
+                    const T pn = pc + omega * (acc * rinv[sm] - pc);
+                    p[sm] = pn;
+                    sP[sm][ly][lx] = pn;
+                }
+            }
+            const int mo = k - R;
+            buf_st(p[(u - R + 2 * L) % L], rOut,
+                   oout + (mo >= k0 && mo < k1 ? (uint32_t)mo * pbytes : 0x80000000u));
+        }
+    }
+}
+
 __device__ inline double wsum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

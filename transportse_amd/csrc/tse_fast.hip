@@ -273,15 +273,19 @@ tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
     }
 }
 
-template <typename T, int NS, int BY>
+template <typename T, int NS, int BY, int VAR>
 __global__ void __launch_bounds__(64 * BY)
 tr_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                  const T *__restrict__ coef, long N, const T *__restrict__ invD,
                  const T *__restrict__ rhs, T omega, int zero_init) {
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;
-    tr_zmarch_tile<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
-                              zero_init);
+    if constexpr (VAR == 0)
+        tr_zmarch_tile<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
+                                  zero_init);
+    else
+        tr_zmarch_tile_xr<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
+                                     zero_init);
 }
 
 template <int MODE>
@@ -435,36 +439,40 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
 }
 
 template <typename T>
-static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, int ns, int by,
+static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
                             const T *Pin, T *Pout, const T *coef, long N,
                             const T *invD, const T *rhs, T omega, int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
-#define TSE_TZ(NS_, BY_)                                                          \
-    if (ns == NS_ && by == BY_) {                                                 \
-        hipLaunchKernelGGL((tr_zmarch_kernel<T, NS_, BY_>), grid, dim3(64, BY_),   \
-                           0, st, zg, Pin, Pout, coef, N, invD, rhs, omega,       \
-                           zero_init);                                            \
+    const int ns = pl.ns, by = pl.by, v = pl.variant;
+#define TSE_TZ(V_, NS_, BY_)                                                      \
+    if (v == V_ && ns == NS_ && by == BY_) {                                      \
+        hipLaunchKernelGGL((tr_zmarch_kernel<T, NS_, BY_, V_>), grid,             \
+                           dim3(64, BY_), 0, st, zg, Pin, Pout, coef, N, invD,    \
+                           rhs, omega, zero_init);                                \
         TSE_HIP_TRY(hipGetLastError());                                           \
         return 0;                                                                 \
     }
-    TSE_TZ(1, 8)
-    TSE_TZ(1, 16)
-    TSE_TZ(2, 16)
+    TSE_TZ(0, 1, 8)
+    TSE_TZ(0, 1, 16)
+    TSE_TZ(0, 2, 16)
+    TSE_TZ(2, 1, 8)
+    TSE_TZ(2, 1, 16)
+    TSE_TZ(2, 2, 16)
     if constexpr (sizeof(T) == 4) {
-        TSE_TZ(3, 16)
+        TSE_TZ(0, 3, 16)
+        TSE_TZ(2, 3, 16)
     }
 #undef TSE_TZ
-    return fail(TSE_ERR_ARG, "unsupported transport sweep plan (ns, by)");
+    return fail(TSE_ERR_ARG, "unsupported transport sweep plan (ns, by, variant)");
 }
 
-// Variant 2 addresses a field through a buffer descriptor (up to 1 GiB).
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
     return pl.variant != 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
 }
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     if (kind == SYS_TR)
-        return pl.variant == 0 && pl.ry == 1 &&
+        return (pl.variant == 0 || pl.variant == 2) && pl.ry == 1 &&
                ((pl.ns == 1 && (pl.by == 8 || pl.by == 16)) ||
                 (pl.ns == 2 && pl.by == 16) || (fp32 && pl.ns == 3 && pl.by == 16));
     // {variant, ns, by, ry}
@@ -517,7 +525,7 @@ static int plain_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
                                        op.Wy, op.Wz, op.invD, op.b, sor, 0);
         } else {
             TrOp op = s->trop();
-            rc = launch_tr_zmarch<double>(ctx->stream, zg, pl.ns, pl.by,
+            rc = launch_tr_zmarch<double>(ctx->stream, zg, pl,
                                           s->p(s->cur), s->p(1 - s->cur), op.c,
                                           s->N, op.invD, op.b, sor, 0);
         }
@@ -542,7 +550,7 @@ static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
                                       op.Wz32, op.invD32, s->r32(), sor, q == 0);
         } else {
             TrOp op = s->trop();
-            rc = launch_tr_zmarch<float>(ctx->stream, zg, pl.ns, pl.by, s->e(e),
+            rc = launch_tr_zmarch<float>(ctx->stream, zg, pl, s->e(e),
                                          s->e(1 - e), op.c32, s->N, op.invD32,
                                          s->r32(), sor, q == 0);
         }
