@@ -62,7 +62,10 @@ BYTES_FP64_SWEEP = 56
 TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64, 0): 3.337e9,
                       (512, 512, 256, 2, 16, 1, 128, 1): 3.167e9,
                       (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9,
-                      (512, 512, 256, 3, 32, 2, 32, 2): 3.776e9}
+                      (512, 512, 256, 3, 32, 2, 32, 2): 3.776e9,
+                      (512, 512, 256, 3, 32, 2, 64, 2): 3.645e9,
+                      (512, 512, 256, 3, 32, 2, 128, 2): 3.473e9,
+                      (512, 512, 256, 2, 32, 2, 32, 2): 3.424e9}
 # sweep kernel and its profile summary per plan variant
 VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
                   1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),

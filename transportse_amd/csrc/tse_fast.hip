@@ -205,25 +205,6 @@ nat_to_split_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ 
 
 // ---- FP64 residual r = b - A p (and p <- p + e) ------------------------------------------------
 
-template <int MODE>
-__global__ void __launch_bounds__(kBlock)
-residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
-                const double *__restrict__ pin, const float *__restrict__ e,
-                double *__restrict__ pout, FastOp op, float *__restrict__ r32,
-                double *__restrict__ partials) {
-    const long N = (long)nx * ny * nz;
-    double sr = 0.0, sb = 0.0;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
-         t += (long)gridDim.x * blockDim.x) {
-        const int row = (int)fdiv((uint32_t)t, dnx);
-        const int i = (int)t - row * nx;
-        const int k = (int)fdiv((uint32_t)row, dny);
-        const int j = row - k * ny;
-        p_residual_cell<MODE>(nx, ny, nz, i, j, k, t, pin, e, pout, op, r32, sr, sb);
-    }
-    block_sum2_to(sr, sb, partials + 2 * blockIdx.x);
-}
-
 __global__ void __launch_bounds__(kBlock)
 reduce2_kernel(const double *__restrict__ partials, int n, double *__restrict__ out) {
     double a = 0, c = 0;
@@ -288,23 +269,103 @@ tr_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                      zero_init);
 }
 
-template <int MODE>
+// Residual as a z-march: a block (kBlock = 64 x 4 threads) takes a tile of
+// 64 x 4 columns over one z-chunk; each thread keeps the unknown of planes
+// k-1 and k+1 (and, for pressure, Wz of plane k-1) in registers, so each
+// operand is read from memory once, and the x / y neighbours, read by the
+// adjacent threads in the same plane, come from cache.  The per-cell
+// arithmetic is that of p_residual_cell / tr_residual_cell (same terms in
+// the same order); only the grouping of the norm partial sums differs.
+struct RzGeo {
+    int tiles_x, tiles_y, zchunk, ntiles;
+};
+
+template <int KIND, int MODE>
 __global__ void __launch_bounds__(kBlock)
-tr_residual_kernel(int nx, int ny, int nz, FastDiv dnx, FastDiv dny,
-                   const double *__restrict__ pin, const float *__restrict__ e,
-                   double *__restrict__ pout, TrOp op, float *__restrict__ r32,
-                   double *__restrict__ partials) {
-    const long N = op.N;
+residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ pin,
+                   const float *__restrict__ e, double *__restrict__ pout, FastOp pop,
+                   TrOp trop, float *__restrict__ r32, double *__restrict__ partials) {
+    const long plane = (long)nx * ny;
+    const long N = plane * nz;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     double sr = 0.0, sb = 0.0;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
-         t += (long)gridDim.x * blockDim.x) {
-        const int row = (int)fdiv((uint32_t)t, dnx);
-        const int i = (int)t - row * nx;
-        const int k = (int)fdiv((uint32_t)row, dny);
-        const int j = row - k * ny;
-        tr_residual_cell<MODE>(nx, ny, nz, i, j, k, t, pin, e, pout, op, r32, sr, sb);
+    auto X = [&](long o) -> double {
+        return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
+    };
+    // x / y neighbours of the edge cells are read at clamped indices and
+    // discarded by the selects below, so every load of a cell is issued
+    // without a branch
+    for (int tile = blockIdx.x; tile < rg.ntiles; tile += gridDim.x) {
+        const int txy = tile % (rg.tiles_x * rg.tiles_y);
+        const int tz = tile / (rg.tiles_x * rg.tiles_y);
+        const int i = (txy % rg.tiles_x) * 64 + lx;
+        const int j = (txy / rg.tiles_x) * 4 + ly;
+        if (i >= nx || j >= ny) continue;
+        const bool ie = i < nx - 1, iw = i > 0, jn = j < ny - 1, js = j > 0;
+        const long dxe = ie ? 1 : 0, dxw = iw ? 1 : 0, dyn = jn ? nx : 0, dys = js ? nx : 0;
+        const int k0 = tz * rg.zchunk, k1 = min(k0 + rg.zchunk, nz);
+        long t = (long)i + (long)nx * j + plane * k0;
+        double pm = k0 > 0 ? X(t - plane) : 0.0, wzm = 0.0;
+        if constexpr (KIND == SYS_P)
+            wzm = k0 > 0 ? pop.Wz[t - plane] : 0.0;
+        double pc = X(t);
+#pragma unroll 2
+        for (int k = k0; k < k1; ++k, t += plane) {
+            const bool kt = k < nz - 1, kb = k > 0;
+            const double pp = X(min(t + plane, N - 1));
+            const double xe = X(t + dxe), xw = X(t - dxw), xn = X(t + dyn), xs = X(t - dys);
+            if (MODE == 1) pout[t] = pc;
+            double r, bb, inv;
+            if constexpr (KIND == SYS_P) {
+                const double wz = pop.Wz[t], we = pop.Wx[t], ww = pop.Wx[t - dxw];
+                const double wn = pop.Wy[t], ws = pop.Wy[t - dys];
+                inv = pop.invD[t] > 0.0 ? 1.0 : 0.0;
+                bb = pop.b[t];
+                double acc = bb;
+                acc = ie ? acc + we * xe : acc;
+                acc = iw ? acc + ww * xw : acc;
+                acc = jn ? acc + wn * xn : acc;
+                acc = js ? acc + ws * xs : acc;
+                acc = kt ? acc + wz * pp : acc;
+                acc = kb ? acc + wzm * pm : acc;
+                r = acc - pop.D[t] * pc;
+                wzm = wz;
+            } else {
+                const long Nc = trop.N;
+                const double c0 = trop.c[0 * Nc + t], c1 = trop.c[1 * Nc + t];
+                const double c2 = trop.c[2 * Nc + t], c3 = trop.c[3 * Nc + t];
+                const double c4 = trop.c[4 * Nc + t], c5 = trop.c[5 * Nc + t];
+                inv = trop.invD[t] != 0.0 ? 1.0 : 0.0;
+                bb = trop.b[t];
+                double acc = bb;
+                acc = ie ? acc + c0 * xe : acc;
+                acc = iw ? acc + c1 * xw : acc;
+                acc = jn ? acc + c2 * xn : acc;
+                acc = js ? acc + c3 * xs : acc;
+                acc = kt ? acc + c4 * pp : acc;
+                acc = kb ? acc + c5 * pm : acc;
+                r = acc - trop.D[t] * pc;
+            }
+            r = inv != 0.0 ? r : 0.0;          // Dirichlet cells: r = 0
+            sr += r * r;
+            sb += inv != 0.0 ? bb * bb : 0.0;
+            if (MODE != 0) r32[t] = (float)r;
+            pm = pc;
+            pc = kt ? pp : 0.0;
+        }
     }
     block_sum2_to(sr, sb, partials + 2 * blockIdx.x);
+}
+
+static RzGeo make_rzgeo(const Geo &G) {
+    RzGeo r;
+    r.tiles_x = (G.nx + 63) / 64;
+    r.tiles_y = (G.ny + 3) / 4;
+    const long txy = (long)r.tiles_x * r.tiles_y;
+    // about 4096 tiles (the partial-sum capacity), chunks of at least 8 planes
+    r.zchunk = (int)std::min<long>(G.nz, std::max<long>(8, (G.nz * txy + 4095) / 4096));
+    r.ntiles = (int)(txy * ((G.nz + r.zchunk - 1) / r.zchunk));
+    return r;
 }
 
 // ---- host side ---------------------------------------------------------------------------------------
@@ -372,16 +433,18 @@ template <int MODE>
 static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
                     double *pout, double *rn, double *bn) {
     const Geo &G = ctx->geo;
-    const int nb = grid_n(s->N);
-    const FastDiv dnx = make_fastdiv(G.nx), dny = make_fastdiv(G.ny);
+    const RzGeo rg = make_rzgeo(G);
+    const int nb = std::max(1, std::min(rg.ntiles, 4096));
+    const FastOp pop = s->kind == SYS_P ? s->pop() : FastOp{};
+    const TrOp trop = s->kind == SYS_TR ? s->trop() : TrOp{};
     if (s->kind == SYS_P)
-        hipLaunchKernelGGL((residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
-                           ctx->stream, G.nx, G.ny, G.nz, dnx, dny, pin, e, pout,
-                           s->pop(), s->r32(), s->partials);
+        hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE>), dim3(nb), dim3(kBlock), 0,
+                           ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
+                           s->r32(), s->partials);
     else
-        hipLaunchKernelGGL((tr_residual_kernel<MODE>), dim3(nb), dim3(kBlock), 0,
-                           ctx->stream, G.nx, G.ny, G.nz, dnx, dny, pin, e, pout,
-                           s->trop(), s->r32(), s->partials);
+        hipLaunchKernelGGL((residual_zm_kernel<SYS_TR, MODE>), dim3(nb), dim3(kBlock), 0,
+                           ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
+                           s->r32(), s->partials);
     TSE_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
                        s->partials, nb, s->red);
