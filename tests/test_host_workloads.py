@@ -53,3 +53,14 @@ def test_footprint_and_initial_fields():
     top = ScenarioSpec(20, 16, 12, salt_patch=0, c_top=0.6)
     g = host_fields(top)
     assert np.all(g["C"][0][:, :, -1] == 0.6) and np.all(g["BC"][:, :, -1] == 1)
+
+
+def test_variant2_plans_filtered_by_field_size():
+    """Variant 2 reads fields through buffer descriptors of at most 1 GiB;
+    the tuners skip it for larger fields (the library would reject it)."""
+    from transportse_amd.autotune import plan_fits
+    assert plan_fits(2, (512, 512, 256), True)            # 256 MiB FP32
+    assert plan_fits(2, (512, 512, 256), False)           # 512 MiB FP64
+    assert plan_fits(2, (1024, 1024, 256), True)          # exactly 1 GiB
+    assert not plan_fits(2, (1024, 1024, 256), False)     # 2 GiB FP64
+    assert plan_fits(1, (2048, 2048, 512), False)         # other variants: any size

@@ -31,6 +31,13 @@ TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
               (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]
 TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0), (3, 16, 1, 2)]
 ZCHUNKS = (0, 128, 64, 32)
+BUF_MAX = 1 << 30        # variant 2 reads fields through buffer descriptors
+
+
+def plan_fits(variant: int, shape, fp32: bool) -> bool:
+    """Variant 2 takes fields of at most 1 GiB (the library rejects larger)."""
+    nx, ny, nz = shape
+    return variant != 2 or nx * ny * nz * (4 if fp32 else 8) <= BUF_MAX
 
 KINDS = {
     "p": ("tse_fast_p_sweeps", PLANS64, PLANS32),
@@ -55,6 +62,8 @@ def tune(hip, fp32: bool, sweeps: int = 12, sor: float = 1.9, zchunks=ZCHUNKS,
     stream = torch.cuda.current_stream()
     nz = hip.shape[2]
     for ns, by, ry, var in (p32 if fp32 else p64):
+        if not plan_fits(var, hip.shape, fp32):
+            continue
         for zc in zchunks:
             if zc and zc >= nz:
                 continue

@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .autotune import PLANS32, PLANS64, TR_PLANS32, TR_PLANS64
+from .autotune import PLANS32, PLANS64, TR_PLANS32, TR_PLANS64, plan_fits
 from .batch import KIND_P, KIND_TR, Batch
 from .scenario import DeviceScenario, ScenarioSpec
 
@@ -110,6 +110,8 @@ class Ensemble:
                      self.sor_tr)):
                 best = None
                 for ns, by, ry, var in plans:
+                    if not all(plan_fits(var, s.hip.shape, fp32) for s in self.scn):
+                        continue
                     for zc in zchunks:
                         plan = (ns, by, ry, zc, var)
                         for s in self.scn:
