@@ -316,6 +316,31 @@ int tse_linear_eos(tse_ctx *ctx, const double *C, const double *T,
  * rate of C through the two x faces (upwind, face area dy*dz)]. */
 int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
                   const double *phi, const double *qx, double *out3);
+/* peclet_C (tse.py:333-360): for cells with BC == 0,
+ * PeC[f*3 + a] = |q_a| / max_s D_a[f*nspec + s] * d_a (NaN where that
+ * maximum is 0); other cells keep their PeC.  PeC and D_fx/y/z are device
+ * arrays in the host layout (C order (nx, ny, nz, 3) and (nx, ny, nz,
+ * nspec)), q and BC split-layout fields.  out3 = max over all cells of
+ * PeC[..., a], NaN if any element is NaN (numpy.max). */
+int tse_peclet_c(tse_ctx *ctx, const double *qx, const double *qy,
+                 const double *qz, const double *D_fx, const double *D_fy,
+                 const double *D_fz, int32_t nspec, const double *BC,
+                 double *PeC, double *out3);
+/* peclet_T (tse.py:364-383): for cells with BT == 0,
+ * PeT[f*3 + a] = |q_a| rho_f cp_f / ((1-phi) l_r + phi l_f) * d_a; other
+ * cells keep their PeT.  Fields split-layout, PeT host layout; out3 as for
+ * tse_peclet_c. */
+int tse_peclet_t(tse_ctx *ctx, const double *qx, const double *qy,
+                 const double *qz, const double *rho_f, const double *cp_f,
+                 const double *phi, const double *l_r, const double *l_f,
+                 const double *BT, double *PeT, double *out3);
+/* The three maxima of the SIA driver's CFL limit (tse.py:1241-1246),
+ * dt_CFL = 0.9 / (out3[0] + out3[1] + out3[2]): max |qx|/phi/dx,
+ * |qy|/phi/dy, |qz|/phi/dz with the reference's reshape of the first two
+ * arrays to (ny, nz, nx) and (nx, nz, ny) before the division, i.e. C-order
+ * element f divided by dx[f % nx] and dy[f % ny].  NaN-propagating. */
+int tse_cfl_maxima(tse_ctx *ctx, const double *qx, const double *qy,
+                   const double *qz, const double *phi, double *out3);
 
 /* ---- graphs of a context's device work ------------------------------------ */
 /* Captures the work that calls issue on the context's stream between

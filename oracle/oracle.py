@@ -254,6 +254,50 @@ class Oracle:
         qx[...], qy[...], qz[...] = outs
         return qx, qy, qz
 
+    # diagnostics of the SIA driver (numpy, as evaluated per cell by the
+    # reference; maxima with numpy.max's NaN propagation) ------------------
+    def peclet_C(self, PeC, qx, qy, qz, D_fx, D_fy, D_fz):
+        """peclet_C (tse.py:333-360): PeC updated in place where BC == 0
+        (BC frozen at the first call, a module global of the kernel); returns
+        the maxima of PeC[..., 0..2]."""
+        (BC,) = self._freeze("BC")
+        act = BC == 0
+        out = []
+        for a, q, D, d in ((0, qx, D_fx, self.dx[:, None, None]),
+                           (1, qy, D_fy, self.dy[None, :, None]),
+                           (2, qz, D_fz, self.dz[None, None, :])):
+            D = np.asarray(D, dtype=np.float64)
+            mD = np.max(D, axis=-1) if D.ndim == 4 else D
+            with np.errstate(divide="ignore", invalid="ignore"):
+                val = np.where(mD != 0, np.abs(q) / mD * d, np.nan)
+            PeC[..., a] = np.where(act, val, PeC[..., a])
+            out.append(float(np.max(PeC[..., a])))
+        return tuple(out)
+
+    def peclet_T(self, PeT, qx, qy, qz, rho_f, cp_f, phi, l_r, l_f):
+        """peclet_T (tse.py:364-383): PeT updated in place where BT == 0
+        (BT frozen at the first call); returns the maxima of PeT[..., 0..2]."""
+        (BT,) = self._freeze("BT")
+        act = BT == 0
+        den = (1 - phi) * l_r + phi * l_f
+        out = []
+        for a, q, d in ((0, qx, self.dx[:, None, None]), (1, qy, self.dy[None, :, None]),
+                        (2, qz, self.dz[None, None, :])):
+            with np.errstate(divide="ignore", invalid="ignore"):
+                val = np.abs(q) * rho_f * cp_f / den * d
+            PeT[..., a] = np.where(act, val, PeT[..., a])
+            out.append(float(np.max(PeT[..., a])))
+        return tuple(out)
+
+    def cfl_maxima(self, qx, qy, qz, phi):
+        """The three maxima of the CFL limit, tse.py:1241-1246 (with its
+        reshapes); dt_CFL = 0.9 / sum."""
+        nx, ny, nz = self.shape
+        with np.errstate(divide="ignore", invalid="ignore"):
+            return (float(np.max(np.reshape(np.abs(qx) / phi, (ny, nz, nx)) / self.dx)),
+                    float(np.max(np.reshape(np.abs(qy) / phi, (nx, nz, ny)) / self.dy)),
+                    float(np.max(np.abs(qz) / phi / self.dz)))
+
     def flow_error(self, P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f, dt_P):
         ins = [_c(a) for a in (P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f)]
         e = np.empty(self.shape)

@@ -4,7 +4,8 @@ Hand-written HIP kernels behind a C-ABI (include/tse_hip.h), called from
 Python through ctypes, with the reference's solver call signatures:
 
 * ``HipSolvers``   -- compute_P / compute_q / flow_error / compute_C /
-  compute_T / l2_norm / smolarkiewicz for one grid (solvers.py)
+  compute_T / l2_norm / smolarkiewicz / peclet_C / peclet_T for one
+  grid (solvers.py)
 * ``Simulation``   -- the time-step / SIA driver that calls them
   (driver.py), ``make_hip_backend`` builds its GPU backend
 * ``load_config``  -- imports a TRANSPORTSE config module (config_loader.py)
@@ -61,6 +62,14 @@ def l2_norm(*a, **k):
 
 def smolarkiewicz(*a, **k):
     return _solvers().smolarkiewicz(*a, **k)
+
+
+def peclet_C(*a, **k):
+    return _solvers().peclet_C(*a, **k)
+
+
+def peclet_T(*a, **k):
+    return _solvers().peclet_T(*a, **k)
 
 
 def __getattr__(name):

@@ -143,6 +143,11 @@ SIGNATURES = {
     "tse_fast_mark_loaded": (_I, [_V, ctypes.c_int32]),
     "tse_fast_current": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_mass_sums": (_I, [_V, _dp, _dp, _dp, _dp, _DPH]),
+    "tse_peclet_c": (_I, [_V, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32, _dp, _dp,
+                          _DPH]),
+    "tse_peclet_t": (_I, [_V, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                          _DPH]),
+    "tse_cfl_maxima": (_I, [_V, _dp, _dp, _dp, _dp, _DPH]),
 }
 
 _lib = None

@@ -96,6 +96,8 @@ class Simulation:
             self.update_fluid_properties()
         # the backend snapshots frozen globals at first call from `st`
         self.solvers = backend_factory(self)
+        if hasattr(self.solvers, "pe_writeback"):
+            self.solvers.pe_writeback = False
         self.trajectory = []
         self.writer = PickleWriter()
 
@@ -183,35 +185,9 @@ class Simulation:
         return True
 
     # ---- diagnostics (tse.py:331-396) ---------------------------------------------
-    def peclet_C(self):
-        st = self.st
-        act = st.BC == 0
-        out = []
-        for a, D, d in ((0, st.D_fx, st.dx[:, None, None]),
-                        (1, st.D_fy, st.dy[None, :, None]),
-                        (2, st.D_fz, st.dz[None, None, :])):
-            q = (st.qx, st.qy, st.qz)[a]
-            mD = numpy.max(D, axis=-1)
-            with numpy.errstate(divide="ignore", invalid="ignore"):
-                val = numpy.where(mD != 0, numpy.abs(q) / mD * d, numpy.nan)
-            st.PeC[..., a] = numpy.where(act, val, st.PeC[..., a])
-            out.append(numpy.max(st.PeC[..., a]))
-        return tuple(out)
-
-    def peclet_T(self):
-        st = self.st
-        act = st.BT == 0
-        out = []
-        den = (1 - st.phi) * st.l_r + st.phi * st.l_f
-        for a, d in ((0, st.dx[:, None, None]), (1, st.dy[None, :, None]),
-                     (2, st.dz[None, None, :])):
-            q = (st.qx, st.qy, st.qz)[a]
-            with numpy.errstate(divide="ignore", invalid="ignore"):
-                val = numpy.abs(q) * st.rho_f * st.cp_f / den * d
-            st.PeT[..., a] = numpy.where(act, val, st.PeT[..., a])
-            out.append(numpy.max(st.PeT[..., a]))
-        return tuple(out)
-
+    # peclet_C / peclet_T and the CFL maxima run on the backend (the device
+    # for the GPU backend, which keeps PeC / PeT there: the driver only
+    # reads their maxima)
     def explicit_C(self):
         st = self.st
         inv = (1 / st.dx[:, None, None] ** 2 + 1 / st.dy[None, :, None] ** 2) \
@@ -317,19 +293,21 @@ class Simulation:
                                                  st.rho_f, self.dt)
                         solve_time += time.perf_counter() - t0
                         if self.t_sim > 0:
-                            dt_CFL = 0.9 / (
-                                numpy.max(numpy.reshape(numpy.abs(st.qx) / st.phi,
-                                                        (st.ny, st.nz, st.nx)) / st.dx)
-                                + numpy.max(numpy.reshape(numpy.abs(st.qy) / st.phi,
-                                                          (st.nx, st.nz, st.ny)) / st.dy)
-                                + numpy.max(numpy.abs(st.qz) / st.phi / st.dz))
+                            # tse.py:1241-1246 (its reshapes included)
+                            m = sv.cfl_maxima(st.qx, st.qy, st.qz, st.phi)
+                            with numpy.errstate(divide="ignore", invalid="ignore"):
+                                dt_CFL = 0.9 / (numpy.float64(m[0]) + m[1] + m[2])
                         else:
                             dt_CFL = 1e10
                         dt_max_C = dt_max_T = dt_max_P = dt_max_abs
                         if c.HEATTRANS:
-                            Pex_T, Pey_T, Pez_T = self.peclet_T()
+                            Pex_T, Pey_T, Pez_T = sv.peclet_T(
+                                st.PeT, st.qx, st.qy, st.qz, st.rho_f, st.cp_f,
+                                st.phi, st.l_r, st.l_f)
                         if c.SPECTRANS:
-                            Pex_C, Pey_C, Pez_C = self.peclet_C()
+                            Pex_C, Pey_C, Pez_C = sv.peclet_C(
+                                st.PeC, st.qx, st.qy, st.qz, st.D_fx, st.D_fy,
+                                st.D_fz)
                         if c.OMEGA_C == 0:
                             dt_max_C = dt_NeC
                         if c.SIA:

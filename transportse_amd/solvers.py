@@ -66,6 +66,11 @@ class HipSolvers:
         self._gen = 0
         self._cache = {}   # role -> [key, gen, tensor, host_ref]
         self._frozen = {}  # name -> tensor
+        self._pe = {}      # "PeC"/"PeT" -> device array in the host layout
+        # peclet_C / peclet_T write PeC / PeT back into the caller's arrays;
+        # a driver that only reads the returned maxima turns this off and the
+        # device copy stays the only one
+        self.pe_writeback = True
         self.last = None
         self.stats = {"uploads": 0, "downloads": 0}
 
@@ -92,6 +97,34 @@ class HipSolvers:
         t = self.hip.upload(a, out=out)
         self.stats["uploads"] += 1
         self._cache[role] = [k, self._gen, t, a]
+        return t
+
+    def _raw(self, role: str, arr):
+        """Device copy of a host array in its own C-order layout, cached
+        like ``_dev``."""
+        import torch
+        a = np.asarray(arr)
+        k = _key(a)
+        ent = self._cache.get(role)
+        if ent is not None and ent[0] == k and ent[1] == self._gen:
+            return ent[2]
+        t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(
+            self.hip.device)
+        self.stats["uploads"] += 1
+        self._cache[role] = [k, self._gen, t, a]
+        return t
+
+    def _pe_buffer(self, name: str, host):
+        """The Peclet array's device copy: from the host array at the first
+        call, device-owned afterwards (cells with B != 0 keep their values)."""
+        import torch
+        t = self._pe.get(name)
+        if t is None:
+            h = np.asarray(host, dtype=np.float64)
+            if h.shape != self.shape + (3,):
+                raise ValueError(f"{name} must have shape {self.shape + (3,)}")
+            t = torch.from_numpy(np.ascontiguousarray(h)).to(self.hip.device)
+            self._pe[name] = t
         return t
 
     def _mark(self, role: str, arr, tensor):
@@ -262,6 +295,64 @@ class HipSolvers:
             self._mark(n, a, d[n])
         self.stats["downloads"] += 3
         return vx, vy, vz
+
+    # diagnostics of the SIA driver --------------------------------------------------
+    def peclet_C(self, PeC, qx, qy, qz, D_fx, D_fy, D_fz):
+        """peclet_C (tse.py:333-360) on the device; returns the maxima of
+        PeC[..., 0..2].  BC is read from ``frozen_source`` at the first call
+        (a module global of the reference kernel)."""
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (("qx", qx), ("qy", qy), ("qz", qz))}
+        Ds = [np.asarray(D) for D in (D_fx, D_fy, D_fz)]
+        nspec = Ds[0].shape[-1] if Ds[0].ndim == 4 else 1
+        if any(D.shape != self.shape + ((nspec,) if Ds[0].ndim == 4 else ())
+               for D in Ds):
+            raise ValueError("D_fx, D_fy, D_fz must share one (nx, ny, nz[, S]) shape")
+        Dd = [self._raw(f"pe_{n}", D) for n, D in zip(("D_fx", "D_fy", "D_fz"), Ds)]
+        bc = self._frozen_field("BC")
+        pe = self._pe_buffer("PeC", PeC)
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.tse_peclet_c(
+            self.hip.ctx, d["qx"].data_ptr(), d["qy"].data_ptr(), d["qz"].data_ptr(),
+            *[t.data_ptr() for t in Dd], int(nspec), bc.data_ptr(), pe.data_ptr(),
+            out), "tse_peclet_c")
+        if self.pe_writeback:
+            PeC[...] = pe.cpu().numpy().reshape(PeC.shape)
+            self.stats["downloads"] += 1
+        return tuple(out)
+
+    def peclet_T(self, PeT, qx, qy, qz, rho_f, cp_f, phi, l_r, l_f):
+        """peclet_T (tse.py:364-383) on the device; returns the maxima of
+        PeT[..., 0..2].  BT is read from ``frozen_source`` at the first call;
+        l_r is the live argument."""
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (
+            ("qx", qx), ("qy", qy), ("qz", qz), ("rho_f", rho_f), ("cp_f", cp_f),
+            ("phi", phi), ("l_r_live", l_r), ("l_f", l_f))}
+        bt = self._frozen_field("BT")
+        pe = self._pe_buffer("PeT", PeT)
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.tse_peclet_t(
+            self.hip.ctx, *[d[n].data_ptr() for n in
+                            ("qx", "qy", "qz", "rho_f", "cp_f", "phi", "l_r_live",
+                             "l_f")], bt.data_ptr(), pe.data_ptr(), out),
+            "tse_peclet_t")
+        if self.pe_writeback:
+            PeT[...] = pe.cpu().numpy().reshape(PeT.shape)
+            self.stats["downloads"] += 1
+        return tuple(out)
+
+    def cfl_maxima(self, qx, qy, qz, phi):
+        """The three maxima of the SIA driver's CFL limit (tse.py:1241-1246,
+        with its reshapes); dt_CFL = 0.9 / their sum."""
+        self._begin()
+        d = {n: self._dev(n, a) for n, a in (
+            ("qx", qx), ("qy", qy), ("qz", qz), ("phi", phi))}
+        out = (ctypes.c_double * 3)()
+        _lib.check(self.lib.tse_cfl_maxima(
+            self.hip.ctx, *[d[n].data_ptr() for n in ("qx", "qy", "qz", "phi")], out),
+            "tse_cfl_maxima")
+        return tuple(out)
 
     def l2_norm(self, New, Old, Process=""):
         """l2_norm (tse.py:273-281) on host arrays, Numba's sequential sums."""
