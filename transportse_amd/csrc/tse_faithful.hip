@@ -270,6 +270,32 @@ p_assemble_kernel(Geo G, const double *__restrict__ Pn,
     }
 }
 
+// The six other-colour neighbours of a cell (0 outside the grid).  Every
+// load is issued, at a slot clamped into the array, and the outside ones are
+// replaced by 0 afterwards, so a cell's loads go out together instead of one
+// branch at a time.
+struct Nb6 {
+    double xp, xm, yp, ym, zp, zm;
+};
+
+__device__ inline Nb6 neighbours6(const Geo &G, const Cell &q,
+                                  const double *__restrict__ Xo) {
+    const long rowb = (long)q.row * G.nxh + q.ih;
+    const long last = (long)G.S - 1, plane = (long)G.ny * G.nxh;
+    auto ld = [&](long o) { return Xo[o < 0 ? 0 : (o > last ? last : o)]; };
+    const double xp = ld(rowb + q.s), xm = ld(rowb + q.s - 1);
+    const double yp = ld(rowb + G.nxh), ym = ld(rowb - G.nxh);
+    const double zp = ld(rowb + plane), zm = ld(rowb - plane);
+    Nb6 n;
+    n.xp = q.i != G.nx - 1 ? xp : 0.0;
+    n.xm = q.i != 0 ? xm : 0.0;
+    n.yp = q.j != G.ny - 1 ? yp : 0.0;
+    n.ym = q.j != 0 ? ym : 0.0;
+    n.zp = q.k != G.nz - 1 ? zp : 0.0;
+    n.zm = q.k != 0 ? zm : 0.0;
+    return n;
+}
+
 // One colour half-sweep of compute_P (tse.py:705-716), update expression of
 // tse.py:692-695 evaluated left to right.  If the iteration checks the norm,
 // per-block sums of (P_new-P_old)^2 and P_old^2 go to partials.
@@ -287,45 +313,38 @@ p_sweep_kernel(Geo G, int colour, double *__restrict__ P,
     const long base = (long)colour * G.S;
     const double *__restrict__ Po = P + (long)(1 - colour) * G.S;
     double sd = 0.0, so = 0.0;
+    // all operands of a cell are loaded before any test; padding slots
+    // (mask bit 0 clear) and Dirichlet cells (bit 1 clear) discard the result
     for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < G.S;
          r += gridDim.x * blockDim.x) {
         const long t = base + r;
         const uint8_t m = mask[t];
-        if (!(m & 1)) continue;
         const double Pc = P[t];
-        double Pnew = Pc;
-        if (m & 2) {
-            const Cell q = decode(G, colour, r);
-            const int rowb = q.row * G.nxh + q.ih;
-            const double pxp = (q.i != G.nx - 1) ? Po[rowb + q.s] : 0.0;
-            const double pxm = (q.i != 0) ? Po[rowb + q.s - 1] : 0.0;
-            const double pyp = (q.j != G.ny - 1) ? Po[rowb + G.nxh] : 0.0;
-            const double pym = (q.j != 0) ? Po[rowb - G.nxh] : 0.0;
-            const double pzp =
-                (q.k != G.nz - 1) ? Po[rowb + (long)G.ny * G.nxh] : 0.0;
-            const double pzm = (q.k != 0) ? Po[rowb - (long)G.ny * G.nxh] : 0.0;
-            const double *rr = rec + t;
-            const double Anew = rr[0 * F] * pxp;
-            const double Cnew = rr[1 * F] * pxm;
-            const double Enew = rr[2 * F] * pyp;
-            const double Gnew = rr[3 * F] * pym;
-            const double Inew = rr[4 * F] * pzp;
-            const double Knew = rr[5 * F] * pzm;
-            double s = Anew + rr[6 * F];
-            s = s + Cnew;
-            s = s - rr[7 * F];
-            s = s + Enew;
-            s = s + Gnew;
-            s = s + Inew;
-            s = s + rr[8 * F];
-            s = s + Knew;
-            s = s - rr[9 * F];
-            s = s + rr[10 * F];
-            s = s + rr[11 * F];
-            Pnew = (1 - sor) * Pc + sor * s / rr[12 * F];
-            P[t] = Pnew;
-        }
-        if (check) {
+        const Cell q = decode(G, colour, r);
+        const Nb6 n = neighbours6(G, q, Po);
+        const double *rr = rec + t;
+        const double Anew = rr[0 * F] * n.xp;
+        const double Cnew = rr[1 * F] * n.xm;
+        const double Enew = rr[2 * F] * n.yp;
+        const double Gnew = rr[3 * F] * n.ym;
+        const double Inew = rr[4 * F] * n.zp;
+        const double Knew = rr[5 * F] * n.zm;
+        double s = Anew + rr[6 * F];
+        s = s + Cnew;
+        s = s - rr[7 * F];
+        s = s + Enew;
+        s = s + Gnew;
+        s = s + Inew;
+        s = s + rr[8 * F];
+        s = s + Knew;
+        s = s - rr[9 * F];
+        s = s + rr[10 * F];
+        s = s + rr[11 * F];
+        const double Pupd = (1 - sor) * Pc + sor * s / rr[12 * F];
+        const bool upd = (m & 3) == 3;
+        const double Pnew = upd ? Pupd : Pc;
+        if (upd) P[t] = Pnew;
+        if (check && (m & 1)) {
             const double d = Pnew - Pc;
             sd += d * d;
             so += Pc * Pc;
@@ -805,34 +824,25 @@ tr_sweep_kernel(Geo G, int colour, double *__restrict__ X,
          r += gridDim.x * blockDim.x) {
         const long t = base + r;
         const uint8_t m = mask[t];
-        if (!(m & 1)) continue;
         const double Xc = X[t];
-        double Xnew = Xc;
-        if (m & 2) {
-            const Cell q = decode(G, colour, r);
-            const int rowb = q.row * G.nxh + q.ih;
-            const double xp = (q.i != G.nx - 1) ? Xo[rowb + q.s] : 0.0;
-            const double xm = (q.i != 0) ? Xo[rowb + q.s - 1] : 0.0;
-            const double yp = (q.j != G.ny - 1) ? Xo[rowb + G.nxh] : 0.0;
-            const double ym = (q.j != 0) ? Xo[rowb - G.nxh] : 0.0;
-            const double zp =
-                (q.k != G.nz - 1) ? Xo[rowb + (long)G.ny * G.nxh] : 0.0;
-            const double zm = (q.k != 0) ? Xo[rowb - (long)G.ny * G.nxh] : 0.0;
-            const double *rr = rec + t;
-            double s = rr[0 * F] * xp;
-            s = s + rr[1 * F] * xm;
-            s = s + rr[2 * F] * yp;
-            s = s + rr[3 * F] * ym;
-            s = s + rr[4 * F] * zp;
-            s = s + rr[5 * F] * zm;
-            double u = omega * s + rr[6 * F];
-            u = u + rr[7 * F];
-            u = u + rr[8 * F];
-            u = u - rr[9 * F];
-            Xnew = (1 - sor) * Xc + sor * u / rr[10 * F];
-            X[t] = Xnew;
-        }
-        if (check) {
+        const Cell q = decode(G, colour, r);
+        const Nb6 n = neighbours6(G, q, Xo);
+        const double *rr = rec + t;
+        double s = rr[0 * F] * n.xp;
+        s = s + rr[1 * F] * n.xm;
+        s = s + rr[2 * F] * n.yp;
+        s = s + rr[3 * F] * n.ym;
+        s = s + rr[4 * F] * n.zp;
+        s = s + rr[5 * F] * n.zm;
+        double u = omega * s + rr[6 * F];
+        u = u + rr[7 * F];
+        u = u + rr[8 * F];
+        u = u - rr[9 * F];
+        const double Xupd = (1 - sor) * Xc + sor * u / rr[10 * F];
+        const bool upd = (m & 3) == 3;
+        const double Xnew = upd ? Xupd : Xc;
+        if (upd) X[t] = Xnew;
+        if (check && (m & 1)) {
             const double d = Xnew - Xc;
             sd += d * d;
             so += Xc * Xc;
