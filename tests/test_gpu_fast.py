@@ -182,3 +182,37 @@ def test_unrolled_ring_variant_equals_lane_exchange_variant(ns, by, ry, zchunk):
           "sweeps", sol[0][1], sol[1][1], flush=True)
     assert np.array_equal(sw[0], sw[1])
     assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
+
+
+DEGENERATE = [(51, 51, 1), (1, 1, 145), (200, 1, 60), (1, 20, 30), (3, 2, 1), (64, 33, 2)]
+
+
+@pytest.mark.parametrize("shape", DEGENERATE)
+def test_fused_variants_on_unit_dimension_grids(shape):
+    """Grids with unit extents or fewer planes than the halo: every sweep
+    variant matches the oracle's half-sweeps, and variant 2 (buffer loads
+    that read 0 outside the grid) equals variant 1 bit for bit."""
+    from oracle import Oracle
+    from transportse_amd import _lib
+    f = Fields(*shape, seed=26, dirichlet_p=shape[0] > 2)
+    dt, sor, n = 3600.0, 1.7, 4
+    o = Oracle(*f.grid_args(), frozen_source=types.SimpleNamespace(c_r=f.c_r))
+    Po = f.P.copy()
+    for _ in range(n):
+        for colour in (0, 1):
+            o.p_half_sweep(Po, f.P, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky,
+                           f.Kz, f.BP, f.WP, dt, sor, colour)
+    change = np.max(np.abs(Po - f.P))
+    out = {}
+    for pl in ((2, 16, 1, 0, 0), (2, 16, 1, 0, 1), (2, 16, 1, 0, 2), (1, 24, 2, 7, 1),
+               (1, 24, 2, 7, 2)):
+        hip, d = _setup(f, dt)
+        plan = _lib.SweepPlan(*pl)
+        _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n, sor, 0),
+                   "tse_fast_p_sweeps")
+        out[pl] = _result(hip, d)
+        err = np.max(np.abs(out[pl] - Po))
+        print("unit-dimension", shape, pl, "change", change, "err", err, flush=True)
+        assert err <= 1e-10 * change + 1e-13 * np.max(np.abs(Po))
+    assert np.array_equal(out[(2, 16, 1, 0, 1)], out[(2, 16, 1, 0, 2)])
+    assert np.array_equal(out[(1, 24, 2, 7, 1)], out[(1, 24, 2, 7, 2)])
