@@ -197,3 +197,21 @@ def test_transport_variant2_equals_variant0(form, plan):
           np.array_equal(out[0][0], out[1][0]), "sweeps", out[0][1], out[1][1],
           flush=True)
     assert out[0][1] == out[1][1] and np.array_equal(out[0][0], out[1][0])
+
+
+@pytest.mark.parametrize("shape", [(51, 51, 1), (1, 20, 30), (64, 33, 2), (3, 2, 1),
+                                   (200, 1, 9)])
+def test_transport_variant2_on_unit_dimension_grids(shape):
+    """Tiles whose edges lie on the grid boundary (variant 2 drops their
+    halo there): the same solve as variant 0, bit for bit."""
+    f = Fields(*shape, seed=32)
+    out = []
+    for var in (0, 2):
+        ft = _fast_c(f, 43200.0, 1.0, True)
+        st = ft.solve("plain64", tol=1e-12, sor=1.1, max_sweeps=4000, inner=4,
+                      plan=(2, 16, 1, 0, var))
+        out.append((ft.result(), st.sweeps, st.converged))
+    print("transport unit-dimension", shape, "sweeps", out[0][1], out[1][1],
+          "identical", np.array_equal(out[0][0], out[1][0]), flush=True)
+    assert out[0][1] == out[1][1] and out[0][2] == out[1][2]
+    assert np.array_equal(out[0][0], out[1][0])

@@ -318,7 +318,8 @@ int grow_slots(tse_batch *b, int need) {
 }
 
 ZGeo member_zgeo(const tse_batch *b, const tse_ctx *c) {
-    return make_zgeo(c->geo, b->plan.ns, b->plan.by, b->plan.zchunk);
+    return make_zgeo(c->geo, b->plan.ns, b->plan.by, b->plan.zchunk,
+                     b->plan.variant == 2);
 }
 
 // Rebuild work-item lists, member table and the persistent grid size.

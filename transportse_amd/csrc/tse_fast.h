@@ -15,7 +15,57 @@ struct ZGeo {
     int tiles_x, tiles_y, zchunk, nzc;
     int ntiles, per_xcd;
     FastDiv div_tx, div_ty;
+    int edge;   // boundary-aware tiling (span_of), used by the variant-2 kernels
 };
+
+// Boundary-aware tiling of an extent n by tiles W cells wide with a halo of
+// R on interior edges: a tile edge on the grid boundary needs no halo (there
+// is no neighbour to lag behind).  One tile if n <= W; else the first tile
+// starts at 0 and owns [0, W-R), middle tiles own W-2R cells each behind an
+// R halo, and the last tile ends at n and owns the rest (at most W-R cells,
+// so its left halo is at least R).
+__host__ __device__ inline int span_count(int n, int W, int R) {
+    if (n <= W) return 1;
+    const int T = W - 2 * R, rest = n - 2 * (W - R);
+    return 2 + (rest > 0 ? (rest + T - 1) / T : 0);
+}
+
+struct Span {
+    int x0, lo, hi;      // first cell of the tile; owned cells [lo, hi)
+    bool lfree, hfree;   // tile edge on the grid boundary
+};
+
+__device__ inline Span span_of(int t, int tiles, int n, int W, int R) {
+    Span s;
+    if (tiles == 1) {
+        s.x0 = 0;
+        s.lo = 0;
+        s.hi = n;
+        s.lfree = s.hfree = true;
+        return s;
+    }
+    const int T = W - 2 * R;
+    s.lo = t == 0 ? 0 : (W - R) + (t - 1) * T;
+    s.hi = t == tiles - 1 ? n : (W - R) + t * T;
+    s.x0 = t == 0 ? 0 : (t == tiles - 1 ? n - W : s.lo - R);
+    s.lfree = t == 0;
+    s.hfree = t == tiles - 1;
+    return s;
+}
+
+// Span of tile t: boundary-aware (edge) or the plain tiling, where every
+// tile carries an R halo on both sides and owns W-2R cells.
+__device__ inline Span tile_span(bool edge, int t, int tiles, int n, int W, int R) {
+    if (edge) return span_of(t, tiles, n, W, R);
+    Span s;
+    const int T = W - 2 * R;
+    s.lo = t * T;
+    s.hi = min(s.lo + T, n);
+    s.x0 = s.lo - R;
+    s.lfree = s.hfree = false;
+    return s;
+}
+constexpr int kFar = 1 << 20;   // distance to a tile edge on the grid boundary
 
 // Blocks are dispatched round-robin over the 8 XCDs; map block b to tile
 // (b % 8) * per_xcd + b / 8 so each XCD (own L2) works on a contiguous band of
@@ -404,15 +454,19 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     constexpr int TY = BY - 2 * R;
     constexpr int L = R + 2;
     static_assert(TY >= 1, "tile too small");
-    __shared__ T sP[L][BY][BX];
-    __shared__ T sWy[L][BY][BX];
+    // LDS rows are shifted by one: rows 0 and BY+1 are zero guards for the
+    // y neighbours of boundary rows in tiles whose edge is the grid boundary
+    __shared__ T sP[L][BY + 2][BX];
+    __shared__ T sWy[L][BY + 2][BX];
 
     const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
     const int tx = tile - zc * g.tiles_x;
     const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
     const int ty = zc - zc2 * g.tiles_y;
     const int lx = threadIdx.x;
-    const int i = tx * TX - R + lx;
+    const Span sx = tile_span(g.edge, tx, g.tiles_x, g.nx, BX, R);
+    const Span sy = tile_span(g.edge, ty, g.tiles_y, g.ny, BY, R);
+    const int i = sx.x0 + lx;
     const int k0 = zc2 * g.zchunk;
     const int k1 = min(k0 + g.zchunk, g.nz);
     const int kl0 = max(k0 - R, 0);
@@ -425,7 +479,7 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     const auto rWz = buf_rsrc(Wz, fbytes), rI = buf_rsrc(invD, fbytes);
     const auto rB = buf_rsrc(rhs, fbytes);
     const auto rOut = buf_rsrc(Pout, fbytes);
-    const int dxe = min(lx, BX - 1 - lx);
+    const int dxe = min(sx.lfree ? kFar : lx, sx.hfree ? kFar : BX - 1 - lx);
 
     int ly[RY];
     bool in_dom[RY], cm[RY], dm[RY][R];
@@ -436,15 +490,18 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
 #pragma unroll
     for (int q = 0; q < RY; ++q) {
         ly[q] = threadIdx.y + q * BYT;
-        const int j = ty * TY - R + ly[q];
+        const int j = sy.x0 + ly[q];
         in_dom[q] = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-        const int d = min(dxe, min(ly[q], BY - 1 - ly[q]));
+        const int d = min(dxe, min(sy.lfree ? kFar : ly[q],
+                                   sy.hfree ? kFar : BY - 1 - ly[q]));
 #pragma unroll
         for (int h = 0; h < R; ++h) dm[q][h] = in_dom[q] && d >= h + 1;
         // colour test of the steps with (k - kl0) even
         cm[q] = ((i + j + kl0 - 1) & 1) == 0;
         ocol[q] = in_dom[q] ? (uint32_t)(i + g.nx * j) * (uint32_t)sizeof(T) : kBufMax;
-        oout[q] = dm[q][R - 1] ? ocol[q] : kBufMax;   // tile interior only
+        // the cells this tile owns (valid after all R phases)
+        const bool own = in_dom[q] && i >= sx.lo && i < sx.hi && j >= sy.lo && j < sy.hi;
+        oout[q] = own ? ocol[q] : kBufMax;
 #pragma unroll
         for (int r = 1; r < L; ++r)
             p[q][r] = rwz[q][r] = rinv[q][r] = rb[q][r] = rwx[q][r] = rwy[q][r] = T(0);
@@ -458,6 +515,12 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
         p[q][0] = rwz[q][0] = T(0);
     }
 
+    if (threadIdx.y == 0)
+#pragma unroll
+        for (int u = 0; u < L; ++u) sP[u][0][lx] = sWy[u][0][lx] = T(0);
+    if (threadIdx.y == BYT - 1)
+#pragma unroll
+        for (int u = 0; u < L; ++u) sP[u][BY + 1][lx] = sWy[u][BY + 1][lx] = T(0);
     for (int kb = kl0; kb <= klast; kb += L) {
 #pragma unroll
         for (int u = 0; u < L; ++u) {
@@ -468,8 +531,8 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
             for (int q = 0; q < RY; ++q) {
                 p[q][u] = np[q];
                 rwz[q][u] = nwz[q];
-                sP[u][ly[q]][lx] = np[q];
-                sWy[u][ly[q]][lx] = rwy[q][u];
+                sP[u][ly[q] + 1][lx] = np[q];
+                sWy[u][ly[q] + 1][lx] = rwy[q][u];
             }
             // plane k+1 (reads 0 past kl1 and outside the grid)
             const uint32_t ok = k + 1 <= kl1 ? (uint32_t)(k + 1) * pbytes : 0x80000000u;
@@ -496,7 +559,7 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
                     const T wxl = from_left(rwx[q][sm]);
                     const bool col_ok = (u & 1) ? !cm[q] : cm[q];
                     if (dm[q][h] && col_ok && rinv[q][sm] > T(0)) {
-                        const int y = ly[q];
+                        const int y = ly[q] + 1;   // LDS row
                         T acc = rb[q][sm];
                         acc += rwx[q][sm] * pr;
                         acc += wxl * pl;
@@ -669,17 +732,20 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     constexpr int TY = BY - 2 * R;
     constexpr int L = R + 2;
     static_assert(TY >= 1, "tile too small");
-    __shared__ T sP[L][BY][BX];
+    __shared__ T sP[L][BY + 2][BX];   // rows shifted by one, zero guards 0 and BY+1
 
     const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
     const int tx = tile - zc * g.tiles_x;
     const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
     const int ty = zc - zc2 * g.tiles_y;
     const int lx = threadIdx.x, ly = threadIdx.y;
-    const int i = tx * TX - R + lx;
-    const int j = ty * TY - R + ly;
+    const Span sx = tile_span(g.edge, tx, g.tiles_x, g.nx, BX, R);
+    const Span sy = tile_span(g.edge, ty, g.tiles_y, g.ny, BY, R);
+    const int i = sx.x0 + lx;
+    const int j = sy.x0 + ly;
     const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
-    const int d = min(min(lx, BX - 1 - lx), min(ly, BY - 1 - ly));
+    const int d = min(min(sx.lfree ? kFar : lx, sx.hfree ? kFar : BX - 1 - lx),
+                      min(sy.lfree ? kFar : ly, sy.hfree ? kFar : BY - 1 - ly));
     const int k0 = zc2 * g.zchunk;
     const int k1 = min(k0 + g.zchunk, g.nz);
     const int kl0 = max(k0 - R, 0);
@@ -698,7 +764,8 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     for (int h = 0; h < R; ++h) dm[h] = in_dom && d >= h + 1;
     const bool cm = ((i + j + kl0 - 1) & 1) == 0;   // colour test, (k-kl0) even
     const uint32_t ocol = in_dom ? (uint32_t)(i + g.nx * j) * (uint32_t)sizeof(T) : kBufMax;
-    const uint32_t oout = dm[R - 1] ? ocol : kBufMax;
+    const bool own = in_dom && i >= sx.lo && i < sx.hi && j >= sy.lo && j < sy.hi;
+    const uint32_t oout = own ? ocol : kBufMax;
 
     T p[L], rinv[L], rb[L], rc[6][L];
 #pragma unroll
@@ -717,6 +784,12 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
 #pragma unroll
         for (int f = 0; f < 6; ++f) rc[f][0] = buf_ld(rC[f], o, T());
     }
+    if (ly == 0)
+#pragma unroll
+        for (int u = 0; u < L; ++u) sP[u][0][lx] = T(0);
+    if (ly == BY - 1)
+#pragma unroll
+        for (int u = 0; u < L; ++u) sP[u][BY + 1][lx] = T(0);
     for (int kb = kl0; kb <= klast; kb += L) {
 #pragma unroll
         for (int u = 0; u < L; ++u) {
@@ -724,7 +797,7 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
             if (k > klast) break;
             const int un = (u + 1) % L;
             p[u] = np;
-            sP[u][ly][lx] = np;
+            sP[u][ly + 1][lx] = np;
             const uint32_t o =
                 ocol + (k + 1 <= kl1 ? (uint32_t)(k + 1) * pbytes : 0x80000000u);
             np = buf_ld(rP, o, T());
@@ -745,14 +818,14 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
                     T acc = rb[sm];
                     acc += rc[0][sm] * pr;
                     acc += rc[1][sm] * pl;
-                    acc += rc[2][sm] * sP[sm][ly + 1][lx];
-                    acc += rc[3][sm] * sP[sm][ly - 1][lx];
+                    acc += rc[2][sm] * sP[sm][ly + 2][lx];
+                    acc += rc[3][sm] * sP[sm][ly][lx];
                     acc += rc[4][sm] * p[sa];
                     acc += rc[5][sm] * p[sb];
                     const T pc = p[sm];
                     const T pn = pc + omega * (acc * rinv[sm] - pc);
                     p[sm] = pn;
-                    sP[sm][ly][lx] = pn;
+                    sP[sm][ly + 1][lx] = pn;
                 }
             }
             const int mo = k - R;
@@ -904,6 +977,6 @@ FastSys *system_of(tse_ctx *ctx, int kind);
 int ensure_sys(tse_ctx *ctx, FastSys *s);
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind);
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32);
-ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk);
+ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk, int edge);
 
 }  // namespace tse

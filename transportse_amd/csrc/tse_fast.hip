@@ -559,14 +559,15 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
     return false;
 }
 
-ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk) {
+ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk, int edge) {
     ZGeo z{};
     z.nx = G.nx;
     z.ny = G.ny;
     z.nz = G.nz;
+    z.edge = edge;
     const int R = 2 * ns, TX = 64 - 2 * R, TY = by - 2 * R;
-    z.tiles_x = (G.nx + TX - 1) / TX;
-    z.tiles_y = (G.ny + TY - 1) / TY;
+    z.tiles_x = edge ? span_count(G.nx, 64, R) : (G.nx + TX - 1) / TX;
+    z.tiles_y = edge ? span_count(G.ny, by, R) : (G.ny + TY - 1) / TY;
     z.zchunk = std::max(1, std::min(zchunk > 0 ? zchunk : G.nz, G.nz));
     z.nzc = (G.nz + z.zchunk - 1) / z.zchunk;
     z.ntiles = z.tiles_x * z.tiles_y * z.nzc;
@@ -661,7 +662,8 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
     if (!fast_plan_fits(*plan, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
-    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk);
+    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk,
+                              plan->variant == 2);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
     int e;  // FP32 timing path: correction sweeps on the last residual
     return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
@@ -680,7 +682,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
     if (!fast_plan_fits(pl, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
-    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk);
+    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant == 2);
     double rn, bn;
     int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
                   : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
