@@ -1,7 +1,7 @@
 """CPU oracle for the TRANSPORTSE SOR / transport hot path.
 
-TEST INFRASTRUCTURE ONLY. This module is the checker for the HIP path and the
-timed CPU baseline. Only tests/, ``__graft_entry__.smoke()`` and the
+TEST INFRASTRUCTURE ONLY. The parity tests compare the HIP path with this
+module, and it is the timed CPU baseline. Only tests/, ``__graft_entry__.smoke()`` and the
 ``cpu_baseline`` leg of ``bench.py`` may import it; the product package never
 does.
 

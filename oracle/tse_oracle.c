@@ -1,8 +1,8 @@
 /*
  * tse_oracle.c -- CPU restatement of TRANSPORTSE's SOR / transport hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker and the timed CPU baseline; the
- * product never links it).
+ * TEST INFRASTRUCTURE ONLY (what the parity tests compare with, and the
+ * timed CPU baseline; the product never links it).
  *
  * Each function restates the Numba kernel of tse.py cited above it, with the
  * same operand order, the same branch structure and the same control flow.

@@ -1,8 +1,8 @@
 /*
  * tse_oracle.h -- CPU restatement of TRANSPORTSE's SOR / transport hot path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP path
- * and the timed CPU baseline; nothing in the product (transportse_amd/)
+ * TEST INFRASTRUCTURE ONLY.  The parity tests compare the HIP path with this
+ * library, and it is the timed CPU baseline; nothing in the product (transportse_amd/)
  * may link, load or call it.  Only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py use it.
  *
