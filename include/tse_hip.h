@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TSE_ABI_VERSION 2
+#define TSE_ABI_VERSION 3
 
 enum tse_status {
     TSE_OK = 0,
@@ -208,9 +208,19 @@ typedef struct {
     tse_sweep_plan plan;
     int32_t inner;      /* sweeps per residual check / correction cycle     */
     int32_t max_sweeps; /* stop after this many sweeps                      */
+    int32_t cycle_mode; /* TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE (solve only) */
+    int32_t pad;
     double sor;         /* over-relaxation factor                           */
     double tol;         /* relative residual target                         */
 } tse_fast_params;
+/* Cycle lengths of tse_fast_*_solve.  FIXED: every cycle is `inner` sweeps.
+ * ADAPTIVE: the first cycle is `inner` sweeps; after each residual check
+ * the rate of the last cycle predicts the sweeps still needed to reach tol,
+ * and the next cycle is that many (plus 2 %), rounded up to a multiple of
+ * the plan's ns and kept within [inner, 8*inner]; the stop test is the same
+ * (relative residual <= tol), with fewer FP64 residual passes.  The batched
+ * solves use fixed cycles. */
+enum { TSE_CYCLE_FIXED = 0, TSE_CYCLE_ADAPTIVE = 1 };
 
 typedef struct {
     int32_t sweeps;     /* sweeps performed                                 */

@@ -216,3 +216,39 @@ def test_fused_variants_on_unit_dimension_grids(shape):
         assert err <= 1e-10 * change + 1e-13 * np.max(np.abs(Po))
     assert np.array_equal(out[(2, 16, 1, 0, 1)], out[(2, 16, 1, 0, 2)])
     assert np.array_equal(out[(1, 24, 2, 7, 1)], out[(1, 24, 2, 7, 2)])
+
+
+@pytest.mark.parametrize("form,tol", [(2, 1e-10), (1, 1e-11)])
+def test_adaptive_cycles_reach_the_same_tolerance(form, tol):
+    """TSE_CYCLE_ADAPTIVE: the same stop test with cycle lengths predicted
+    from the convergence rate -- converged to tol, fewer residual passes
+    than fixed cycles, and the solution within tol of the fixed-cycle one."""
+    from transportse_amd import _lib
+    f = Fields(60, 44, 30, seed=27)
+    out = {}
+    for mode in (_lib.CYCLE_FIXED, _lib.CYCLE_ADAPTIVE):
+        hip, d = _setup(f, 3600.0)
+        prm = _lib.FastParams(form=form, plan=_lib.SweepPlan(2, 16, 1, 0, 2), inner=10,
+                              max_sweeps=200000, cycle_mode=mode, sor=1.9, tol=tol)
+        res = _lib.FastResult()
+        _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                            ctypes.byref(res)), "solve")
+        assert res.converged and res.rel_residual <= tol
+        out[mode] = (_result(hip, d), res.sweeps, res.cycles)
+    (a, na, ca), (b, nb, cb) = out[_lib.CYCLE_FIXED], out[_lib.CYCLE_ADAPTIVE]
+    change = np.max(np.abs(a - f.P))
+    print("adaptive cycles: fixed", na, "sweeps", ca, "cycles; adaptive", nb, "sweeps",
+          cb, "cycles; diff/change", np.max(np.abs(a - b)) / change, flush=True)
+    assert cb < ca
+    assert nb <= na + 80 * 2
+    assert np.max(np.abs(a - b)) <= 1e-6 * change
+
+
+def test_bad_cycle_mode_rejected():
+    from transportse_amd import _lib
+    f = Fields(20, 12, 10, seed=28)
+    hip, d = _setup(f, 3600.0)
+    prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(2, 16, 1, 0, 2), inner=10,
+                          max_sweeps=100, cycle_mode=7, sor=1.9, tol=1e-10)
+    res = _lib.FastResult()
+    assert hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm), ctypes.byref(res)) < 0

@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtse_hip.so")
 
-ABI_VERSION = 2
+ABI_VERSION = 3
 
 FORM_FAITHFUL64 = 0
 FORM_PLAIN64 = 1
@@ -49,9 +49,13 @@ class SweepPlan(ctypes.Structure):
                 ("variant", ctypes.c_int32)]
 
 
+CYCLE_FIXED, CYCLE_ADAPTIVE = 0, 1
+
+
 class FastParams(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("plan", SweepPlan),
                 ("inner", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
+                ("cycle_mode", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("sor", ctypes.c_double), ("tol", ctypes.c_double)]
 
 

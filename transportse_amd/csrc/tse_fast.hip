@@ -676,10 +676,13 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
         return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
     const tse_sweep_plan &pl = prm->plan;
-    const int chunk = prm->inner;
-    if (!fast_plan_ok(pl, fp32, s->kind) || chunk < pl.ns || chunk % pl.ns ||
+    const int inner = prm->inner;
+    int chunk = inner;
+    if (!fast_plan_ok(pl, fp32, s->kind) || inner < pl.ns || inner % pl.ns ||
         prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    if (prm->cycle_mode != TSE_CYCLE_FIXED && prm->cycle_mode != TSE_CYCLE_ADAPTIVE)
+        return fail(TSE_ERR_ARG, "cycle_mode must be TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE");
     if (!fast_plan_fits(pl, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
     const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant == 2);
@@ -708,8 +711,21 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
         }
         sweeps += chunk;
         cycles += 1;
+        const double rel_prev = rel;
         rel = rn / bs;
         if (!(rel == rel)) break;  // NaN
+        if (prm->cycle_mode == TSE_CYCLE_ADAPTIVE) {
+            // sweeps still needed at the last cycle's rate, clamped
+            long next = inner;
+            if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
+                const double rate = std::log(rel / rel_prev) / chunk;   // < 0
+                const double need = std::log(prm->tol / rel) / rate;     // > 0
+                next = (long)std::ceil(need * 1.02);
+            }
+            next = std::min<long>(std::max<long>(next, inner), 8L * inner);
+            next = (next + pl.ns - 1) / pl.ns * pl.ns;
+            chunk = (int)next;
+        }
     }
     out->sweeps = sweeps;
     out->cycles = cycles;

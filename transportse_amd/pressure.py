@@ -73,13 +73,18 @@ class FastSystem:
         return self.plans[form]
 
     def solve(self, form="mixed", tol=1e-10, sor=1.996, max_sweeps=200000,
-              inner=None, plan=None) -> SolveStats:
+              inner=None, plan=None, adaptive=False) -> SolveStats:
+        """Solve to relative residual `tol`.  adaptive: cycle lengths from
+        the observed convergence rate (first cycle `inner`, then up to
+        8*inner), fewer FP64 residual passes; else every cycle is `inner`."""
         pl = plan or self.plan(form)
         ns = pl[0]
         inner = inner or 100
         inner = max(ns, (inner // ns) * ns)
         prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*pl),
                               inner=inner, max_sweeps=int(max_sweeps),
+                              cycle_mode=_lib.CYCLE_ADAPTIVE if adaptive
+                              else _lib.CYCLE_FIXED,
                               sor=float(sor), tol=float(tol))
         res = _lib.FastResult()
         t0 = time.perf_counter()
