@@ -466,7 +466,7 @@ def ensemble_section(args):
     specs = ensemble_specs(args.ens_members)
     cells = sum(s.cells for s in specs)
     t0 = time.time()
-    ens = Ensemble(specs, graph=True, inner_p=51, inner_tr=10)
+    ens = Ensemble(specs, graph=True, inner_p=51, inner_tr=6)
     log(f"ensemble: {len(specs)} realizations, {cells / 1e6:.1f}M cells, set up "
         f"in {time.time() - t0:.1f}s")
     tuned = ens.tune_plans(cycles=2, zchunks=(0, 32))
