@@ -101,7 +101,7 @@ class ServeStats:
 class Server:
     def __init__(self, budget_bytes=200e9, form="mixed", tol=1e-10,
                  plan_p=(3, 32, 2, 64, 2), plan_tr=(2, 16, 1, 64, 2), inner_p=51,
-                 inner_tr=10, sor_p=1.95, sor_tr=1.0, device=0,
+                 inner_tr=6, sor_p=1.95, sor_tr=1.0, device=0,
                  max_sweeps=400000, keep_results=False):
         import torch
         self.budget = float(budget_bytes)
