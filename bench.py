@@ -422,8 +422,8 @@ def sia_section(args, nx, ny, nz):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         out = scn.run(args.sia_steps, form=form, tol=tol, plan_p=plan,
-                      plan_tr=(2, 16, 1, 64), inner_p=100, inner_tr=10,
-                      max_sweeps=30000, strict=(form == "mixed"),
+                      plan_tr=(2, 16, 1, 64, 2), inner_p=99, inner_tr=10,
+                      max_sweeps=30000, strict=(form == "mixed"), adaptive=True,
                       progress=lambda k, r: log(
                           f"  sia {form} {'P' if k == 0 else 'C/T'} solve: "
                           f"{r.sweeps} sweeps, rel {r.rel_residual:.2e}, "

@@ -353,17 +353,19 @@ class DeviceScenario:
 
     def run(self, nsteps: int, form="mixed", tol=1e-10, plan_p=None,
             plan_tr=None, sor_p=None, sor_tr=1.0, inner_p=100, inner_tr=10,
-            max_sweeps=400000, strict=True, progress=None):
+            max_sweeps=400000, strict=True, progress=None, adaptive=False):
         """Drive ``steps`` with single-system solves; returns per-solve stats.
         sor_p None: the pressure factor comes from the scenario's tuner.
         strict: raise if a solve stops at max_sweeps above its tolerance.
-        progress: called with (kind, stats) after every solve."""
+        progress: called with (kind, stats) after every solve.
+        adaptive: pressure cycles sized from the convergence rate (the
+        batched drivers use fixed cycles)."""
         out = []
         for kind, sysm in self.steps(nsteps):
             if kind == 0:
                 om = self.omega_p.omega if sor_p is None else sor_p
                 r = sysm.solve(form, tol=tol, sor=om, inner=inner_p,
-                               plan=plan_p, max_sweeps=max_sweeps)
+                               plan=plan_p, max_sweeps=max_sweeps, adaptive=adaptive)
                 if sor_p is None:
                     self.omega_p.report(r.sweeps)
             else:
