@@ -554,12 +554,19 @@ def make_hip_backend(sim: Simulation, p_form: str = "faithful64", device: int = 
 
 
 def main(argv=None):
-    """``python -m transportse_amd.driver config.py`` (tse.py's CLI)."""
-    argv = sys.argv if argv is None else argv
-    if len(argv) < 2:
-        print("usage: python -m transportse_amd.driver <config.py>")
-        return 2
-    sim = Simulation(argv[1], make_hip_backend)
+    """``python -m transportse_amd.driver config.py`` (tse.py's CLI: the
+    config module runs to its end time; output goes next to the config)."""
+    import argparse
+    ap = argparse.ArgumentParser(prog="python -m transportse_amd.driver",
+                                 description="Run a TRANSPORTSE config on the GPU "
+                                             "(faithful FP64 solvers).")
+    ap.add_argument("config", help="config module (path or importable name)")
+    ap.add_argument("--max-steps", type=int, default=None,
+                    help="stop after this many time steps")
+    ap.add_argument("--quiet", action="store_true", help="no per-step printout")
+    args = ap.parse_args(sys.argv[1:] if argv is None else argv[1:])
+    sim = Simulation(args.config, make_hip_backend, verbose=not args.quiet,
+                     max_steps=args.max_steps)
     try:
         sim.run()
     except ConvergenceError as e:
