@@ -333,6 +333,19 @@ int rebuild(tse_batch *b) {
         for (int q = 0; q < nch; ++q) ritems.push_back(make_int2(s, q));
         for (int t = 0; t < m.g.ntiles; ++t) items.push_back(make_int2(s, t));
     }
+    // Longest tiles first (plane steps of the z-march: the chunk plus its
+    // halo planes), so a launch does not end on a few tall columns; tiles
+    // write disjoint cells from the launch's input buffer, so the order
+    // does not change any result.
+    const int R = 2 * b->plan.ns;
+    auto steps = [&](const int2 &w) {
+        const ZGeo &g = b->hmem[w.x].g;
+        const int k0 = (w.y / (g.tiles_x * g.tiles_y)) * g.zchunk;
+        const int k1 = std::min(k0 + g.zchunk, g.nz);
+        return (k1 + R) - std::max(k0 - R, 0);
+    };
+    std::stable_sort(items.begin(), items.end(),
+                     [&](const int2 &a, const int2 &c) { return steps(a) > steps(c); });
     if ((int)items.size() > b->items_cap) {
         if (b->ditems) (void)hipFree(b->ditems);
         b->items_cap = std::max<int>(items.size(), 2 * b->items_cap);
