@@ -41,6 +41,9 @@ Further sections of the same line (each bounded; --no-configs / --no-sia /
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
        python bench.py --profile-kernel N   (N launches of the sweep kernel
                                               and a 1 GiB copy; for rocprof)
+       python bench.py --profile-faithful N (N red/black iterations of the
+                                              drop-in compute_P sweep kernel on
+                                              the same inputs; for rocprof)
 """
 from __future__ import annotations
 
@@ -90,6 +93,7 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=8)
     ap.add_argument("--profile-kernel", type=int, default=0)
+    ap.add_argument("--profile-faithful", type=int, default=0)
     ap.add_argument("--plan", type=int, nargs="+", default=None,
                     metavar="N", help="FP32 sweep plan NS BY RY ZCHUNK [VARIANT]"
                     " instead of autotuning")
@@ -127,6 +131,8 @@ def main(argv=None):
                            seed=3)
     log(f"inputs generated in {time.time() - t0:.1f}s")
     dt = 86400.0
+    if args.profile_faithful:
+        return profile_faithful(args, prob, dt)
     fp = FastPressure(*prob.grid_args())
     fp.load(prob.P, dt, **prob.p_inputs())
     torch.cuda.synchronize()
@@ -339,6 +345,49 @@ def cpu_sweep_rate(orc, prob, dt, omega, threads, sweeps):
     n = prob.nx * prob.ny * prob.nz
     return {"value": n * sweeps / sec, "unit": "cell-updates/s", "cores": threads,
             "sweeps": sweeps, "seconds": sec}
+
+
+def profile_faithful(args, prob, dt):
+    """Fixed-dt red/black iterations of the drop-in compute_P half-sweep
+    kernel (tse_p_sweeps: assemble once, then sweeps without control
+    launches), timed with HIP events on the context's stream.  Algorithmic
+    bytes per half-sweep: 129 per slot of the colour (13 record doubles, the
+    old and new value, the other colour's value and the mask byte)."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from transportse_amd import _lib
+    from transportse_amd.device import HipContext
+    hx = HipContext(*prob.grid_args())
+    d = {n: hx.upload(np.ascontiguousarray(a)) for n, a in prob.p_inputs().items()}
+    d["P"] = hx.upload(prob.P)
+    d["Pn"] = hx.upload(prob.P)
+    pf = _lib.PFields(**{n: d[n].data_ptr() for n in _lib.P_FIELDS})
+    _lib.check(hx.lib.tse_p_sweeps(hx.ctx, ctypes.byref(pf), dt, 1.8, 1, 1),
+               "tse_p_sweeps")
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    _lib.check(hx.lib.tse_p_sweeps(hx.ctx, ctypes.byref(pf), dt, 1.8,
+                                   args.profile_faithful, 0), "tse_p_sweeps")
+    e1.record(stream)
+    torch.cuda.synchronize()
+    src = torch.empty(1 << 27, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)      # calibration: 1 GiB read + 1 GiB write
+    torch.cuda.synchronize()
+    us = e0.elapsed_time(e1) * 1e3 / (2 * args.profile_faithful)
+    slots = hx.F // 2
+    gbs = 129.0 * slots / (us * 1e-6) / 1e9
+    print(json.dumps({"profile_faithful_iterations": args.profile_faithful,
+                      "grid": list(prob.grid_args()[:3]),
+                      "us_per_half_sweep": us, "algorithmic_GBps": gbs,
+                      "roofline_frac": gbs / 8000.0,
+                      "copy_bytes": 2 * src.numel() * 8}))
+    hx.close()
+    return 0
 
 
 def configs_section(args):

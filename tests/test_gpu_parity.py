@@ -16,7 +16,7 @@ from helpers import CASES, GOLDEN, Fields, oracle_backend
 
 pytestmark = pytest.mark.gpu
 
-GRIDS = [(7, 6, 5), (16, 9, 12), (33, 20, 17), (64, 32, 32)]
+GRIDS = [(7, 6, 5), (9, 7, 5), (16, 9, 12), (33, 20, 17), (64, 32, 32)]
 
 
 def _pair(f, **kw):

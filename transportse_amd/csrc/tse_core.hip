@@ -147,7 +147,7 @@ static int sor_solve(tse_ctx *ctx, const SolveHooks &hk,
     if (rc) return rc;
     rc = hk.assemble(ctx, hk.fields, h.dt, prm->omega);
     if (rc) return rc;
-    const int nb = sweep_blocks(G.S);
+    const int nb = pair_sweep_blocks(G);
     int halvings = 0;
     int chunk = 4;
     for (;;) {
@@ -344,6 +344,8 @@ int tse_set_grid(tse_ctx *ctx, int32_t nx, int32_t ny, int32_t nz,
     G.nz = nz;
     G.nxh = (int)nxh;
     G.S = (int)S;
+    G.Sp = (int)(S + (S & 1));
+    G.Fr = 2L * G.Sp + kRecPad;
     G.div_nxh = make_fastdiv((uint32_t)nxh);
     G.div_ny = make_fastdiv((uint32_t)ny);
     G.dx = ctx->d_dxyz;
@@ -501,8 +503,8 @@ int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
     if (assemble) {
         rc = launch_p_assemble(ctx, f, dt);
         if (rc) return rc;
-    } else if (ctx->rec_elems < 13ull * 2ull * ctx->geo.S ||
-               ctx->mask_elems < 2ull * ctx->geo.S) {
+    } else if (ctx->rec_elems < rec_buffer_elems(ctx->geo, kPRec) ||
+               ctx->mask_elems < mask_bytes(ctx->geo)) {
         return fail(TSE_ERR_STATE, "tse_p_sweeps: no records assembled");
     }
     for (int n = 0; n < n_iter; ++n)

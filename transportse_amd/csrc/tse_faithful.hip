@@ -15,6 +15,8 @@
 //     from the record and the six neighbours, term by term in its order.
 //   * control: one workgroup reduces the change norm and applies the
 //     accept / grow / halve / fail logic in a device control block.
+#include <algorithm>
+
 #include "tse_internal.h"
 
 namespace tse {
@@ -175,9 +177,10 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Sums (a, b) over the 256-thread block; thread 0 writes them to out[0..1].
+// Sums (a, b) over a block of NT threads; thread 0 writes them to out[0..1].
+template <int NT = kBlock>
 __device__ inline void block_sum2(double a, double b, double *out) {
-    __shared__ double sa[kBlock / 64], sb[kBlock / 64];
+    __shared__ double sa[NT / 64], sb[NT / 64];
     a = wave_sum(a);
     b = wave_sum(b);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -189,13 +192,23 @@ __device__ inline void block_sum2(double a, double b, double *out) {
     if (threadIdx.x == 0) {
         double ta = 0.0, tb = 0.0;
 #pragma unroll
-        for (int q = 0; q < kBlock / 64; ++q) {
+        for (int q = 0; q < NT / 64; ++q) {
             ta += sa[q];
             tb += sb[q];
         }
         out[0] = ta;
         out[1] = tb;
     }
+}
+
+// Zero the record and mask of the pad slots (r = S in each colour when S is
+// odd): a pair sweep reads them with the slot before and must see mask 0.
+__device__ inline void zero_pad_records(const Geo &G, int nfields, double *rec,
+                                        uint8_t *mask) {
+    if (G.Sp == G.S || blockIdx.x != 0 || threadIdx.x >= 2) return;
+    const long ri = (long)threadIdx.x * G.Sp + G.S;
+    for (int f = 0; f < nfields; ++f) rec[f * G.Fr + ri] = 0.0;
+    mask[ri] = 0;
 }
 
 // ---- compute_P records ---------------------------------------------------------------
@@ -264,10 +277,12 @@ p_assemble_kernel(Geo G, const double *__restrict__ Pn,
             v[12] = R + M;
             m = (BP[t] != 0) ? 1 : 3;  // bit0 valid cell, bit1 updated
         }
+        const long ri = t + (long)c * (G.Sp - G.S);  // record slot
 #pragma unroll
-        for (int f = 0; f < kPRec; ++f) rec[f * F + t] = v[f];
-        mask[t] = m;
+        for (int f = 0; f < kPRec; ++f) rec[f * G.Fr + ri] = v[f];
+        mask[ri] = m;
     }
+    zero_pad_records(G, kPRec, rec, mask);
 }
 
 // The six other-colour neighbours of a cell (0 outside the grid).  Every
@@ -296,6 +311,66 @@ __device__ inline Nb6 neighbours6(const Geo &G, const Cell &q,
     return n;
 }
 
+// Half-sweep body shared by compute_P and the transport solvers: each thread
+// takes the pair of slots (2m, 2m+1) of the colour, reads both records with
+// one 16-byte load per field and the two mask bytes with one load, and
+// updates each slot with `update(record, neighbours, old value)`.  All loads
+// of a pair are issued before any test; padding slots (mask bit 0 clear) and
+// Dirichlet cells (bit 1 clear) discard the result.  With `check`, per-block
+// sums of (X_new - X_old)^2 and X_old^2 over valid cells go to partials.
+typedef double dpair __attribute__((ext_vector_type(2)));
+
+template <int NREC, typename Update>
+__device__ inline void pair_sweep(const Geo &G, int colour, double *__restrict__ X,
+                                  const double *__restrict__ rec,
+                                  const uint8_t *__restrict__ mask, bool check,
+                                  double *__restrict__ partials, Update update) {
+    const long rbase = (long)colour * G.Sp, xbase = (long)colour * G.S;
+    const double *__restrict__ Xo = X + (long)(1 - colour) * G.S;
+    double sd = 0.0, so = 0.0;
+    for (int r0 = 2 * (blockIdx.x * blockDim.x + threadIdx.x); r0 < G.S;
+         r0 += 2 * gridDim.x * blockDim.x) {
+        const long ri = rbase + r0;  // even: the pair is 16-byte aligned
+        dpair R[NREC];
+#pragma unroll
+        for (int f = 0; f < NREC; ++f)
+            R[f] = *reinterpret_cast<const dpair *>(rec + f * G.Fr + ri);
+        const uint16_t mm = *reinterpret_cast<const uint16_t *>(mask + ri);
+        // r0 = S-1 (S odd): the second slot is the pad slot, mask 0
+        const int r1 = r0 + 1 < G.S ? r0 + 1 : r0;
+        const double X0 = X[xbase + r0], X1 = X[xbase + r1];
+        const Nb6 n0 = neighbours6(G, decode(G, colour, r0), Xo);
+        const Nb6 n1 = neighbours6(G, decode(G, colour, r1), Xo);
+        double a[NREC], b[NREC];
+#pragma unroll
+        for (int f = 0; f < NREC; ++f) {
+            a[f] = R[f].x;
+            b[f] = R[f].y;
+        }
+        const double u0 = update(a, n0, X0), u1 = update(b, n1, X1);
+        const unsigned m0 = mm & 0xffu, m1 = mm >> 8;
+        const bool up0 = (m0 & 3) == 3, up1 = (m1 & 3) == 3;
+        const double N0 = up0 ? u0 : X0, N1 = up1 ? u1 : X1;
+        if (up0) X[xbase + r0] = N0;
+        if (up1) X[xbase + r1] = N1;
+        if (check) {
+            if (m0 & 1) {
+                const double d = N0 - X0;
+                sd += d * d;
+                so += X0 * X0;
+            }
+            if (m1 & 1) {
+                const double d = N1 - X1;
+                sd += d * d;
+                so += X1 * X1;
+            }
+        }
+    }
+    if (check)
+        block_sum2(sd, so,
+                   partials + 2 * ((long)colour * kMaxSweepBlocks + blockIdx.x));
+}
+
 // One colour half-sweep of compute_P (tse.py:705-716), update expression of
 // tse.py:692-695 evaluated left to right.  If the iteration checks the norm,
 // per-block sums of (P_new-P_old)^2 and P_old^2 go to partials.
@@ -309,55 +384,32 @@ p_sweep_kernel(Geo G, int colour, double *__restrict__ P,
         if (ctl->status != RUNNING) return;
         check = is_check_iter(ctl->check_mode, ctl->num_it + 1);
     }
-    const long F = 2L * G.S;
-    const long base = (long)colour * G.S;
-    const double *__restrict__ Po = P + (long)(1 - colour) * G.S;
-    double sd = 0.0, so = 0.0;
-    // all operands of a cell are loaded before any test; padding slots
-    // (mask bit 0 clear) and Dirichlet cells (bit 1 clear) discard the result
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < G.S;
-         r += gridDim.x * blockDim.x) {
-        const long t = base + r;
-        const uint8_t m = mask[t];
-        const double Pc = P[t];
-        const Cell q = decode(G, colour, r);
-        const Nb6 n = neighbours6(G, q, Po);
-        const double *rr = rec + t;
-        const double Anew = rr[0 * F] * n.xp;
-        const double Cnew = rr[1 * F] * n.xm;
-        const double Enew = rr[2 * F] * n.yp;
-        const double Gnew = rr[3 * F] * n.ym;
-        const double Inew = rr[4 * F] * n.zp;
-        const double Knew = rr[5 * F] * n.zm;
-        double s = Anew + rr[6 * F];
-        s = s + Cnew;
-        s = s - rr[7 * F];
-        s = s + Enew;
-        s = s + Gnew;
-        s = s + Inew;
-        s = s + rr[8 * F];
-        s = s + Knew;
-        s = s - rr[9 * F];
-        s = s + rr[10 * F];
-        s = s + rr[11 * F];
-        const double Pupd = (1 - sor) * Pc + sor * s / rr[12 * F];
-        const bool upd = (m & 3) == 3;
-        const double Pnew = upd ? Pupd : Pc;
-        if (upd) P[t] = Pnew;
-        if (check && (m & 1)) {
-            const double d = Pnew - Pc;
-            sd += d * d;
-            so += Pc * Pc;
-        }
-    }
-    if (check)
-        block_sum2(sd, so,
-                   partials + 2 * ((long)colour * kMaxSweepBlocks + blockIdx.x));
+    pair_sweep<kPRec>(G, colour, P, rec, mask, check, partials,
+                      [sor](const double *rr, const Nb6 &n, double Pc) {
+                          const double Anew = rr[0] * n.xp;
+                          const double Cnew = rr[1] * n.xm;
+                          const double Enew = rr[2] * n.yp;
+                          const double Gnew = rr[3] * n.ym;
+                          const double Inew = rr[4] * n.zp;
+                          const double Knew = rr[5] * n.zm;
+                          double s = Anew + rr[6];
+                          s = s + Cnew;
+                          s = s - rr[7];
+                          s = s + Enew;
+                          s = s + Gnew;
+                          s = s + Inew;
+                          s = s + rr[8];
+                          s = s + Knew;
+                          s = s - rr[9];
+                          s = s + rr[10];
+                          s = s + rr[11];
+                          return (1 - sor) * Pc + sor * s / rr[12];
+                      });
 }
 
 // Control step after each iteration (tse.py:699-740 and the transport
 // counterparts 896-914, 1064-1079).  One block of kBlock threads.
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kCtlBlock)
 control_kernel(Ctl *__restrict__ ctl, const double *__restrict__ partials,
                int nblocks) {
     __shared__ int s_status, s_check;
@@ -369,6 +421,7 @@ control_kernel(Ctl *__restrict__ ctl, const double *__restrict__ partials,
     if (s_status != RUNNING) return;
     double sd = 0.0, so = 0.0;
     if (s_check) {
+#pragma unroll 4
         for (int b = threadIdx.x; b < 2 * nblocks; b += blockDim.x) {
             const int colour = b >= nblocks ? 1 : 0;
             const int blk = b - colour * nblocks;
@@ -378,7 +431,7 @@ control_kernel(Ctl *__restrict__ ctl, const double *__restrict__ partials,
         }
     }
     __shared__ double res[2];
-    block_sum2(sd, so, res);
+    block_sum2<kCtlBlock>(sd, so, res);
     __syncthreads();
     if (threadIdx.x != 0) return;
     Ctl &C = *ctl;
@@ -666,10 +719,12 @@ c_assemble_kernel(Geo G, const double *__restrict__ Cn,
             transport_tail(v, a, nbo, Cn[t], R, M, WC[t], EC, omega);
             m = (BC[t] != 0) ? 1 : 3;
         }
+        const long ri = t + (long)c * (G.Sp - G.S);  // record slot
 #pragma unroll
-        for (int f = 0; f < kTRec; ++f) rec[f * F + t] = v[f];
-        mask[t] = m;
+        for (int f = 0; f < kTRec; ++f) rec[f * G.Fr + ri] = v[f];
+        mask[ri] = m;
     }
+    zero_pad_records(G, kTRec, rec, mask);
 }
 
 // One face of compute_T's solve() (tse.py:943-1031).
@@ -798,10 +853,12 @@ t_assemble_kernel(Geo G, const double *__restrict__ Tn,
             transport_tail(v, a, nbo, Tn[t], R, M, WT[t], ET, omega);
             m = (BT[t] != 0) ? 1 : 3;
         }
+        const long ri = t + (long)c * (G.Sp - G.S);  // record slot
 #pragma unroll
-        for (int f = 0; f < kTRec; ++f) rec[f * F + t] = v[f];
-        mask[t] = m;
+        for (int f = 0; f < kTRec; ++f) rec[f * G.Fr + ri] = v[f];
+        mask[ri] = m;
     }
+    zero_pad_records(G, kTRec, rec, mask);
 }
 
 // One colour half-sweep of compute_C / compute_T (tse.py:883-894,
@@ -816,41 +873,20 @@ tr_sweep_kernel(Geo G, int colour, double *__restrict__ X,
         if (ctl->status != RUNNING) return;
         check = is_check_iter(ctl->check_mode, ctl->num_it + 1);
     }
-    const long F = 2L * G.S;
-    const long base = (long)colour * G.S;
-    const double *__restrict__ Xo = X + (long)(1 - colour) * G.S;
-    double sd = 0.0, so = 0.0;
-    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < G.S;
-         r += gridDim.x * blockDim.x) {
-        const long t = base + r;
-        const uint8_t m = mask[t];
-        const double Xc = X[t];
-        const Cell q = decode(G, colour, r);
-        const Nb6 n = neighbours6(G, q, Xo);
-        const double *rr = rec + t;
-        double s = rr[0 * F] * n.xp;
-        s = s + rr[1 * F] * n.xm;
-        s = s + rr[2 * F] * n.yp;
-        s = s + rr[3 * F] * n.ym;
-        s = s + rr[4 * F] * n.zp;
-        s = s + rr[5 * F] * n.zm;
-        double u = omega * s + rr[6 * F];
-        u = u + rr[7 * F];
-        u = u + rr[8 * F];
-        u = u - rr[9 * F];
-        const double Xupd = (1 - sor) * Xc + sor * u / rr[10 * F];
-        const bool upd = (m & 3) == 3;
-        const double Xnew = upd ? Xupd : Xc;
-        if (upd) X[t] = Xnew;
-        if (check && (m & 1)) {
-            const double d = Xnew - Xc;
-            sd += d * d;
-            so += Xc * Xc;
-        }
-    }
-    if (check)
-        block_sum2(sd, so,
-                   partials + 2 * ((long)colour * kMaxSweepBlocks + blockIdx.x));
+    pair_sweep<kTRec>(G, colour, X, rec, mask, check, partials,
+                      [sor, omega](const double *rr, const Nb6 &n, double Xc) {
+                          double s = rr[0] * n.xp;
+                          s = s + rr[1] * n.xm;
+                          s = s + rr[2] * n.yp;
+                          s = s + rr[3] * n.ym;
+                          s = s + rr[4] * n.zp;
+                          s = s + rr[5] * n.zm;
+                          double u = omega * s + rr[6];
+                          u = u + rr[7];
+                          u = u + rr[8];
+                          u = u - rr[9];
+                          return (1 - sor) * Xc + sor * u / rr[10];
+                      });
 }
 
 // ---- l2_norm of two fields --------------------------------------------------------------
@@ -898,9 +934,9 @@ static inline int grid_for(long n) {
 int launch_p_assemble(tse_ctx *ctx, const tse_p_fields *f, double dt) {
     const Geo &G = ctx->geo;
     const long F = 2L * G.S;
-    int rc = ensure_rec(ctx, (size_t)kPRec * F);
+    int rc = ensure_rec(ctx, rec_buffer_elems(G, kPRec));
     if (rc) return rc;
-    rc = ensure_mask(ctx, (size_t)F);
+    rc = ensure_mask(ctx, mask_bytes(G));
     if (rc) return rc;
     hipLaunchKernelGGL(p_assemble_kernel, dim3(grid_for(F)), dim3(kBlock), 0,
                        ctx->stream, G, f->Pn, f->c_f, f->nue, f->rho_f, f->phi,
@@ -913,7 +949,7 @@ int launch_p_assemble(tse_ctx *ctx, const tse_p_fields *f, double dt) {
 int launch_p_sweep(tse_ctx *ctx, double *P, int colour, double sor,
                    bool with_ctl) {
     const Geo &G = ctx->geo;
-    hipLaunchKernelGGL(p_sweep_kernel, dim3(sweep_blocks(G.S)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(p_sweep_kernel, dim3(pair_sweep_blocks(G)), dim3(kBlock), 0,
                        ctx->stream, G, colour, P, ctx->rec, ctx->mask, sor,
                        with_ctl ? ctx->ctl : nullptr, ctx->partials);
     TSE_HIP_TRY(hipGetLastError());
@@ -921,7 +957,7 @@ int launch_p_sweep(tse_ctx *ctx, double *P, int colour, double sor,
 }
 
 int launch_control(tse_ctx *ctx, int nblocks) {
-    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
+    hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kCtlBlock), 0, ctx->stream,
                        ctx->ctl, ctx->partials, nblocks);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
@@ -968,9 +1004,9 @@ int launch_c_assemble(tse_ctx *ctx, const tse_c_fields *f, double dt,
                       double omega) {
     const Geo &G = ctx->geo;
     const long F = 2L * G.S;
-    int rc = ensure_rec(ctx, (size_t)kTRec * F);
+    int rc = ensure_rec(ctx, rec_buffer_elems(G, kTRec));
     if (rc) return rc;
-    rc = ensure_mask(ctx, (size_t)F);
+    rc = ensure_mask(ctx, mask_bytes(G));
     if (rc) return rc;
     hipLaunchKernelGGL(c_assemble_kernel, dim3(grid_for(F)), dim3(kBlock), 0,
                        ctx->stream, G, f->Cn, f->D_fx, f->D_fy, f->D_fz, f->phi,
@@ -985,9 +1021,9 @@ int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
                       double omega) {
     const Geo &G = ctx->geo;
     const long F = 2L * G.S;
-    int rc = ensure_rec(ctx, (size_t)kTRec * F);
+    int rc = ensure_rec(ctx, rec_buffer_elems(G, kTRec));
     if (rc) return rc;
-    rc = ensure_mask(ctx, (size_t)F);
+    rc = ensure_mask(ctx, mask_bytes(G));
     if (rc) return rc;
     hipLaunchKernelGGL(t_assemble_kernel, dim3(grid_for(F)), dim3(kBlock), 0,
                        ctx->stream, G, f->Tn, f->l_f, f->cp_f, f->rho_f, f->phi,
@@ -1001,7 +1037,7 @@ int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
 int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
                     double omega) {
     const Geo &G = ctx->geo;
-    hipLaunchKernelGGL(tr_sweep_kernel, dim3(sweep_blocks(G.S)), dim3(kBlock),
+    hipLaunchKernelGGL(tr_sweep_kernel, dim3(pair_sweep_blocks(G)), dim3(kBlock),
                        0, ctx->stream, G, colour, X, ctx->rec, ctx->mask, sor,
                        omega, ctx->ctl, ctx->partials);
     TSE_HIP_TRY(hipGetLastError());
@@ -1010,7 +1046,7 @@ int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
 
 int launch_l2(tse_ctx *ctx, const double *New, const double *Old) {
     const Geo &G = ctx->geo;
-    const int nb = sweep_blocks(2 * G.S);
+    const int nb = std::min(sweep_blocks(2 * G.S), 4096);
     hipLaunchKernelGGL(l2_partial_kernel, dim3(nb), dim3(kBlock), 0,
                        ctx->stream, G, New, Old, ctx->partials);
     TSE_HIP_TRY(hipGetLastError());

@@ -229,14 +229,14 @@ tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
                        const double *__restrict__ rec,
                        const uint8_t *__restrict__ mask, double omega, TrOp op) {
     const long N = op.N;
-    const long F = 2L * G.S;
+    const long F = G.Fr;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
         const int row = (int)fdiv((uint32_t)t, dnx);
         const int i = (int)t - row * G.nx;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * G.ny;
-        const long c = slot(G, i, j, k);
+        const long c = rec_slot(G, i, j, k);
         const double *r = rec + c;
         const bool active = (mask[c] & 2) != 0;
         const double den = r[10 * F];

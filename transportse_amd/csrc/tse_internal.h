@@ -56,6 +56,8 @@ __host__ __device__ inline uint32_t fdiv(uint32_t n, const FastDiv &f) {
 struct Geo {
     int nx, ny, nz, nxh;
     int S;             // slots per colour = nxh*ny*nz
+    int Sp;            // record slots per colour: S rounded up to even
+    long Fr;           // record field stride: 2*Sp + kRecPad
     FastDiv div_nxh;   // slot -> row
     FastDiv div_ny;    // row  -> (j, k)
     const double *dx, *dy, *dz;  // device copies
@@ -85,6 +87,18 @@ __host__ __device__ inline int slot(const Geo &G, int i, int j, int k) {
     return ((i + j + k) & 1) * G.S + (k * G.ny + j) * G.nxh + (i >> 1);
 }
 
+// Coefficient records and masks of the faithful solvers use their own
+// layout: colour c starts at c*Sp (Sp even, so a pair of slots (2m, 2m+1) is
+// one 16-byte load), and field f of a record starts at f*Fr.  Fr = 2*Sp +
+// kRecPad: with a field stride that is a large power of two the 13 record
+// streams of a sweep fall on the same HBM channels at the same time (a
+// 512x512x256 half-sweep ran at 52 % of the HBM peak, 67 % with the pad).
+// A pad slot (r = S when S is odd) holds a zero record and mask 0.
+constexpr long kRecPad = 544;
+__host__ __device__ inline long rec_slot(const Geo &G, int i, int j, int k) {
+    return (long)((i + j + k) & 1) * G.Sp + (long)(k * G.ny + j) * G.nxh + (i >> 1);
+}
+
 // ---- device control block of one SOR solve ------------------------------------
 // Written by the control kernel after every iteration; the sweep kernels read
 // `status` and return at once when the solve has stopped, so the host can
@@ -110,7 +124,8 @@ constexpr int kBlock = 256;          // 4 wavefronts of 64
 // R - omega*M.  Each field is a 2*S array in the split layout.
 constexpr int kPRec = 13;
 constexpr int kTRec = 11;
-constexpr int kMaxSweepBlocks = 4096;
+constexpr int kMaxSweepBlocks = 65536;
+constexpr int kCtlBlock = 1024;      // control kernel: reads 2 x sweep blocks partials
 
 inline int sweep_blocks(int S) {
     long b = ((long)S + kBlock - 1) / kBlock;
@@ -118,6 +133,14 @@ inline int sweep_blocks(int S) {
     if (b < 1) b = 1;
     return (int)b;
 }
+
+// Blocks of a faithful half-sweep: one pair of slots per thread, up to
+// kMaxSweepBlocks (beyond that the blocks stride over the slots).
+inline int pair_sweep_blocks(const Geo &G) { return sweep_blocks(G.Sp / 2); }
+
+// Record buffer sizes (doubles / bytes) for nfields fields.
+inline size_t rec_buffer_elems(const Geo &G, int nfields) { return (size_t)nfields * G.Fr; }
+inline size_t mask_bytes(const Geo &G) { return 2 * (size_t)G.Sp; }
 
 }  // namespace tse
 
@@ -133,7 +156,7 @@ struct tse_ctx {
     size_t rec_elems = 0;
     uint8_t *mask = nullptr;
     size_t mask_elems = 0;
-    double *partials = nullptr;  // [2 colours][kMaxSweepBlocks][2]
+    double *partials = nullptr;  // [2 colours][kMaxSweepBlocks][2] (2 MiB)
     tse::Ctl *ctl = nullptr;      // device
     tse::Ctl *ctl_host = nullptr; // pinned host mirror
     double *red_out = nullptr;    // device scratch (4 doubles) for reductions
