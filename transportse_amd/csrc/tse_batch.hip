@@ -149,11 +149,11 @@ batch_tr_zmarch_kernel(const BMember *__restrict__ mem,
             zero = 0;
         }
         if constexpr (VAR == 0)
-            tr_zmarch_tile<T, NS, BY>(m.g, w.y, pin, pout, coef, m.N, inv, rhs, omega,
-                                      zero);
+            tr_zmarch_tile<T, NS, BY>(m.g, w.y, pin, pout, coef, m.top.ld, inv, rhs,
+                                      omega, zero);
         else
-            tr_zmarch_tile_xr<T, NS, BY>(m.g, w.y, pin, pout, coef, m.N, inv, rhs,
-                                         omega, zero);
+            tr_zmarch_tile_xr<T, NS, BY>(m.g, w.y, pin, pout, coef, m.top.ld, inv,
+                                         rhs, omega, zero);
         __syncthreads();
     }
 }

@@ -589,10 +589,11 @@ struct FastOp {  // natural-layout operator arrays
 };
 
 
-struct TrOp {  // natural layout, coefficient f of cell t at c[f*N + t]
+struct TrOp {  // natural layout, coefficient f of cell t at c[f*ld + t]
     double *c, *D, *invD, *b;
     float *c32, *invD32;
-    long N;
+    long N;   // cells
+    long ld;  // element distance between the coefficient arrays
 };
 
 
@@ -604,7 +605,7 @@ struct TrOp {  // natural layout, coefficient f of cell t at c[f*N + t]
 template <typename T, int NS, int BY>
 __device__ __forceinline__ void
 tr_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
-               T *__restrict__ Pout, const T *__restrict__ coef, long N,
+               T *__restrict__ Pout, const T *__restrict__ coef, long ld,
                const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
                int zero_init) {
     constexpr int R = 2 * NS;
@@ -651,7 +652,7 @@ tr_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
         ni = invD[o];
         nb = rhs[o];
 #pragma unroll
-        for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
+        for (int f = 0; f < 6; ++f) nc[f] = coef[f * ld + o];
     }
     int s0 = kl0 % NSLOT;
     for (int k = kl0; k <= k1 + R - 1; ++k) {
@@ -669,7 +670,7 @@ tr_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
             ni = invD[o];
             nb = rhs[o];
 #pragma unroll
-            for (int f = 0; f < 6; ++f) nc[f] = coef[f * N + o];
+            for (int f = 0; f < 6; ++f) nc[f] = coef[f * ld + o];
         }
 #pragma unroll
         for (int q = R + 1; q > 0; --q) p[q] = p[q - 1];
@@ -723,7 +724,7 @@ tr_zmarch_tile(const ZGeo &g, int tile, const T *__restrict__ Pin,
 template <typename T, int NS, int BY>
 __device__ __forceinline__ void
 tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
-                  T *__restrict__ Pout, const T *__restrict__ coef, long N,
+                  T *__restrict__ Pout, const T *__restrict__ coef, long ld,
                   const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
                   int zero_init) {
     constexpr int R = 2 * NS;
@@ -758,7 +759,7 @@ tr_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     const auto rOut = buf_rsrc(Pout, fbytes);
     __amdgpu_buffer_rsrc_t rC[6];
 #pragma unroll
-    for (int f = 0; f < 6; ++f) rC[f] = buf_rsrc(coef + f * N, fbytes);
+    for (int f = 0; f < 6; ++f) rC[f] = buf_rsrc(coef + f * ld, fbytes);
     bool dm[R];
 #pragma unroll
     for (int h = 0; h < R; ++h) dm[h] = in_dom && d >= h + 1;
@@ -880,7 +881,7 @@ tr_residual_cell(int nx, int ny, int nz, int i, int j, int k, long t,
                  const double *__restrict__ pin, const float *__restrict__ e,
                  double *__restrict__ pout, const TrOp &op,
                  float *__restrict__ r32, double &sr, double &sb) {
-    const long N = op.N;
+    const long L = op.ld;
     const long plane = (long)nx * ny;
     auto X = [&](long o) -> double {
         return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
@@ -890,12 +891,12 @@ tr_residual_cell(int nx, int ny, int nz, int i, int j, int k, long t,
     double r = 0.0;
     if (op.invD[t] != 0.0) {
         double acc = op.b[t];
-        if (i < nx - 1) acc += op.c[0 * N + t] * X(t + 1);
-        if (i > 0) acc += op.c[1 * N + t] * X(t - 1);
-        if (j < ny - 1) acc += op.c[2 * N + t] * X(t + nx);
-        if (j > 0) acc += op.c[3 * N + t] * X(t - nx);
-        if (k < nz - 1) acc += op.c[4 * N + t] * X(t + plane);
-        if (k > 0) acc += op.c[5 * N + t] * X(t - plane);
+        if (i < nx - 1) acc += op.c[0 * L + t] * X(t + 1);
+        if (i > 0) acc += op.c[1 * L + t] * X(t - 1);
+        if (j < ny - 1) acc += op.c[2 * L + t] * X(t + nx);
+        if (j > 0) acc += op.c[3 * L + t] * X(t - nx);
+        if (k < nz - 1) acc += op.c[4 * L + t] * X(t + plane);
+        if (k > 0) acc += op.c[5 * L + t] * X(t - plane);
         r = acc - op.D[t] * xc;
         sr += r * r;
         sb += op.b[t] * op.b[t];
@@ -932,6 +933,7 @@ enum SysKind { SYS_P = 0, SYS_TR = 1 };
 struct FastSys {
     int kind = SYS_P;
     long N = 0;
+    long ld = 0;          // element distance between the carved arrays
     double *buf64 = nullptr;
     float *buf32 = nullptr;
     double *partials = nullptr;
@@ -942,32 +944,33 @@ struct FastSys {
 
     int n64() const { return kind == SYS_P ? 6 : 9; }
     int n32() const { return kind == SYS_P ? 5 : 8; }
-    double *p(int q) const { return buf64 + (n64() + q) * N; }
-    float *r32() const { return buf32 + (n32() - 1) * N; }
-    float *e(int q) const { return buf32 + (n32() + q) * N; }
+    double *p(int q) const { return buf64 + (n64() + q) * ld; }
+    float *r32() const { return buf32 + (n32() - 1) * ld; }
+    float *e(int q) const { return buf32 + (n32() + q) * ld; }
     FastOp pop() const {
         FastOp op;
         op.Wx = buf64;
-        op.Wy = buf64 + N;
-        op.Wz = buf64 + 2 * N;
-        op.D = buf64 + 3 * N;
-        op.invD = buf64 + 4 * N;
-        op.b = buf64 + 5 * N;
+        op.Wy = buf64 + ld;
+        op.Wz = buf64 + 2 * ld;
+        op.D = buf64 + 3 * ld;
+        op.invD = buf64 + 4 * ld;
+        op.b = buf64 + 5 * ld;
         op.Wx32 = buf32;
-        op.Wy32 = buf32 + N;
-        op.Wz32 = buf32 + 2 * N;
-        op.invD32 = buf32 + 3 * N;
+        op.Wy32 = buf32 + ld;
+        op.Wz32 = buf32 + 2 * ld;
+        op.invD32 = buf32 + 3 * ld;
         return op;
     }
     TrOp trop() const {
         TrOp op;
         op.c = buf64;
-        op.D = buf64 + 6 * N;
-        op.invD = buf64 + 7 * N;
-        op.b = buf64 + 8 * N;
+        op.D = buf64 + 6 * ld;
+        op.invD = buf64 + 7 * ld;
+        op.b = buf64 + 8 * ld;
         op.c32 = buf32;
-        op.invD32 = buf32 + 6 * N;
+        op.invD32 = buf32 + 6 * ld;
         op.N = N;
+        op.ld = ld;
         return op;
     }
 };

@@ -244,8 +244,8 @@ tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
 #pragma unroll
         for (int f = 0; f < 6; ++f) {
             const double cf = omega * r[f * F];
-            op.c[f * N + t] = cf;
-            op.c32[f * N + t] = (float)cf;
+            op.c[f * op.ld + t] = cf;
+            op.c32[f * op.ld + t] = (float)cf;
         }
         op.D[t] = den;
         op.invD[t] = inv;
@@ -257,15 +257,15 @@ tr_from_records_kernel(Geo G, FastDiv dnx, FastDiv dny,
 template <typename T, int NS, int BY, int VAR>
 __global__ void __launch_bounds__(64 * BY)
 tr_zmarch_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
-                 const T *__restrict__ coef, long N, const T *__restrict__ invD,
+                 const T *__restrict__ coef, long ld, const T *__restrict__ invD,
                  const T *__restrict__ rhs, T omega, int zero_init) {
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;
     if constexpr (VAR == 0)
-        tr_zmarch_tile<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
+        tr_zmarch_tile<T, NS, BY>(g, tile, Pin, Pout, coef, ld, invD, rhs, omega,
                                   zero_init);
     else
-        tr_zmarch_tile_xr<T, NS, BY>(g, tile, Pin, Pout, coef, N, invD, rhs, omega,
+        tr_zmarch_tile_xr<T, NS, BY>(g, tile, Pin, Pout, coef, ld, invD, rhs, omega,
                                      zero_init);
 }
 
@@ -331,7 +331,7 @@ residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ 
                 r = acc - pop.D[t] * pc;
                 wzm = wz;
             } else {
-                const long Nc = trop.N;
+                const long Nc = trop.ld;
                 const double c0 = trop.c[0 * Nc + t], c1 = trop.c[1 * Nc + t];
                 const double c2 = trop.c[2 * Nc + t], c3 = trop.c[3 * Nc + t];
                 const double c4 = trop.c[4 * Nc + t], c5 = trop.c[5 * Nc + t];
@@ -415,9 +415,14 @@ int ensure_sys(tse_ctx *ctx, FastSys *s) {
         s->buf32 = nullptr;
     }
     s->N = N;
-    if (hipMalloc((void **)&s->buf64, sizeof(double) * (s->n64() + 2) * N) != hipSuccess)
+    // the arrays are 1088 elements apart beyond a multiple of 64 (FP32:
+    // 4.25 KiB, FP64: 8.5 KiB), so that equal offsets in different arrays
+    // do not fall on the same HBM channel when N is a large power of two
+    // (the 512x512x256 FP32 pressure sweep: 604 -> 577 us per launch)
+    s->ld = ((N + 63) / 64) * 64 + 1088;
+    if (hipMalloc((void **)&s->buf64, sizeof(double) * (s->n64() + 2) * s->ld) != hipSuccess)
         return fail(TSE_ERR_NOMEM, "fast-form FP64 operator allocation failed");
-    if (hipMalloc((void **)&s->buf32, sizeof(float) * (s->n32() + 2) * N) != hipSuccess)
+    if (hipMalloc((void **)&s->buf32, sizeof(float) * (s->n32() + 2) * s->ld) != hipSuccess)
         return fail(TSE_ERR_NOMEM, "fast-form FP32 operator allocation failed");
     s->ready = false;
     return 0;
@@ -503,14 +508,14 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
 
 template <typename T>
 static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
-                            const T *Pin, T *Pout, const T *coef, long N,
+                            const T *Pin, T *Pout, const T *coef, long ld,
                             const T *invD, const T *rhs, T omega, int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
     const int ns = pl.ns, by = pl.by, v = pl.variant;
 #define TSE_TZ(V_, NS_, BY_)                                                      \
     if (v == V_ && ns == NS_ && by == BY_) {                                      \
         hipLaunchKernelGGL((tr_zmarch_kernel<T, NS_, BY_, V_>), grid,             \
-                           dim3(64, BY_), 0, st, zg, Pin, Pout, coef, N, invD,    \
+                           dim3(64, BY_), 0, st, zg, Pin, Pout, coef, ld, invD,   \
                            rhs, omega, zero_init);                                \
         TSE_HIP_TRY(hipGetLastError());                                           \
         return 0;                                                                 \
@@ -591,7 +596,7 @@ static int plain_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
             TrOp op = s->trop();
             rc = launch_tr_zmarch<double>(ctx->stream, zg, pl,
                                           s->p(s->cur), s->p(1 - s->cur), op.c,
-                                          s->N, op.invD, op.b, sor, 0);
+                                          op.ld, op.invD, op.b, sor, 0);
         }
         if (rc) return rc;
         s->cur = 1 - s->cur;
@@ -615,7 +620,7 @@ static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
         } else {
             TrOp op = s->trop();
             rc = launch_tr_zmarch<float>(ctx->stream, zg, pl, s->e(e),
-                                         s->e(1 - e), op.c32, s->N, op.invD32,
+                                         s->e(1 - e), op.c32, op.ld, op.invD32,
                                          s->r32(), sor, q == 0);
         }
         if (rc) return rc;
