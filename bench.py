@@ -59,17 +59,12 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 BYTES_FP64_SWEEP = 56
-# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plans
-# the autotuner picks there: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, each
-# calibrated on a 1 GiB copy (profiles/rb_zmarch*_fp32_pressure_512x512x256.md)
-TRAFFIC_PER_LAUNCH = {(512, 512, 256, 2, 16, 1, 64, 0): 3.337e9,
-                      (512, 512, 256, 2, 16, 1, 128, 1): 3.167e9,
-                      (512, 512, 256, 2, 16, 1, 64, 1): 3.080e9,
-                      (512, 512, 256, 3, 32, 2, 32, 2): 3.776e9,
-                      (512, 512, 256, 3, 32, 2, 64, 2): 3.645e9,
-                      (512, 512, 256, 3, 32, 2, 128, 2): 3.473e9,
-                      (512, 512, 256, 2, 32, 2, 32, 2): 3.424e9,
-                      (512, 512, 256, 3, 32, 2, 0, 2): 2.923e9}
+# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plan
+# the autotuner picks there, with the operator arrays at their padded
+# distance: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, each calibrated on a 1 GiB
+# copy (profiles/rb_zmarch_xr_fp32_pressure_512x512x256.md).  Other plans
+# report traffic null.
+TRAFFIC_PER_LAUNCH = {(512, 512, 256, 3, 32, 2, 0, 2): 2.652e9}
 # sweep kernel and its profile summary per plan variant
 VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
                   1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),
