@@ -114,10 +114,12 @@ static int t_assemble_hook(tse_ctx *ctx, const void *f, double dt,
     return launch_t_assemble(ctx, (const tse_t_fields *)f, dt, omega);
 }
 
+// Both state slots get the host's state.
 static int write_ctl(tse_ctx *ctx, const Ctl &h) {
     *ctx->ctl_host = h;
-    TSE_HIP_TRY(hipMemcpyAsync(ctx->ctl, ctx->ctl_host, sizeof(Ctl),
-                               hipMemcpyHostToDevice, ctx->stream));
+    for (int k = 0; k < 2; ++k)
+        TSE_HIP_TRY(hipMemcpyAsync(ctx->ctl + k, ctx->ctl_host, sizeof(Ctl),
+                                   hipMemcpyHostToDevice, ctx->stream));
     return 0;
 }
 
@@ -147,19 +149,39 @@ static int sor_solve(tse_ctx *ctx, const SolveHooks &hk,
     if (rc) return rc;
     rc = hk.assemble(ctx, hk.fields, h.dt, prm->omega);
     if (rc) return rc;
+    // The solver state lives in two device slots.  Iteration group q (red
+    // sweep, black sweep) writes its norm partials to parity q & 1.  On
+    // larger grids a control launch follows every group and writes both
+    // slots.  On grids of at most kFuseBlocks sweep blocks the control step
+    // of group q-1 runs inside the red sweep of group q: it reads slot q & 1
+    // (the state after group q-2) and stores slot (q-1) & 1, which the black
+    // sweep then reads; a control launch ends each chunk of groups and
+    // writes both slots again, so the first group of a chunk and the host
+    // see one state.
     const int nb = pair_sweep_blocks(G);
+    const bool fuse = nb <= kFuseBlocks;
     int halvings = 0;
     int chunk = 4;
+    long q = 0;
     for (;;) {
         for (int n = 0; n < chunk; ++n) {
+            ++q;
+            double *part = ctx->partials + (q & 1) * kPartialsPerIter;
+            const double *prev = ctx->partials + ((q - 1) & 1) * kPartialsPerIter;
+            Ctl *const before = ctx->ctl + ((q - 1) & 1);  // state after group q-1
             for (int colour = 0; colour < 2; ++colour) {
+                SweepCtl sc{before, nullptr, nullptr, part, nb, 0};
+                if (colour == 0 && fuse && n > 0)
+                    sc = SweepCtl{ctx->ctl + (q & 1), before, prev, part, nb, 1};
                 rc = hk.transport
-                         ? launch_tr_sweep(ctx, hk.X, colour, prm->sor, prm->omega)
-                         : launch_p_sweep(ctx, hk.X, colour, prm->sor, true);
+                         ? launch_tr_sweep(ctx, hk.X, colour, prm->sor, prm->omega, sc)
+                         : launch_p_sweep(ctx, hk.X, colour, prm->sor, sc);
                 if (rc) return rc;
             }
-            rc = launch_control(ctx, nb);
-            if (rc) return rc;
+            if (!fuse || n == chunk - 1) {
+                rc = launch_control(ctx, before, part, nb);
+                if (rc) return rc;
+            }
         }
         TSE_HIP_TRY(hipMemcpyAsync(ctx->ctl_host, ctx->ctl, sizeof(Ctl),
                                    hipMemcpyDeviceToHost, ctx->stream));
@@ -222,10 +244,10 @@ int tse_ctx_create(tse_ctx **out, int device, void *stream) {
     tse_ctx *ctx = new tse_ctx();
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
-    if (hipMalloc((void **)&ctx->ctl, sizeof(Ctl)) != hipSuccess ||
+    if (hipMalloc((void **)&ctx->ctl, 2 * sizeof(Ctl)) != hipSuccess ||
         hipHostMalloc((void **)&ctx->ctl_host, sizeof(Ctl)) != hipSuccess ||
         hipMalloc((void **)&ctx->partials,
-                  sizeof(double) * 2 * 2 * kMaxSweepBlocks) != hipSuccess ||
+                  sizeof(double) * 2 * kPartialsPerIter) != hipSuccess ||
         hipMalloc((void **)&ctx->red_out, 4 * sizeof(double)) != hipSuccess ||
         hipHostMalloc((void **)&ctx->red_host, 4 * sizeof(double)) !=
             hipSuccess) {
@@ -509,7 +531,8 @@ int tse_p_sweeps(tse_ctx *ctx, const tse_p_fields *f, double dt, double sor,
     }
     for (int n = 0; n < n_iter; ++n)
         for (int colour = 0; colour < 2; ++colour) {
-            rc = launch_p_sweep(ctx, f->P, colour, sor, false);
+            rc = launch_p_sweep(ctx, f->P, colour, sor,
+                                SweepCtl{nullptr, nullptr, nullptr, ctx->partials, 0, 0});
             if (rc) return rc;
         }
     return 0;

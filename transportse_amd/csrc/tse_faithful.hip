@@ -311,6 +311,91 @@ __device__ inline Nb6 neighbours6(const Geo &G, const Cell &q,
     return n;
 }
 
+// Control step after an iteration (tse.py:718-740 and the transport
+// counterparts 896-914, 1064-1079): `C` is the state before the iteration,
+// (sd, so) the norm sums of its two half-sweeps (used on check iterations).
+// A stopped solve keeps its state.
+__device__ inline Ctl control_apply(Ctl C, bool check, double sd, double so) {
+    if (C.status != RUNNING) return C;
+    C.num_it += 1;
+    C.num_sub_it += 1;
+    double diff = 1e100;
+    if (check) {
+        diff = sqrt(sd) / sqrt(so + 1e-200);
+        if (isnan(diff)) diff = 1.0;
+    }
+    C.diff = diff;
+    if (diff < C.conv || C.omega_zero) {
+        C.t += C.dt;
+        if ((double)C.num_sub_it < C.max_it / 2.0) {
+            const double g = C.dt * C.inc_it;
+            C.dt = (C.max_dt < g) ? C.max_dt : g;
+        }
+        C.status = CONVERGED;
+    } else if (C.num_sub_it >= C.max_it) {
+        C.status = NEED_HALVE;  // the host halves dt and restores the field
+    } else if (C.dt <= C.min_dt) {
+        C.status = FAILED;
+    }
+    return C;
+}
+
+// Norm partials of one iteration: block b of colour c wrote its two sums at
+// partials[2*(c*kMaxSweepBlocks + b)].
+__device__ inline void partial_pair(const double *__restrict__ partials, int nblocks,
+                                    int b, double &sd, double &so) {
+    const int colour = b >= nblocks ? 1 : 0;
+    const double *p = partials + 2 * ((long)colour * kMaxSweepBlocks + (b - colour * nblocks));
+    sd += p[0];
+    so += p[1];
+}
+
+// Stand-alone control step (one block of kCtlBlock threads): state `in`,
+// the iteration's partials; writes the new state to both slots ctl[0..1].
+__global__ void __launch_bounds__(kCtlBlock)
+control_kernel(const Ctl *__restrict__ in, Ctl *__restrict__ ctl,
+               const double *__restrict__ partials, int nblocks) {
+    __shared__ int s_check;
+    if (threadIdx.x == 0)
+        s_check = in->status == RUNNING && is_check_iter(in->check_mode, in->num_it + 1);
+    __syncthreads();
+    double sd = 0.0, so = 0.0;
+    if (s_check) {
+#pragma unroll 4
+        for (int b = threadIdx.x; b < 2 * nblocks; b += blockDim.x)
+            partial_pair(partials, nblocks, b, sd, so);
+    }
+    __shared__ double res[2];
+    block_sum2<kCtlBlock>(sd, so, res);
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    const Ctl r = control_apply(*in, s_check != 0, res[0], res[1]);
+    ctl[0] = r;
+    ctl[1] = r;
+}
+
+// The control step folded into the next red half-sweep (grids of at most
+// kFuseBlocks sweep blocks): every block reads the previous state and the
+// previous iteration's partials (one pair per thread, summed in the same
+// order in every block, so every block reaches the same state), block 0
+// stores the state, and the block sweeps under it.  Returns the state.
+__device__ inline Ctl fused_control(const Ctl *__restrict__ in, Ctl *__restrict__ out,
+                                    const double *__restrict__ prev, int nblocks) {
+    __shared__ Ctl s_ctl;
+    __shared__ double res[2];
+    double sd = 0.0, so = 0.0;
+    if ((int)threadIdx.x < 2 * nblocks) partial_pair(prev, nblocks, threadIdx.x, sd, so);
+    block_sum2(sd, so, res);
+    if (threadIdx.x == 0) {
+        const Ctl c = *in;
+        const bool check = is_check_iter(c.check_mode, c.num_it + 1);
+        s_ctl = control_apply(c, check, res[0], res[1]);
+        if (blockIdx.x == 0) *out = s_ctl;
+    }
+    __syncthreads();
+    return s_ctl;
+}
+
 // Half-sweep body shared by compute_P and the transport solvers: each thread
 // takes the pair of slots (2m, 2m+1) of the colour, reads both records with
 // one 16-byte load per field and the two mask bytes with one load, and
@@ -377,13 +462,15 @@ __device__ inline void pair_sweep(const Geo &G, int colour, double *__restrict__
 __global__ void __launch_bounds__(kBlock)
 p_sweep_kernel(Geo G, int colour, double *__restrict__ P,
                const double *__restrict__ rec, const uint8_t *__restrict__ mask,
-               double sor, const Ctl *__restrict__ ctl,
-               double *__restrict__ partials) {
+               double sor, SweepCtl sc) {
     bool check = false;
-    if (ctl) {
-        if (ctl->status != RUNNING) return;
-        check = is_check_iter(ctl->check_mode, ctl->num_it + 1);
+    if (sc.in) {
+        const Ctl c = sc.fuse ? fused_control(sc.in, sc.out, sc.prev, sc.nblocks)
+                              : *sc.in;
+        if (c.status != RUNNING) return;
+        check = is_check_iter(c.check_mode, c.num_it + 1);
     }
+    double *__restrict__ partials = sc.partials;
     pair_sweep<kPRec>(G, colour, P, rec, mask, check, partials,
                       [sor](const double *rr, const Nb6 &n, double Pc) {
                           const double Anew = rr[0] * n.xp;
@@ -405,56 +492,6 @@ p_sweep_kernel(Geo G, int colour, double *__restrict__ P,
                           s = s + rr[11];
                           return (1 - sor) * Pc + sor * s / rr[12];
                       });
-}
-
-// Control step after each iteration (tse.py:699-740 and the transport
-// counterparts 896-914, 1064-1079).  One block of kBlock threads.
-__global__ void __launch_bounds__(kCtlBlock)
-control_kernel(Ctl *__restrict__ ctl, const double *__restrict__ partials,
-               int nblocks) {
-    __shared__ int s_status, s_check;
-    if (threadIdx.x == 0) {
-        s_status = ctl->status;
-        s_check = is_check_iter(ctl->check_mode, ctl->num_it + 1);
-    }
-    __syncthreads();
-    if (s_status != RUNNING) return;
-    double sd = 0.0, so = 0.0;
-    if (s_check) {
-#pragma unroll 4
-        for (int b = threadIdx.x; b < 2 * nblocks; b += blockDim.x) {
-            const int colour = b >= nblocks ? 1 : 0;
-            const int blk = b - colour * nblocks;
-            const double *p = partials + 2 * ((long)colour * kMaxSweepBlocks + blk);
-            sd += p[0];
-            so += p[1];
-        }
-    }
-    __shared__ double res[2];
-    block_sum2<kCtlBlock>(sd, so, res);
-    __syncthreads();
-    if (threadIdx.x != 0) return;
-    Ctl &C = *ctl;
-    C.num_it += 1;
-    C.num_sub_it += 1;
-    double diff = 1e100;
-    if (s_check) {
-        diff = sqrt(res[0]) / sqrt(res[1] + 1e-200);
-        if (isnan(diff)) diff = 1.0;
-    }
-    C.diff = diff;
-    if (diff < C.conv || C.omega_zero) {
-        C.t += C.dt;
-        if ((double)C.num_sub_it < C.max_it / 2.0) {
-            const double g = C.dt * C.inc_it;
-            C.dt = (C.max_dt < g) ? C.max_dt : g;
-        }
-        C.status = CONVERGED;
-    } else if (C.num_sub_it >= C.max_it) {
-        C.status = NEED_HALVE;  // the host halves dt and restores the field
-    } else if (C.dt <= C.min_dt) {
-        C.status = FAILED;
-    }
 }
 
 // ---- compute_q (tse.py:744-780) --------------------------------------------------------
@@ -867,12 +904,15 @@ __global__ void __launch_bounds__(kBlock)
 tr_sweep_kernel(Geo G, int colour, double *__restrict__ X,
                 const double *__restrict__ rec,
                 const uint8_t *__restrict__ mask, double sor, double omega,
-                const Ctl *__restrict__ ctl, double *__restrict__ partials) {
+                SweepCtl sc) {
     bool check = false;
-    if (ctl) {
-        if (ctl->status != RUNNING) return;
-        check = is_check_iter(ctl->check_mode, ctl->num_it + 1);
+    if (sc.in) {
+        const Ctl c = sc.fuse ? fused_control(sc.in, sc.out, sc.prev, sc.nblocks)
+                              : *sc.in;
+        if (c.status != RUNNING) return;
+        check = is_check_iter(c.check_mode, c.num_it + 1);
     }
+    double *__restrict__ partials = sc.partials;
     pair_sweep<kTRec>(G, colour, X, rec, mask, check, partials,
                       [sor, omega](const double *rr, const Nb6 &n, double Xc) {
                           double s = rr[0] * n.xp;
@@ -947,18 +987,18 @@ int launch_p_assemble(tse_ctx *ctx, const tse_p_fields *f, double dt) {
 }
 
 int launch_p_sweep(tse_ctx *ctx, double *P, int colour, double sor,
-                   bool with_ctl) {
+                   const SweepCtl &sc) {
     const Geo &G = ctx->geo;
     hipLaunchKernelGGL(p_sweep_kernel, dim3(pair_sweep_blocks(G)), dim3(kBlock), 0,
-                       ctx->stream, G, colour, P, ctx->rec, ctx->mask, sor,
-                       with_ctl ? ctx->ctl : nullptr, ctx->partials);
+                       ctx->stream, G, colour, P, ctx->rec, ctx->mask, sor, sc);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }
 
-int launch_control(tse_ctx *ctx, int nblocks) {
+int launch_control(tse_ctx *ctx, const Ctl *in, const double *partials,
+                   int nblocks) {
     hipLaunchKernelGGL(control_kernel, dim3(1), dim3(kCtlBlock), 0, ctx->stream,
-                       ctx->ctl, ctx->partials, nblocks);
+                       in, ctx->ctl, partials, nblocks);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1035,11 +1075,11 @@ int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
 }
 
 int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
-                    double omega) {
+                    double omega, const SweepCtl &sc) {
     const Geo &G = ctx->geo;
     hipLaunchKernelGGL(tr_sweep_kernel, dim3(pair_sweep_blocks(G)), dim3(kBlock),
                        0, ctx->stream, G, colour, X, ctx->rec, ctx->mask, sor,
-                       omega, ctx->ctl, ctx->partials);
+                       omega, sc);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }

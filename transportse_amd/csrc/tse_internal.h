@@ -126,6 +126,24 @@ constexpr int kPRec = 13;
 constexpr int kTRec = 11;
 constexpr int kMaxSweepBlocks = 65536;
 constexpr int kCtlBlock = 1024;      // control kernel: reads 2 x sweep blocks partials
+// Grids of at most kFuseBlocks sweep blocks run the control step inside the
+// next red half-sweep (one partial pair per thread of a kBlock block).
+constexpr int kFuseBlocks = kBlock / 2;
+// Doubles of one iteration's norm partials: [2 colours][kMaxSweepBlocks][2].
+constexpr long kPartialsPerIter = 2L * 2 * kMaxSweepBlocks;
+
+// Control wiring of one faithful half-sweep launch.  `in` is the solver
+// state the sweep runs under (null: fixed sweeps, no control); with `fuse`
+// the sweep first applies the previous iteration's control step to `in`
+// with its partials `prev`, and block 0 stores the result at `out`.
+struct SweepCtl {
+    const Ctl *in;
+    Ctl *out;
+    const double *prev;
+    double *partials;
+    int nblocks;
+    int fuse;
+};
 
 inline int sweep_blocks(int S) {
     long b = ((long)S + kBlock - 1) / kBlock;
@@ -156,8 +174,8 @@ struct tse_ctx {
     size_t rec_elems = 0;
     uint8_t *mask = nullptr;
     size_t mask_elems = 0;
-    double *partials = nullptr;  // [2 colours][kMaxSweepBlocks][2] (2 MiB)
-    tse::Ctl *ctl = nullptr;      // device
+    double *partials = nullptr;  // [2 iteration parities][kPartialsPerIter] (4 MiB)
+    tse::Ctl *ctl = nullptr;      // device, two slots (see sor_solve)
     tse::Ctl *ctl_host = nullptr; // pinned host mirror
     double *red_out = nullptr;    // device scratch (4 doubles) for reductions
     double *red_host = nullptr;   // pinned
@@ -173,8 +191,9 @@ int check_ctx(tse_ctx *ctx);
 // launchers implemented in tse_faithful.hip
 int launch_p_assemble(tse_ctx *ctx, const tse_p_fields *f, double dt);
 int launch_p_sweep(tse_ctx *ctx, double *P, int colour, double sor,
-                   bool with_ctl);
-int launch_control(tse_ctx *ctx, int nblocks);
+                   const SweepCtl &sc);
+int launch_control(tse_ctx *ctx, const Ctl *in, const double *partials,
+                   int nblocks);
 int launch_q(tse_ctx *ctx, const double *P, const double *BP,
              const double *Kx, const double *Ky, const double *Kz,
              double *qx, double *qy, double *qz, const double *nue,
@@ -189,7 +208,7 @@ int launch_c_assemble(tse_ctx *ctx, const tse_c_fields *f, double dt,
 int launch_t_assemble(tse_ctx *ctx, const tse_t_fields *f, double dt,
                       double omega);
 int launch_tr_sweep(tse_ctx *ctx, double *X, int colour, double sor,
-                    double omega);
+                    double omega, const SweepCtl &sc);
 int launch_l2(tse_ctx *ctx, const double *New, const double *Old);
 int launch_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
                          const double *qy, const double *qz, double *vx,
