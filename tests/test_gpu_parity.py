@@ -125,6 +125,24 @@ def test_compute_p_halving():
     print("halving ok", o.last.num_halvings, flush=True)
 
 
+# 65 x 65 x 21 = 88,725 slots per colour (odd): 174 sweep blocks, more than
+# the 128 up to which the control step runs inside the next red sweep, so
+# these solves use a control launch after every iteration
+LARGE = (129, 65, 21)
+
+
+def test_compute_p_control_launches():
+    test_compute_p(LARGE, 1.8, 1e-8)
+
+
+def test_compute_c_control_launches():
+    test_compute_c(LARGE, 0.6, 1.2, False)
+
+
+def test_compute_t_control_launches():
+    test_compute_t(LARGE, 1, 1.0, True)
+
+
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("omega,sor,imbl", [(1, 1.0, True), (0.6, 1.2, False),
                                             (0, 1.0, True)])
