@@ -296,7 +296,9 @@ int tse_batch_rearm(tse_batch *b, int32_t slot);
 int tse_batch_rearm_many(tse_batch *b, const int32_t *slots, int32_t n);
 /* Hands the member's iterate back to its context (tse_fast_*_result). */
 int tse_batch_writeback(tse_batch *b, int32_t slot);
-/* Over-relaxation factor of one member (default: the batch's sor). */
+/* Over-relaxation factor of one member (default: the batch's sor); takes
+   effect at the next cycle or re-arm, which uploads it without a
+   synchronisation of its own. */
 int tse_batch_set_sor(tse_batch *b, int32_t slot, double sor);
 /* Writes back and removes a member. */
 int tse_batch_remove(tse_batch *b, int32_t slot);

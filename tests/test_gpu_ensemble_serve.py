@@ -87,6 +87,24 @@ def test_ensemble_graph_alone_single():
     scn.close()
 
 
+def test_ensemble_runs_continue():
+    """run(1) then run(1) (the second one asynchronous) ends where run(2)
+    ends: the realizations stay in their batches between runs."""
+    from transportse_amd.ensemble import Ensemble
+    specs = _specs(3)
+    e1 = Ensemble(specs, graph=True, **KW)
+    e1.run(2)
+    f1 = _fields(e1)
+    e1.close()
+    e2 = Ensemble(specs, graph=True, **KW)
+    e2.run(1)
+    e2.run(1, mode="async")
+    f2 = _fields(e2)
+    e2.close()
+    for a, b in zip(f1, f2):
+        assert _same(a, b)
+
+
 def test_ensemble_tuner_picks_a_plan():
     from transportse_amd.ensemble import Ensemble
     e = Ensemble(_specs(3), graph=True, **KW)

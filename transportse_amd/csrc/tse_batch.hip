@@ -263,6 +263,7 @@ struct tse_batch {
     unsigned *dqueue = nullptr;
     int grid = 0;                     // persistent blocks per sweep launch
     bool dirty = true;                // items / graph need a rebuild
+    bool sor_dirty = false;           // hmem's sor values not yet on the device
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
 };
@@ -498,9 +499,17 @@ int enqueue_cycle(tse_batch *b) {
 }
 
 int prepare(tse_batch *b) {
-    if (!b->dirty) return 0;
-    int rc = rebuild(b);
+    if (!b->dirty) {
+        if (b->sor_dirty) {   // member table from host memory, before the next sweeps
+            TSE_B_TRY(hipMemcpyAsync(b->dmem, b->hmem.data(), sizeof(BMember) * b->cap,
+                                     hipMemcpyHostToDevice, b->stream));
+            b->sor_dirty = false;
+        }
+        return 0;
+    }
+    int rc = rebuild(b);   // uploads the whole member table
     if (rc) return rc;
+    b->sor_dirty = false;
     if (b->n_items == 0) return 0;
     const bool fp32 = b->form == TSE_FORM_MIXED;
     return fp32 ? launch_sweeps<float>(b, 0, 0, true)
@@ -702,13 +711,40 @@ int tse_batch_rearm_many(tse_batch *b, const int32_t *slots, int32_t n) {
     rc = prepare(b);
     if (rc) return rc;
     const bool fp32 = b->form == TSE_FORM_MIXED;
+    // hst mirrors the device state of every slot (each cycle and re-arm ends
+    // with a read-back, and nothing runs on the batch between calls), so the
+    // re-armed entries go up with the others in one copy
     for (int q = 0; q < n; ++q) {
         const int slot = slots[q];
         BStat st0{};
         st0.cur = member_sys(b, b->ctx[slot])->cur;
         b->hst[slot] = st0;
-        TSE_B_TRY(hipMemcpyAsync(b->dst + slot, &b->hst[slot], sizeof(BStat),
+    }
+    if (n > 0)
+        TSE_B_TRY(hipMemcpyAsync(b->dst, b->hst, sizeof(BStat) * b->cap,
                                  hipMemcpyHostToDevice, b->stream));
+    // every member re-armed (the lockstep case): one initial-residual launch
+    // over all items and one reduction over all slots; per-member sums are
+    // the same as with a launch per member
+    std::vector<char> mark(b->cap, 0);
+    int distinct = 0, used = 0;
+    for (int q = 0; q < n; ++q)
+        if (!mark[slots[q]]) {
+            mark[slots[q]] = 1;
+            ++distinct;
+        }
+    for (int q = 0; q < b->cap; ++q) used += b->hused[q] ? 1 : 0;
+    if (n > 0 && distinct == used && b->n_ritems > 0) {
+        rc = residual_any(b, fp32 ? 2 : 0, 0, b->n_ritems, 0, 0, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(batch_reduce_kernel, dim3(b->cap), dim3(kBlock), 0, b->stream,
+                           b->dmem, b->dst, b->dused, 0, b->dpartials, 1, 0,
+                           b->inner, b->max_sweeps, b->tol);
+        TSE_B_TRY(hipGetLastError());
+        n = 0;   // done
+    }
+    for (int q = 0; q < n; ++q) {
+        const int slot = slots[q];
         const BMember &mm = b->hmem[slot];
         const int nch = (int)((mm.N + kChunk - 1) / kChunk);
         rc = residual_any(b, fp32 ? 2 : 0, mm.rchunk0, nch, 0, 0, 1);
@@ -737,7 +773,8 @@ int tse_batch_writeback(tse_batch *b, int32_t slot) {
 }
 
 // Sets the over-relaxation factor of one member (default: the batch's).
-// Takes effect from the next cycle; a captured cycle stays valid.
+// Takes effect from the next cycle or re-arm, which uploads the member table
+// once for all changed members; a captured cycle stays valid.
 int tse_batch_set_sor(tse_batch *b, int32_t slot, double sor) {
     int rc = check_batch(b);
     if (rc) return rc;
@@ -745,11 +782,7 @@ int tse_batch_set_sor(tse_batch *b, int32_t slot, double sor) {
         return fail(TSE_ERR_ARG, "no member in that slot");
     if (!(sor > 0.0 && sor < 2.0)) return fail(TSE_ERR_ARG, "sor must be in (0, 2)");
     b->hmem[slot].sor = sor;
-    if (!b->dirty) {
-        TSE_B_TRY(hipMemcpyAsync(&b->dmem[slot].sor, &b->hmem[slot].sor,
-                                 sizeof(double), hipMemcpyHostToDevice, b->stream));
-        TSE_B_TRY(hipStreamSynchronize(b->stream));
-    }
+    b->sor_dirty = true;
     return 0;
 }
 

@@ -88,6 +88,7 @@ class Ensemble:
             self.scn = [DeviceScenario(s, device, use_graphs=graph) for s in specs]
         self.plan_p, self.plan_tr = tuple(plan_p), tuple(plan_tr)
         self.bp = self.btr = None
+        self._slots = {KIND_P: {}, KIND_TR: {}}   # realization -> batch slot
 
     def _batch(self, kind, plan, inner, sor, tol=None):
         return Batch(kind, self.form, plan, inner=inner,
@@ -163,7 +164,8 @@ class Ensemble:
                 self.bp = self._batch(KIND_P, self.plan_p, self.inner_p, self.sor_p)
                 self.btr = self._batch(KIND_TR, self.plan_tr, self.inner_tr,
                                        self.sor_tr)
-            slots = {KIND_P: {}, KIND_TR: {}}
+                self._slots = {KIND_P: {}, KIND_TR: {}}
+            slots = self._slots     # members stay in the batches between runs
             batches = {KIND_P: self.bp, KIND_TR: self.btr}
             torch.cuda.synchronize()
             t0 = time.perf_counter()
@@ -179,15 +181,15 @@ class Ensemble:
                     if not ids:
                         continue
                     b = batches[kind]
+                    rearm = []
                     for i in ids:
                         if i not in slots[kind]:
                             slots[kind][i] = b.add(pend[i][1])
-                            if kind == KIND_P and self.tune_omega:
-                                b.set_sor(slots[kind][i], self.scn[i].omega_p.omega)
                         else:
-                            if kind == KIND_P and self.tune_omega:
-                                b.set_sor(slots[kind][i], self.scn[i].omega_p.omega)
-                            b.rearm(slots[kind][i])
+                            rearm.append(slots[kind][i])
+                        if kind == KIND_P and self.tune_omega:
+                            b.set_sor(slots[kind][i], self.scn[i].omega_p.omega)
+                    b.rearm_many(rearm)
                     cycles += b.run(graph=self.graph)
                     for i in ids:
                         st = b.status(slots[kind][i])
@@ -218,8 +220,9 @@ class Ensemble:
                 self.bp = self._batch(KIND_P, self.plan_p, self.inner_p, self.sor_p)
                 self.btr = self._batch(KIND_TR, self.plan_tr, self.inner_tr,
                                        self.sor_tr)
+                self._slots = {KIND_P: {}, KIND_TR: {}}
             batches = {KIND_P: self.bp, KIND_TR: self.btr}
-            slots = {KIND_P: {}, KIND_TR: {}}
+            slots = self._slots     # members stay in the batches between runs
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             gens = {i: s.steps(nsteps) for i, s in enumerate(self.scn)}
@@ -239,6 +242,8 @@ class Ensemble:
             rearm = {KIND_P: [], KIND_TR: []}
             for i, (kind, sysm) in pend.items():
                 enter(i, kind, sysm, rearm)
+            for kind, lst in rearm.items():   # members kept from an earlier run
+                batches[kind].rearm_many(lst)
             ticks = solves = cycles = 0
             cell_sweeps = 0.0
             sweeps = {KIND_P: 0, KIND_TR: 0}

@@ -309,16 +309,29 @@ class DeviceScenario:
             _lib.check(L.tse_fast_mark_loaded(ctx, k), "tse_fast_mark_loaded")
 
     # the SIA step ----------------------------------------------------------------
+    def _save_step_start(self):
+        """Pn, Cn, Tn = the fields at the start of the step."""
+        self.Pn.copy_(self.P)
+        for s in range(self.spec.species):
+            self.Cn[s].copy_(self.C[s])
+        self.Tn.copy_(self.T)
+
+    def _sia_reset(self):
+        """A rejected SIA iteration (tse.py:1430-1438): keep the density,
+        restart P, T, C from the step start."""
+        self.rho_prev.copy_(self.rho)
+        self.P.copy_(self.Pn)
+        self.T.copy_(self.Tn)
+        for s in range(self.spec.species):
+            self.C[s].copy_(self.Cn[s])
+
     def steps(self, nsteps: int):
         """Generator over ``nsteps`` SIA steps; yields (kind, system) for each
         solve and resumes once the system holds its solution."""
         sp = self.spec
         for _ in range(nsteps):
             st = StepStats()
-            self.Pn.copy_(self.P)
-            for s in range(sp.species):
-                self.Cn[s].copy_(self.C[s])
-            self.Tn.copy_(self.T)
+            self._segment("step_begin", self._save_step_start)
             for it in range(sp.sia_maxit):
                 st.sia_iters = it + 1
                 self._segment("setup_p", self.setup_p, (0,))
@@ -341,11 +354,7 @@ class DeviceScenario:
                 diff = self.density_norm()
                 if diff < sp.sia_conv or it == sp.sia_maxit - 1:
                     break
-                self.rho_prev.copy_(self.rho)            # tse.py:1430-1438
-                self.P.copy_(self.Pn)
-                self.T.copy_(self.Tn)
-                for s in range(sp.species):
-                    self.C[s].copy_(self.Cn[s])
+                self._segment("sia_reset", self._sia_reset)
             st.mass = self.mass()
             self.t += sp.dt
             self.n_steps += 1
