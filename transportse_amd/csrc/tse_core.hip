@@ -1,10 +1,12 @@
 // tse_core.hip -- context, layout conversion and the C-ABI entry points.
 //
 // The SOR solvers run as a queue of (red sweep, black sweep, control) launch
-// triples on the context's stream.  The control kernel keeps the solver state
-// (iteration counters, t, dt, status) in device memory and the sweeps return
-// immediately once the solve has stopped, so the host enqueues iterations in
-// growing chunks and synchronises once per chunk, not once per iteration.
+// triples on the context's stream (on small grids the control step runs
+// inside the next red sweep, see sor_solve).  The control step keeps the
+// solver state (iteration counters, t, dt, status) in device memory and the
+// sweeps return immediately once the solve has stopped, so the host enqueues
+// iterations in growing chunks and synchronises once per chunk, not once per
+// iteration.
 // Only a dt halving needs the host: it restores the field, halves dt,
 // re-assembles the coefficient records (the storage term depends on dt) and
 // resumes.
