@@ -146,3 +146,55 @@ def test_alone_equals_mix_and_rearm(kind):
         print("alone == mix, re-arm reproduces; kind", kind, "sweeps",
               st2.sweeps, flush=True)
         b2.close()
+
+
+def test_per_member_sor_and_rearm_many():
+    """Per-member over-relaxation factors (set after add: uploaded with the
+    member table before the next cycle), then a second solve of every member
+    re-armed at once (one initial-residual launch for all), then a re-arm of
+    two members only: each solve must match the member's single-system solve
+    at its own factor bit for bit, and the members left alone keep their
+    iterate."""
+    import torch
+    from transportse_amd.batch import Batch
+    form, plan, tol = "mixed", (2, 16, 1, 0), 1e-10
+    sors = [1.85, 1.9, 1.95, 1.8, 1.93]
+    with torch.cuda.stream(_stream()):
+        systems = _systems(0)
+        alone = []
+        for s, w in zip(systems, sors):
+            st = s.solve(form, tol=tol, sor=w, inner=20, plan=plan, max_sweeps=20000)
+            assert st.converged
+            alone.append((st, s.result()))
+            s.assemble()
+        b = Batch(0, form, plan, inner=20, tol=tol, sor=1.9, max_sweeps=20000)
+        slots = [b.add(s) for s in systems]
+        for slot, w in zip(slots, sors):
+            b.set_sor(slot, w)
+
+        def check(which):
+            for n in which:
+                st, x = alone[n]
+                ms = b.status(slots[n])
+                b.writeback(slots[n])
+                assert ms.converged and ms.sweeps == st.sweeps
+                assert np.array_equal(x, systems[n].result())
+
+        b.run(graph=True)
+        check(range(len(systems)))
+        for s in systems:
+            s.assemble()
+        b.rearm_many(slots)                       # every member at once
+        b.run(graph=True)
+        check(range(len(systems)))
+        kept = {n: systems[n].result() for n in (1, 3, 4)}
+        for n in (0, 2):
+            systems[n].assemble()
+        b.rearm_many([slots[0], slots[2]])        # two members only
+        b.run(graph=True)
+        check((0, 2))
+        for n, x in kept.items():
+            b.writeback(slots[n])
+            assert np.array_equal(x, systems[n].result())
+        print("per-member sor and rearm_many ok", flush=True)
+        b.close()
