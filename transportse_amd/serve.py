@@ -163,7 +163,9 @@ class Server:
                 # one cycle of each batch over every request in a solve
                 for b in self.batches.values():
                     b.cycle(graph=False)
-                # advance requests whose solve finished
+                # advance requests whose solve finished; their re-arms go to
+                # each batch at once after the pass
+                rearm = {KIND_P: [], KIND_TR: []}
                 for rid in list(live):
                     r, scn, gen, kind, slots, need = live[rid]
                     b = self.batches[kind]
@@ -193,8 +195,10 @@ class Server:
                         del live[rid]
                         done.append(r)
                         continue
-                    self._enter(scn, kind, sysm, slots)
+                    self._enter(scn, kind, sysm, slots, rearm)
                     live[rid][3] = kind
+                for k, lst in rearm.items():
+                    self.batches[k].rearm_many(lst)
             sec = now()
         lat = np.array([r.done - r.arrival * time_scale for r in done])
         return ServeStats(
@@ -205,18 +209,19 @@ class Server:
             solves_per_s=len(done) / sec if sec > 0 else 0.0, peak_bytes=peak,
             max_concurrent=max_conc, latencies=lat.tolist())
 
-    def _enter(self, scn, kind, sysm, slots):
+    def _enter(self, scn, kind, sysm, slots, rearm=None):
         """Put a scenario's set-up system into the batch of its kind (a slot
-        per kind for the request's lifetime; re-armed for later solves)."""
+        per kind for the request's lifetime; re-armed for later solves, at
+        once if `rearm` is None, else queued in rearm[kind])."""
         b = self.batches[kind]
         if slots[kind] is None:
             slots[kind] = b.add(sysm)
-            if kind == KIND_P:
-                b.set_sor(slots[kind], scn.omega_p.omega)
-        else:
-            if kind == KIND_P:
-                b.set_sor(slots[kind], scn.omega_p.omega)
+        elif rearm is None:
             b.rearm(slots[kind])
+        else:
+            rearm[kind].append(slots[kind])
+        if kind == KIND_P:
+            b.set_sor(slots[kind], scn.omega_p.omega)
 
     def close(self):
         for b in self.batches.values():
