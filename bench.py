@@ -59,16 +59,20 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 BYTES_FP64_SWEEP = 56
-# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plan
-# the autotuner picks there, with the operator arrays at their padded
-# distance: TCC_EA0_RDREQ_128B x 128 + WRITE_SIZE, each calibrated on a 1 GiB
-# copy (profiles/rb_zmarch_xr_fp32_pressure_512x512x256.md).  Other plans
-# report traffic null.
-TRAFFIC_PER_LAUNCH = {(512, 512, 256, 3, 32, 2, 0, 2): 2.652e9}
+# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plans
+# the autotuner picks there (variant 3, and variant 2 before it), with the
+# operator arrays at their padded distance: TCC_EA0_RDREQ_128B x 128 +
+# WRITE_SIZE, each calibrated on a 1 GiB copy
+# (profiles/rb_zmarch_x2_fp32_pressure_512x512x256.md,
+# profiles/rb_zmarch_xr_fp32_pressure_512x512x256.md).  Other plans report
+# traffic null.
+TRAFFIC_PER_LAUNCH = {(512, 512, 256, 3, 32, 2, 0, 2): 2.652e9,
+                      (512, 512, 256, 3, 32, 2, 0, 3): 2.555e9}
 # sweep kernel and its profile summary per plan variant
 VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
                   1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),
-                  2: ("rb_zmarch_xr_kernel<float>", "rb_zmarch_xr")}
+                  2: ("rb_zmarch_xr_kernel<float>", "rb_zmarch_xr"),
+                  3: ("rb_zmarch_x2_kernel<float>", "rb_zmarch_x2")}
 
 
 def log(*a):

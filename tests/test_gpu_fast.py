@@ -184,6 +184,51 @@ def test_unrolled_ring_variant_equals_lane_exchange_variant(ns, by, ry, zchunk):
     assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
 
 
+# variant 3 (an x-pair of cells per thread): even nx; partial and single
+# tiles in x and y, several middle tiles, unit y / z extents
+PAIR_GRIDS = [(84, 41, 23), (40, 37, 21), (160, 90, 30), (64, 33, 2), (2, 20, 30),
+              (52, 1, 60)]
+
+
+@pytest.mark.parametrize("shape", PAIR_GRIDS)
+@pytest.mark.parametrize("ns,zchunk", [(3, 0), (2, 7), (3, 9)])
+def test_pair_variant_equals_unrolled_ring_variant(shape, ns, zchunk):
+    """Variant 3 (each lane updates one cell of its x-pair in every phase)
+    reads the same operands in the same order as variant 2: whole mixed
+    solves (FP32 sweeps inside FP64 defect correction) are bit-identical."""
+    from transportse_amd import _lib
+    f = Fields(*shape, seed=26)
+    sol = []
+    for var in (2, 3):
+        hip, d = _setup(f, 3600.0)
+        prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(ns=ns, by=32, ry=2,
+                                                          zchunk=zchunk, variant=var),
+                              inner=6 * ns, max_sweeps=12000, sor=1.9, tol=1e-10)
+        res = _lib.FastResult()
+        _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                            ctypes.byref(res)), "solve")
+        assert res.converged
+        sol.append((_result(hip, d), res.sweeps))
+    print("variant 3 vs 2", shape, (ns, zchunk), "sweeps", sol[0][1], sol[1][1],
+          "identical", np.array_equal(sol[0][0], sol[1][0]), flush=True)
+    assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
+
+
+def test_pair_variant_needs_even_nx_and_a_single_system():
+    from transportse_amd import _lib
+    from transportse_amd.batch import Batch
+    f = Fields(83, 41, 23, seed=27)
+    hip, d = _setup(f, 3600.0)
+    prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(ns=3, by=32, ry=2, zchunk=0,
+                                                      variant=3),
+                          inner=18, max_sweeps=12000, sor=1.9, tol=1e-10)
+    res = _lib.FastResult()
+    assert hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm), ctypes.byref(res)) < 0
+    assert b"even nx" in hip.lib.tse_last_error()
+    with pytest.raises(_lib.TseError):
+        Batch(0, "mixed", (3, 32, 2, 0, 3), inner=18, tol=1e-10, sor=1.9)
+
+
 DEGENERATE = [(51, 51, 1), (1, 1, 145), (200, 1, 60), (1, 20, 30), (3, 2, 1), (64, 33, 2)]
 
 

@@ -64,3 +64,6 @@ def test_variant2_plans_filtered_by_field_size():
     assert plan_fits(2, (1024, 1024, 256), True)          # exactly 1 GiB
     assert not plan_fits(2, (1024, 1024, 256), False)     # 2 GiB FP64
     assert plan_fits(1, (2048, 2048, 512), False)         # other variants: any size
+    assert plan_fits(3, (512, 512, 256), True)            # cell pairs: even nx
+    assert not plan_fits(3, (511, 512, 256), True)
+    assert not plan_fits(3, (1024, 1024, 512), True)      # 2 GiB

@@ -25,19 +25,24 @@ PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0), (1, 24, 2, 0),
 PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
                      (2, 24, 2, 1), (2, 32, 2, 1), (3, 16, 1, 1), (3, 24, 2, 1),
                      (2, 24, 2, 2), (2, 32, 2, 2), (3, 16, 1, 2), (3, 24, 2, 2),
-                     (3, 32, 2, 2)]
+                     (3, 32, 2, 2), (2, 32, 2, 3), (3, 32, 2, 3)]
+# the batched engine runs variants 0-2 (variant 3 is single-system only)
+BATCH_PLANS32 = [p for p in PLANS32 if p[3] <= 2]
 # transport: the cell's own coefficients stay in registers, one row per thread
 TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
               (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]
 TR_PLANS32 = TR_PLANS64 + [(3, 16, 1, 0), (3, 16, 1, 2)]
 ZCHUNKS = (0, 128, 64, 32)
-BUF_MAX = 1 << 30        # variant 2 reads fields through buffer descriptors
+BUF_MAX = 1 << 30        # variants 2 and 3 read fields through buffer descriptors
 
 
 def plan_fits(variant: int, shape, fp32: bool) -> bool:
-    """Variant 2 takes fields of at most 1 GiB (the library rejects larger)."""
+    """Variants 2 and 3 take fields of at most 1 GiB, variant 3 (cell pairs)
+    an even nx; the library rejects the others."""
     nx, ny, nz = shape
-    return variant != 2 or nx * ny * nz * (4 if fp32 else 8) <= BUF_MAX
+    if variant == 3 and nx % 2:
+        return False
+    return variant < 2 or nx * ny * nz * (4 if fp32 else 8) <= BUF_MAX
 
 KINDS = {
     "p": ("tse_fast_p_sweeps", PLANS64, PLANS32),

@@ -628,6 +628,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
     if (!fast_plan_ok(prm->plan, fp32, kind) || prm->inner < prm->plan.ns ||
         prm->inner % prm->plan.ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    if (prm->plan.variant > 2)
+        return fail(TSE_ERR_ARG, "the batched engine runs sweep variants 0-2");
     for (int s = 0; s < b->cap; ++s)
         if (b->hused[s]) return fail(TSE_ERR_STATE, "batch still has members");
     b->kind = kind;

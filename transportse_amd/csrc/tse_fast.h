@@ -583,6 +583,204 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     }
 }
 
+// Variant 3 (FP32, even nx): each thread holds an x-pair of cells (x, x+1,
+// x even) of one row, and each wavefront two rows of equal parity (block
+// row y of thread (lane, ty) is 4*(ty/2) + (ty&1) + 2*(lane/32)).  Phase h
+// at step k updates the cells with (i+j+k-1) even, so in a plane step every
+// phase updates the same element of every pair: element (j+k-1)&1.  The z
+// loop is unrolled by the even ring length, so that element is a compile-
+// time function of the row parity of the wavefront (template Q, chosen by
+// a uniform branch) and of the unrolled step; every lane updates one cell in
+// every phase, where the one-cell-per-lane variants leave half the lanes
+// idle.  Pairs are read and written with 8-byte buffer accesses.  The x
+// neighbour inside the pair is the other element, the one across the pair
+// boundary comes from the adjacent lane (DPP); lanes 0 and 32 start a row,
+// so the shifted-in west face coefficient is zeroed there.  Same operands
+// in the same order as the other variants: bit-identical results.
+typedef float tse_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ tse_f2 buf_ld2(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_bit_cast(tse_f2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+__device__ __forceinline__ void buf_st2(tse_f2 v, __amdgpu_buffer_rsrc_t r, uint32_t off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tse_u32x2, v), r, (int)off, 0, 0);
+}
+
+template <int NS, int BYT, int Q>
+__device__ __forceinline__ void
+rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pbytes,
+                   uint32_t ocol, uint32_t oout, const bool (&dm)[2][2 * NS],
+                   __amdgpu_buffer_rsrc_t rP, __amdgpu_buffer_rsrc_t rWx,
+                   __amdgpu_buffer_rsrc_t rWy, __amdgpu_buffer_rsrc_t rWz,
+                   __amdgpu_buffer_rsrc_t rI, __amdgpu_buffer_rsrc_t rB,
+                   __amdgpu_buffer_rsrc_t rOut, float omega,
+                   float (&sP)[2 * NS + 2][2 * BYT + 2][64],
+                   float (&sWy)[2 * NS + 2][2 * BYT + 2][64]) {
+    constexpr int R = 2 * NS;
+    constexpr int L = R + 2;
+    const int klast = k1 + R - 1;
+    const int y = ly + 1;             // LDS row
+    // LDS row r holds two 32-word blocks, element e of pair ip at word
+    // 32 * (e ^ ((r >> 1) & 1)) + ip: a half-wavefront touches 32 consecutive
+    // words and the two halves (rows r and r + 2) use different blocks
+    const int cm1[2] = {(((0 ^ ((y - 1) >> 1)) & 1) << 5) + ip, (((1 ^ ((y - 1) >> 1)) & 1) << 5) + ip};
+    const int c0[2] = {(((0 ^ (y >> 1)) & 1) << 5) + ip, (((1 ^ (y >> 1)) & 1) << 5) + ip};
+    const int cp1[2] = {(((0 ^ ((y + 1) >> 1)) & 1) << 5) + ip, (((1 ^ ((y + 1) >> 1)) & 1) << 5) + ip};
+    float p[2][L], rwz[2][L], rinv[2][L], rb[2][L], rwx[2][L], rwy[2][L];
+    tse_f2 np, nwz;
+#pragma unroll
+    for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int r = 0; r < L; ++r)
+            p[e][r] = rwz[e][r] = rinv[e][r] = rb[e][r] = rwx[e][r] = rwy[e][r] = 0.f;
+    {
+        const uint32_t o = ocol + (kl0 <= kl1 ? (uint32_t)kl0 * pbytes : 0x80000000u);
+        np = buf_ld2(rP, o);
+        nwz = buf_ld2(rWz, o);
+        const tse_f2 a = buf_ld2(rI, o), b = buf_ld2(rB, o), c = buf_ld2(rWx, o),
+                     d = buf_ld2(rWy, o);
+        rinv[0][0] = a.x; rinv[1][0] = a.y;
+        rb[0][0] = b.x;   rb[1][0] = b.y;
+        rwx[0][0] = c.x;  rwx[1][0] = c.y;
+        rwy[0][0] = d.x;  rwy[1][0] = d.y;
+    }
+    const bool row_start = ip == 0;   // lanes 0 and 32
+    for (int kb = kl0; kb <= klast; kb += L) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            const int k = kb + u;
+            if (k > klast) break;
+            const int un = (u + 1) % L;
+            p[0][u] = np.x;
+            p[1][u] = np.y;
+            rwz[0][u] = nwz.x;
+            rwz[1][u] = nwz.y;
+            sP[u][y][c0[0]] = np.x;
+            sP[u][y][c0[1]] = np.y;
+            sWy[u][y][c0[0]] = rwy[0][u];
+            sWy[u][y][c0[1]] = rwy[1][u];
+            {   // plane k+1 (reads 0 past kl1 and outside the grid)
+                const uint32_t o =
+                    ocol + (k + 1 <= kl1 ? (uint32_t)(k + 1) * pbytes : 0x80000000u);
+                np = buf_ld2(rP, o);
+                nwz = buf_ld2(rWz, o);
+                const tse_f2 a = buf_ld2(rI, o), b = buf_ld2(rB, o), c = buf_ld2(rWx, o),
+                             d = buf_ld2(rWy, o);
+                rinv[0][un] = a.x; rinv[1][un] = a.y;
+                rb[0][un] = b.x;   rb[1][un] = b.y;
+                rwx[0][un] = c.x;  rwx[1][un] = c.y;
+                rwy[0][un] = d.x;  rwy[1][un] = d.y;
+            }
+            __syncthreads();
+            const int eact = (Q + u) & 1;   // compile-time after unrolling
+#pragma unroll
+            for (int h = 0; h < R; ++h) {
+                const int sm = (u - 1 - h + 2 * L) % L;     // plane m = k-1-h
+                const int sa = (u - h + 2 * L) % L;         // plane m + 1
+                const int sb = (u - 2 - h + 2 * L) % L;     // plane m - 1
+                float pl, pr, wxl;
+                if (eact == 0) {
+                    pl = from_left(p[1][sm]);
+                    pr = p[1][sm];
+                    const float w = from_left(rwx[1][sm]);
+                    wxl = row_start ? 0.f : w;
+                } else {
+                    pl = p[0][sm];
+                    pr = from_right(p[0][sm]);
+                    wxl = rwx[0][sm];
+                }
+                if (dm[eact][h] && rinv[eact][sm] > 0.f) {
+                    float acc = rb[eact][sm];
+                    acc += rwx[eact][sm] * pr;
+                    acc += wxl * pl;
+                    acc += rwy[eact][sm] * sP[sm][y + 1][cp1[eact]];
+                    acc += sWy[sm][y - 1][cm1[eact]] * sP[sm][y - 1][cm1[eact]];
+                    acc += rwz[eact][sm] * p[eact][sa];
+                    acc += rwz[eact][sb] * p[eact][sb];
+                    const float pc = p[eact][sm];
+                    const float pn = pc + omega * (acc * rinv[eact][sm] - pc);
+                    p[eact][sm] = pn;
+                    sP[sm][y][c0[eact]] = pn;
+                }
+            }
+            const int mo = k - R;
+            const uint32_t oo = mo >= k0 && mo < k1 ? (uint32_t)mo * pbytes : 0x80000000u;
+            const int so = (u - R + 2 * L) % L;
+            tse_f2 out;
+            out.x = p[0][so];
+            out.y = p[1][so];
+            buf_st2(out, rOut, oout + oo);
+        }
+    }
+}
+
+template <int NS, int BYT>
+__device__ __forceinline__ void
+rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                  float *__restrict__ Pout, const float *__restrict__ Wx,
+                  const float *__restrict__ Wy, const float *__restrict__ Wz,
+                  const float *__restrict__ invD, const float *__restrict__ rhs,
+                  float omega, int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int BY = 2 * BYT;
+    constexpr int L = R + 2;
+    static_assert(BY - 2 * R >= 1, "tile too small");
+    __shared__ float sP[L][BY + 2][BX];
+    __shared__ float sWy[L][BY + 2][BX];
+
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const Span sx = tile_span(g.edge, tx, g.tiles_x, g.nx, BX, R);
+    const Span sy = tile_span(g.edge, ty, g.tiles_y, g.ny, BY, R);
+    const int ip = threadIdx.x & 31;
+    const int ly = ((threadIdx.y >> 1) << 2) + (threadIdx.y & 1) + ((threadIdx.x >> 5) << 1);
+    const int i = sx.x0 + 2 * ip;     // even (nx even, so every tile starts at an even x)
+    const int j = sy.x0 + ly;
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const uint32_t pbytes = (uint32_t)g.nx * (uint32_t)g.ny * 4u;
+    const uint32_t fbytes = pbytes * (uint32_t)g.nz;
+    const auto rP = buf_rsrc(Pin, zero_init ? 0u : fbytes);
+    const auto rWx = buf_rsrc(Wx, fbytes), rWy = buf_rsrc(Wy, fbytes);
+    const auto rWz = buf_rsrc(Wz, fbytes), rI = buf_rsrc(invD, fbytes);
+    const auto rB = buf_rsrc(rhs, fbytes);
+    const auto rOut = buf_rsrc(Pout, fbytes);
+    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const int dy = min(sy.lfree ? kFar : ly, sy.hfree ? kFar : BY - 1 - ly);
+    bool dm[2][R];
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const int xt = 2 * ip + e;
+        const int d = min(dy, min(sx.lfree ? kFar : xt, sx.hfree ? kFar : BX - 1 - xt));
+#pragma unroll
+        for (int h = 0; h < R; ++h) dm[e][h] = in_dom && d >= h + 1;
+    }
+    const uint32_t ocol = in_dom ? (uint32_t)(i + g.nx * j) * 4u : kBufMax;
+    // pairs owned by this tile (lo and hi are even)
+    const bool own = in_dom && i >= sx.lo && i < sx.hi && j >= sy.lo && j < sy.hi;
+    const uint32_t oout = own ? ocol : kBufMax;
+    if (threadIdx.y == 0)
+#pragma unroll
+        for (int u = 0; u < L; ++u) sP[u][0][threadIdx.x] = sWy[u][0][threadIdx.x] = 0.f;
+    if (threadIdx.y == BYT - 1)
+#pragma unroll
+        for (int u = 0; u < L; ++u)
+            sP[u][BY + 1][threadIdx.x] = sWy[u][BY + 1][threadIdx.x] = 0.f;
+    // active element of the first step: (j + kl0 - 1) & 1, the same for the
+    // whole wavefront (its rows have one parity)
+    const int q = __builtin_amdgcn_readfirstlane((j + kl0 - 1) & 1);
+    if (q == 0)
+        rb_zmarch_x2_march<NS, BYT, 0>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout, dm,
+                                       rP, rWx, rWy, rWz, rI, rB, rOut, omega, sP, sWy);
+    else
+        rb_zmarch_x2_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout, dm,
+                                       rP, rWx, rWy, rWz, rI, rB, rOut, omega, sP, sWy);
+}
+
 struct FastOp {  // natural-layout operator arrays
     double *Wx, *Wy, *Wz, *D, *invD, *b;
     float *Wx32, *Wy32, *Wz32, *invD32;
@@ -980,6 +1178,7 @@ FastSys *system_of(tse_ctx *ctx, int kind);
 int ensure_sys(tse_ctx *ctx, FastSys *s);
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind);
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32);
+bool fast_plan_grid_ok(const tse_sweep_plan &pl, int nx);
 ZGeo make_zgeo(const Geo &G, int ns, int by, int zchunk, int edge);
 
 }  // namespace tse

@@ -68,6 +68,19 @@ rb_zmarch_xr_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                       omega, zero_init);
 }
 
+template <typename T, int NS, int BYT, int RY>
+__global__ void __launch_bounds__(64 * BYT)
+rb_zmarch_x2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
+                    const T *__restrict__ Wx, const T *__restrict__ Wy,
+                    const T *__restrict__ Wz, const T *__restrict__ invD,
+                    const T *__restrict__ rhs, T omega, int zero_init) {
+    static_assert(sizeof(T) == 4 && RY == 2, "variant 3: FP32, two cells per thread");
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    rb_zmarch_tile_x2<NS, BYT>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
+                               zero_init);
+}
+
 // ---- operator assembly (from split-layout fields) ---------------------------------------
 
 __device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
@@ -501,6 +514,8 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
         TSE_ZM(2, rb_zmarch_xr_kernel, 3, 16, 1)
         TSE_ZM(2, rb_zmarch_xr_kernel, 3, 12, 2)
         TSE_ZM(2, rb_zmarch_xr_kernel, 3, 16, 2)
+        TSE_ZM(3, rb_zmarch_x2_kernel, 2, 16, 2)
+        TSE_ZM(3, rb_zmarch_x2_kernel, 3, 16, 2)
     }
 #undef TSE_ZM
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
@@ -535,7 +550,11 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan
 }
 
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
-    return pl.variant != 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
+    return pl.variant < 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
+}
+
+bool fast_plan_grid_ok(const tse_sweep_plan &pl, int nx) {
+    return pl.variant != 3 || nx % 2 == 0;   // variant 3 reads cell pairs
 }
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
@@ -552,7 +571,7 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
                                  {0, 3, 16, 1}, {1, 2, 24, 2}, {1, 2, 32, 2},
                                  {1, 3, 16, 1}, {1, 3, 24, 2}, {2, 2, 24, 2},
                                  {2, 2, 32, 2}, {2, 3, 16, 1}, {2, 3, 24, 2},
-                                 {2, 3, 32, 2}};
+                                 {2, 3, 32, 2}, {3, 2, 32, 2}, {3, 3, 32, 2}};
     auto hit = [&](const int c[4]) {
         return pl.variant == c[0] && pl.ns == c[1] && pl.by == c[2] && pl.ry == c[3];
     };
@@ -666,9 +685,11 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     if (!fast_plan_ok(*plan, fp32, s->kind) || n_sweeps < 0 || n_sweeps % plan->ns)
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
     if (!fast_plan_fits(*plan, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
+        return fail(TSE_ERR_ARG, "sweep variants 2 and 3 need fields of at most 1 GiB");
+    if (!fast_plan_grid_ok(*plan, ctx->geo.nx))
+        return fail(TSE_ERR_ARG, "sweep variant 3 needs an even nx");
     const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk,
-                              plan->variant == 2);
+                              plan->variant >= 2);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
     int e;  // FP32 timing path: correction sweeps on the last residual
     return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
@@ -689,8 +710,10 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (prm->cycle_mode != TSE_CYCLE_FIXED && prm->cycle_mode != TSE_CYCLE_ADAPTIVE)
         return fail(TSE_ERR_ARG, "cycle_mode must be TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE");
     if (!fast_plan_fits(pl, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
-    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant == 2);
+        return fail(TSE_ERR_ARG, "sweep variants 2 and 3 need fields of at most 1 GiB");
+    if (!fast_plan_grid_ok(pl, ctx->geo.nx))
+        return fail(TSE_ERR_ARG, "sweep variant 3 needs an even nx");
+    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant >= 2);
     double rn, bn;
     int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
                   : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);

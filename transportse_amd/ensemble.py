@@ -25,7 +25,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .autotune import PLANS32, PLANS64, TR_PLANS32, TR_PLANS64, plan_fits
+from .autotune import BATCH_PLANS32, PLANS64, TR_PLANS32, TR_PLANS64, plan_fits
 from .batch import KIND_P, KIND_TR, Batch
 from .scenario import DeviceScenario, ScenarioSpec
 
@@ -104,7 +104,7 @@ class Ensemble:
         res = {}
         with torch.cuda.stream(self.stream):
             for kind, plans, setup, sys_of, inner, sor in (
-                    (KIND_P, PLANS32 if fp32 else PLANS64,
+                    (KIND_P, BATCH_PLANS32 if fp32 else PLANS64,
                      lambda s: s.setup_p(), lambda s: s.fp, self.inner_p, self.sor_p),
                     (KIND_TR, TR_PLANS32 if fp32 else TR_PLANS64,
                      lambda s: s.setup_c(0), lambda s: s.ft, self.inner_tr,
