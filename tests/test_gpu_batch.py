@@ -198,3 +198,38 @@ def test_per_member_sor_and_rearm_many():
             assert np.array_equal(x, systems[n].result())
         print("per-member sor and rearm_many ok", flush=True)
         b.close()
+
+
+EVEN_SHAPES = [(36, 23, 19), (64, 40, 33), (20, 50, 17), (90, 31, 12), (46, 45, 45)]
+
+
+@pytest.mark.parametrize("plan", [(3, 32, 2, 0, 3), (2, 32, 2, 7, 3)])
+def test_batch_pair_variant_matches_single_solves(plan):
+    """Sweep variant 3 (cell pairs, even nx) in the persistent batch kernel:
+    each member bit-identical to its single-system solve, with and without
+    graph replay."""
+    import torch
+    from transportse_amd.batch import Batch
+    tol = 1e-10
+    with torch.cuda.stream(_stream()):
+        systems = [_pressure(sh, 70 + n) for n, sh in enumerate(EVEN_SHAPES)]
+        alone = []
+        for s in systems:
+            st = s.solve("mixed", tol=tol, sor=1.9, inner=18, plan=plan,
+                         max_sweeps=20000)
+            assert st.converged
+            alone.append((st, s.result()))
+            s.assemble()
+        for graph in (False, True):
+            b = Batch(0, "mixed", plan, inner=18, tol=tol, sor=1.9, max_sweeps=20000)
+            slots = [b.add(s) for s in systems]
+            b.run(graph=graph)
+            for (st, x), s, slot in zip(alone, systems, slots):
+                ms = b.status(slot)
+                b.writeback(slot)
+                assert ms.converged and ms.sweeps == st.sweeps
+                assert np.array_equal(x, s.result())
+            b.close()
+            for s in systems:
+                s.assemble()
+        print("batch variant 3", plan, "ok", flush=True)

@@ -214,9 +214,11 @@ def test_pair_variant_equals_unrolled_ring_variant(shape, ns, zchunk):
     assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
 
 
-def test_pair_variant_needs_even_nx_and_a_single_system():
+def test_pair_variant_needs_even_nx():
+    """An odd nx is refused by the single-system solve and by a batch."""
     from transportse_amd import _lib
     from transportse_amd.batch import Batch
+    from transportse_amd.pressure import FastPressure
     f = Fields(83, 41, 23, seed=27)
     hip, d = _setup(f, 3600.0)
     prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(ns=3, by=32, ry=2, zchunk=0,
@@ -225,8 +227,14 @@ def test_pair_variant_needs_even_nx_and_a_single_system():
     res = _lib.FastResult()
     assert hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm), ctypes.byref(res)) < 0
     assert b"even nx" in hip.lib.tse_last_error()
-    with pytest.raises(_lib.TseError):
-        Batch(0, "mixed", (3, 32, 2, 0, 3), inner=18, tol=1e-10, sor=1.9)
+    fp = FastPressure(*f.grid_args())
+    fp.load(f.P, 3600.0, **{n: getattr(f, n) for n in
+                            ("c_f", "nue", "rho_f", "phi", "Kx", "Ky", "Kz",
+                             "BP", "WP", "c_r")})
+    b = Batch(0, "mixed", (3, 32, 2, 0, 3), inner=18, tol=1e-10, sor=1.9)
+    with pytest.raises(_lib.TseError, match="even nx"):
+        b.add(fp)
+    b.close()
 
 
 DEGENERATE = [(51, 51, 1), (1, 1, 145), (200, 1, 60), (1, 20, 30), (3, 2, 1), (64, 33, 2)]

@@ -109,9 +109,14 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
         else if constexpr (VAR == 1)
             rb_zmarch_tile_xl<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
                                               rhs, omega, zero);
-        else
+        else if constexpr (VAR == 2)
             rb_zmarch_tile_xr<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz, inv,
                                               rhs, omega, zero);
+        else {
+            static_assert(sizeof(T) == 4 && RY == 2, "variant 3: FP32, cell pairs");
+            rb_zmarch_tile_x2<NS, BYT>(m.g, w.y, pin, pout, wx, wy, wz, inv, rhs,
+                                       omega, zero);
+        }
         __syncthreads();  // LDS reuse by the next tile
     }
 }
@@ -320,7 +325,7 @@ int grow_slots(tse_batch *b, int need) {
 
 ZGeo member_zgeo(const tse_batch *b, const tse_ctx *c) {
     return make_zgeo(c->geo, b->plan.ns, b->plan.by, b->plan.zchunk,
-                     b->plan.variant == 2);
+                     b->plan.variant >= 2);
 }
 
 // Rebuild work-item lists, member table and the persistent grid size.
@@ -440,6 +445,8 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_BP(2, 3, 16, 1)
         TSE_BP(2, 3, 12, 2)
         TSE_BP(2, 3, 16, 2)
+        TSE_BP(3, 2, 16, 2)
+        TSE_BP(3, 3, 16, 2)
         TSE_BT(0, 3, 16)
         TSE_BT(2, 3, 16)
     }
@@ -628,8 +635,6 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
     if (!fast_plan_ok(prm->plan, fp32, kind) || prm->inner < prm->plan.ns ||
         prm->inner % prm->plan.ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
-    if (prm->plan.variant > 2)
-        return fail(TSE_ERR_ARG, "the batched engine runs sweep variants 0-2");
     for (int s = 0; s < b->cap; ++s)
         if (b->hused[s]) return fail(TSE_ERR_STATE, "batch still has members");
     b->kind = kind;
@@ -654,7 +659,9 @@ int tse_batch_add(tse_batch *b, tse_ctx *c) {
     FastSys *s = member_sys(b, c);
     if (!s || !s->ready) return fail(TSE_ERR_STATE, "member's fast system not set up");
     if (!fast_plan_fits(b->plan, s->N, b->form == TSE_FORM_MIXED))
-        return fail(TSE_ERR_ARG, "sweep variant 2 needs fields of at most 1 GiB");
+        return fail(TSE_ERR_ARG, "sweep variants 2 and 3 need fields of at most 1 GiB");
+    if (!fast_plan_grid_ok(b->plan, c->geo.nx))
+        return fail(TSE_ERR_ARG, "sweep variant 3 needs an even nx");
     for (int q = 0; q < b->cap; ++q)
         if (b->hused[q] && b->ctx[q] == c)
             return fail(TSE_ERR_ARG, "context already in the batch");

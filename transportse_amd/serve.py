@@ -100,10 +100,16 @@ class ServeStats:
 
 class Server:
     def __init__(self, budget_bytes=200e9, form="mixed", tol=1e-10,
-                 plan_p=(3, 32, 2, 64, 2), plan_tr=(2, 16, 1, 64, 2), inner_p=51,
+                 plan_p=None, plan_tr=(2, 16, 1, 64, 2), inner_p=51,
                  inner_tr=6, sor_p=1.95, sor_tr=1.0, device=0,
                  max_sweeps=400000, keep_results=False):
+        """plan_p None: the cell-pair sweep (variant 3) when every request of
+        a run has an even nx, else the unrolled-ring sweep (variant 2)."""
         import torch
+        self.auto_plan_p = plan_p is None
+        if plan_p is None:
+            plan_p = (3, 32, 2, 64, 2)
+        self.p_args = (form, tol, inner_p, sor_p, max_sweeps)
         self.budget = float(budget_bytes)
         self.device = device
         self.stream = torch.cuda.Stream(device)   # default priority
@@ -119,6 +125,13 @@ class Server:
         time_scale); returns latency / throughput statistics."""
         import torch
         reqs = sorted(requests, key=lambda r: r.arrival)
+        if self.auto_plan_p:
+            even = all(r.spec.nx % 2 == 0 for r in reqs)
+            plan = (3, 32, 2, 64, 3 if even else 2)
+            bp = self.batches[KIND_P]
+            if bp.plan != plan and not bp.members:
+                form, tol, inner, sor, max_sweeps = self.p_args
+                bp.configure(form, plan, inner, tol, sor, max_sweeps)
         waiting: deque = deque()
         live = {}            # rid -> [request, scenario, generator, kind, slot]
         used = 0
