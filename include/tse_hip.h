@@ -189,14 +189,19 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  * 64 x by columns with ry tile rows per thread (blocks of 64 x by/ry
  * threads), zchunk z planes per block (<= 0: whole column), variant 0
  * (x neighbours through LDS), 1 (x neighbours exchanged across the lanes
- * of a wavefront; smaller LDS ring) or 2 (as 1, with the register rings
+ * of a wavefront; smaller LDS ring), 2 (as 1, with the register rings
  * rotated by unrolling and buffer loads; fields of at most 1 GiB, else
+ * TSE_ERR_ARG) or 3 (FP32 only, as 2 with an x-pair of cells per thread,
+ * so that every lane updates in every phase; nx must be even, else
  * TSE_ERR_ARG).  Supported (ns, by, ry):
  *   variant 0: (1,8,1) (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also
  *              (1,32,2) (2,32,2) (2,32,4) (3,16,1)
  *   variant 1: (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also (2,24,2)
  *              (2,32,2) (3,16,1) (3,24,2)
  *   variant 2: as variant 1, FP32 sweeps also (3,32,2)
+ *   variant 3: (2,32,2) (3,32,2)
+ * Batches (tse_batch_begin) also take variant 4 with (2,32,2) (3,32,2):
+ * variant 3 for members with an even nx, variant 2 for the others.
  * Transport systems: variant 0 or 2, (1,8,1) (1,16,1) (2,16,1), FP32
  * (3,16,1). */
 typedef struct {

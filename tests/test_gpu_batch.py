@@ -233,3 +233,35 @@ def test_batch_pair_variant_matches_single_solves(plan):
             for s in systems:
                 s.assemble()
         print("batch variant 3", plan, "ok", flush=True)
+
+
+@pytest.mark.parametrize("ns,zchunk", [(2, 0), (2, 7), (3, 0), (3, 9)])
+def test_batch_mixed_parity_variant(ns, zchunk):
+    """Batch variant 4: members with an even nx run the cell-pair sweep,
+    the others the unrolled-ring sweep, in one persistent launch; each is
+    bit-identical to its single-system solve (variants 2 and 3 agree bit
+    for bit)."""
+    import torch
+    from transportse_amd.batch import Batch
+    tol = 1e-10
+    shapes = [(36, 23, 19), (37, 23, 19), (64, 40, 33), (45, 45, 45), (90, 31, 12)]
+    with torch.cuda.stream(_stream()):
+        systems = [_pressure(sh, 80 + n) for n, sh in enumerate(shapes)]
+        alone = []
+        for s in systems:
+            st = s.solve("mixed", tol=tol, sor=1.9, inner=18, plan=(ns, 32, 2, zchunk, 2),
+                         max_sweeps=20000)
+            assert st.converged
+            alone.append((st, s.result()))
+            s.assemble()
+        b = Batch(0, "mixed", (ns, 32, 2, zchunk, 4), inner=18, tol=tol, sor=1.9,
+                  max_sweeps=20000)
+        slots = [b.add(s) for s in systems]
+        b.run(graph=True)
+        for (st, x), s, slot in zip(alone, systems, slots):
+            ms = b.status(slot)
+            b.writeback(slot)
+            assert ms.converged and ms.sweeps == st.sweeps
+            assert np.array_equal(x, s.result())
+        b.close()
+        print("batch variant 4", (ns, zchunk), "ok", flush=True)

@@ -27,8 +27,9 @@ PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
                      (2, 24, 2, 2), (2, 32, 2, 2), (3, 16, 1, 2), (3, 24, 2, 2),
                      (3, 32, 2, 2), (2, 32, 2, 3), (3, 32, 2, 3)]
 # the batched engine runs the same plans (plan_fits keeps variant 3 to
-# batches whose members all have an even nx)
-BATCH_PLANS32 = PLANS32
+# batches whose members all have an even nx) and variant 4: the cell-pair
+# sweep for members with an even nx, the unrolled-ring sweep for the others
+BATCH_PLANS32 = PLANS32 + [(2, 32, 2, 4), (3, 32, 2, 4)]
 # transport: the cell's own coefficients stay in registers, one row per thread
 TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
               (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]

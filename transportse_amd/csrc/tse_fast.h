@@ -440,13 +440,20 @@ __device__ __forceinline__ void buf_st(double v, __amdgpu_buffer_rsrc_t r, uint3
 // colour h&1, i.e. the cells with (i+j+k-1) even whatever h is, so the
 // colour test is a lane constant that alternates between steps.  Same
 // operands in the same order as rb_zmarch_tile_xl: bit-identical results.
+// LDS rings of the variant-2/3 tiles: [ring slot][row + 1][x]; rows 0 and
+// BY+1 are zero guards for the y neighbours of boundary rows in tiles whose
+// edge is the grid boundary.
+template <typename T, int NS, int BY>
+using ZRing = T[2 * NS + 2][BY + 2][64];
+
 template <typename T, int NS, int BYT, int RY>
 __device__ __forceinline__ void
-rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
-                  T *__restrict__ Pout, const T *__restrict__ Wx,
-                  const T *__restrict__ Wy, const T *__restrict__ Wz,
-                  const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
-                  int zero_init) {
+rb_zmarch_tile_xr_in(const ZGeo &g, int tile, const T *__restrict__ Pin,
+                     T *__restrict__ Pout, const T *__restrict__ Wx,
+                     const T *__restrict__ Wy, const T *__restrict__ Wz,
+                     const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
+                     int zero_init, ZRing<T, NS, BYT * RY> &sP,
+                     ZRing<T, NS, BYT * RY> &sWy) {
     constexpr int R = 2 * NS;
     constexpr int BX = 64;
     constexpr int BY = BYT * RY;
@@ -454,10 +461,6 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
     constexpr int TY = BY - 2 * R;
     constexpr int L = R + 2;
     static_assert(TY >= 1, "tile too small");
-    // LDS rows are shifted by one: rows 0 and BY+1 are zero guards for the
-    // y neighbours of boundary rows in tiles whose edge is the grid boundary
-    __shared__ T sP[L][BY + 2][BX];
-    __shared__ T sWy[L][BY + 2][BX];
 
     const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
     const int tx = tile - zc * g.tiles_x;
@@ -581,6 +584,18 @@ rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
                 buf_st(p[q][(u - R + 2 * L) % L], rOut, oout[q] + oo);
         }
     }
+}
+
+template <typename T, int NS, int BYT, int RY>
+__device__ __forceinline__ void
+rb_zmarch_tile_xr(const ZGeo &g, int tile, const T *__restrict__ Pin,
+                  T *__restrict__ Pout, const T *__restrict__ Wx,
+                  const T *__restrict__ Wy, const T *__restrict__ Wz,
+                  const T *__restrict__ invD, const T *__restrict__ rhs, T omega,
+                  int zero_init) {
+    __shared__ ZRing<T, NS, BYT * RY> sP, sWy;
+    rb_zmarch_tile_xr_in<T, NS, BYT, RY>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs,
+                                         omega, zero_init, sP, sWy);
 }
 
 // Variant 3 (FP32, even nx): each thread holds an x-pair of cells (x, x+1,
@@ -715,18 +730,17 @@ rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
 
 template <int NS, int BYT>
 __device__ __forceinline__ void
-rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
-                  float *__restrict__ Pout, const float *__restrict__ Wx,
-                  const float *__restrict__ Wy, const float *__restrict__ Wz,
-                  const float *__restrict__ invD, const float *__restrict__ rhs,
-                  float omega, int zero_init) {
+rb_zmarch_tile_x2_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                     float *__restrict__ Pout, const float *__restrict__ Wx,
+                     const float *__restrict__ Wy, const float *__restrict__ Wz,
+                     const float *__restrict__ invD, const float *__restrict__ rhs,
+                     float omega, int zero_init, ZRing<float, NS, 2 * BYT> &sP,
+                     ZRing<float, NS, 2 * BYT> &sWy) {
     constexpr int R = 2 * NS;
     constexpr int BX = 64;
     constexpr int BY = 2 * BYT;
     constexpr int L = R + 2;
     static_assert(BY - 2 * R >= 1, "tile too small");
-    __shared__ float sP[L][BY + 2][BX];
-    __shared__ float sWy[L][BY + 2][BX];
 
     const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
     const int tx = tile - zc * g.tiles_x;
@@ -779,6 +793,18 @@ rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
     else
         rb_zmarch_x2_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout, dm,
                                        rP, rWx, rWy, rWz, rI, rB, rOut, omega, sP, sWy);
+}
+
+template <int NS, int BYT>
+__device__ __forceinline__ void
+rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                  float *__restrict__ Pout, const float *__restrict__ Wx,
+                  const float *__restrict__ Wy, const float *__restrict__ Wz,
+                  const float *__restrict__ invD, const float *__restrict__ rhs,
+                  float omega, int zero_init) {
+    __shared__ ZRing<float, NS, 2 * BYT> sP, sWy;
+    rb_zmarch_tile_x2_in<NS, BYT>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
+                                  zero_init, sP, sWy);
 }
 
 struct FastOp {  // natural-layout operator arrays
