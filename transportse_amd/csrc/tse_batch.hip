@@ -114,8 +114,8 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
                                               rhs, omega, zero);
         else if constexpr (VAR == 3) {
             static_assert(sizeof(T) == 4 && RY == 2, "variant 3: FP32, cell pairs");
-            rb_zmarch_tile_x2<NS, BYT>(m.g, w.y, pin, pout, wx, wy, wz, inv, rhs,
-                                       omega, zero);
+            rb_zmarch_tile_x2<NS, BYT, false>(m.g, w.y, pin, pout, wx, wy, wz, inv, rhs,
+                                              omega, zero);
         } else {
             // variant 4 (batches only): members with an even nx take the
             // cell-pair sweep, the others the unrolled-ring sweep, on one set
@@ -123,8 +123,8 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
             static_assert(sizeof(T) == 4 && RY == 2, "variant 4: FP32, 2 rows");
             __shared__ ZRing<float, NS, BYT * RY> sP, sWy;
             if (m.nx % 2 == 0)
-                rb_zmarch_tile_x2_in<NS, BYT>(m.g, w.y, pin, pout, wx, wy, wz, inv,
-                                              rhs, omega, zero, sP, sWy);
+                rb_zmarch_tile_x2_in<NS, BYT, false>(m.g, w.y, pin, pout, wx, wy, wz,
+                                                     inv, rhs, omega, zero, sP, sWy);
             else
                 rb_zmarch_tile_xr_in<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz,
                                                      inv, rhs, omega, zero, sP, sWy);

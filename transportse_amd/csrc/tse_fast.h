@@ -620,7 +620,7 @@ __device__ __forceinline__ void buf_st2(tse_f2 v, __amdgpu_buffer_rsrc_t r, uint
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(tse_u32x2, v), r, (int)off, 0, 0);
 }
 
-template <int NS, int BYT, int Q>
+template <int NS, int BYT, int Q, bool BF>
 __device__ __forceinline__ void
 rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pbytes,
                    uint32_t ocol, uint32_t oout, const bool (&dm)[2][2 * NS],
@@ -703,7 +703,8 @@ rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                     pr = from_right(p[0][sm]);
                     wxl = rwx[0][sm];
                 }
-                if (dm[eact][h] && rinv[eact][sm] > 0.f) {
+                const bool go = dm[eact][h] && rinv[eact][sm] > 0.f;
+                auto update = [&]() {
                     float acc = rb[eact][sm];
                     acc += rwx[eact][sm] * pr;
                     acc += wxl * pl;
@@ -712,7 +713,18 @@ rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                     acc += rwz[eact][sm] * p[eact][sa];
                     acc += rwz[eact][sb] * p[eact][sb];
                     const float pc = p[eact][sm];
-                    const float pn = pc + omega * (acc * rinv[eact][sm] - pc);
+                    return pc + omega * (acc * rinv[eact][sm] - pc);
+                };
+                if constexpr (BF) {
+                    // branch-free: every lane computes and cells that must not
+                    // change keep their value, so a phase's LDS reads need not
+                    // wait for the branch of the phase before (more registers)
+                    const float pn = update();
+                    const float pv = go ? pn : p[eact][sm];
+                    p[eact][sm] = pv;
+                    sP[sm][y][c0[eact]] = pv;
+                } else if (go) {
+                    const float pn = update();
                     p[eact][sm] = pn;
                     sP[sm][y][c0[eact]] = pn;
                 }
@@ -728,7 +740,7 @@ rb_zmarch_x2_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
     }
 }
 
-template <int NS, int BYT>
+template <int NS, int BYT, bool BF>
 __device__ __forceinline__ void
 rb_zmarch_tile_x2_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
                      float *__restrict__ Pout, const float *__restrict__ Wx,
@@ -788,14 +800,19 @@ rb_zmarch_tile_x2_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
     // whole wavefront (its rows have one parity)
     const int q = __builtin_amdgcn_readfirstlane((j + kl0 - 1) & 1);
     if (q == 0)
-        rb_zmarch_x2_march<NS, BYT, 0>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout, dm,
-                                       rP, rWx, rWy, rWz, rI, rB, rOut, omega, sP, sWy);
+        rb_zmarch_x2_march<NS, BYT, 0, BF>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout,
+                                           dm, rP, rWx, rWy, rWz, rI, rB, rOut, omega,
+                                           sP, sWy);
     else
-        rb_zmarch_x2_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout, dm,
-                                       rP, rWx, rWy, rWz, rI, rB, rOut, omega, sP, sWy);
+        rb_zmarch_x2_march<NS, BYT, 1, BF>(ly, ip, k0, k1, kl0, kl1, pbytes, ocol, oout,
+                                           dm, rP, rWx, rWy, rWz, rI, rB, rOut, omega,
+                                           sP, sWy);
 }
 
-template <int NS, int BYT>
+// BF: branch-free phases (the single-system kernel: 485 against 503 us per
+// 512x512x256 launch, with 8 bytes of scratch); the batch kernels keep the
+// branch, where the branch-free form spills more and runs slower.
+template <int NS, int BYT, bool BF>
 __device__ __forceinline__ void
 rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
                   float *__restrict__ Pout, const float *__restrict__ Wx,
@@ -803,8 +820,8 @@ rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
                   const float *__restrict__ invD, const float *__restrict__ rhs,
                   float omega, int zero_init) {
     __shared__ ZRing<float, NS, 2 * BYT> sP, sWy;
-    rb_zmarch_tile_x2_in<NS, BYT>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
-                                  zero_init, sP, sWy);
+    rb_zmarch_tile_x2_in<NS, BYT, BF>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
+                                      zero_init, sP, sWy);
 }
 
 struct FastOp {  // natural-layout operator arrays

@@ -77,8 +77,8 @@ rb_zmarch_x2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
     static_assert(sizeof(T) == 4 && RY == 2, "variant 3: FP32, two cells per thread");
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;
-    rb_zmarch_tile_x2<NS, BYT>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
-                               zero_init);
+    rb_zmarch_tile_x2<NS, BYT, true>(g, tile, Pin, Pout, Wx, Wy, Wz, invD, rhs, omega,
+                                     zero_init);
 }
 
 // ---- operator assembly (from split-layout fields) ---------------------------------------
