@@ -169,6 +169,13 @@ int tse_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
  * reduction (rounding differs from Numba's sequential sum by O(1e-16)). */
 int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
                 double *out);
+/* The same reduction, enqueued without waiting: the result is read by
+ * tse_reduction_result(ctx, out, 1).  No counterpart in the reference; for
+ * callers that drive many contexts on one stream and wait once for all. */
+int tse_l2_norm_async(tse_ctx *ctx, const double *New, const double *Old);
+/* Waits for the context's stream and returns the last enqueued reduction
+ * (tse_l2_norm_async: n = 1; tse_mass_sums_async: n = 3). */
+int tse_reduction_result(tse_ctx *ctx, double *out, int32_t n);
 
 /* l2_norm on host arrays of n doubles with Numba's sequential C-order sums
  * (used by the SIA density test, tse.py:1426). Returns the norm, or 1.0 and
@@ -333,6 +340,9 @@ int tse_linear_eos(tse_ctx *ctx, const double *C, const double *T,
  * rate of C through the two x faces (upwind, face area dy*dz)]. */
 int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
                   const double *phi, const double *qx, double *out3);
+/* The same sums enqueued without waiting (tse_reduction_result, n = 3). */
+int tse_mass_sums_async(tse_ctx *ctx, const double *C, const double *rho_f,
+                        const double *phi, const double *qx);
 /* peclet_C (tse.py:333-360): for cells with BC == 0,
  * PeC[f*3 + a] = |q_a| / max_s D_a[f*nspec + s] * d_a (NaN where that
  * maximum is 0); other cells keep their PeC.  PeC and D_fx/y/z are device

@@ -108,6 +108,8 @@ SIGNATURES = {
     "tse_compute_t": (_I, [_V, ctypes.POINTER(TFields),
                            ctypes.POINTER(SorParams), ctypes.POINTER(Result)]),
     "tse_l2_norm": (_I, [_V, _dp, _dp, _DPH]),
+    "tse_l2_norm_async": (_I, [_V, _dp, _dp]),
+    "tse_reduction_result": (_I, [_V, _DPH, ctypes.c_int32]),
     "tse_smolarkiewicz": (_I, [_V] + [_dp] * 7 + [_D, _D, _D]),
     "tse_host_l2_norm_seq": (_D, [_DPH, _DPH, _L, ctypes.POINTER(ctypes.c_int32)]),
     "tse_p_sweeps": (_I, [_V, ctypes.POINTER(PFields), _D, _D, ctypes.c_int32,
@@ -147,6 +149,7 @@ SIGNATURES = {
     "tse_fast_mark_loaded": (_I, [_V, ctypes.c_int32]),
     "tse_fast_current": (_I, [_V, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)]),
     "tse_mass_sums": (_I, [_V, _dp, _dp, _dp, _dp, _DPH]),
+    "tse_mass_sums_async": (_I, [_V, _dp, _dp, _dp, _dp]),
     "tse_peclet_c": (_I, [_V, _dp, _dp, _dp, _dp, _dp, _dp, ctypes.c_int32, _dp, _dp,
                           _DPH]),
     "tse_peclet_t": (_I, [_V, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp, _dp,

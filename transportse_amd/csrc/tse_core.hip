@@ -488,18 +488,31 @@ int tse_flow_error(tse_ctx *ctx, const double *P, const double *Pn,
                              dt_P, e);
 }
 
-int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
-                double *out) {
+int tse_l2_norm_async(tse_ctx *ctx, const double *New, const double *Old) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if (!New || !Old || !out) return fail(TSE_ERR_ARG, "null pointer");
+    if (!New || !Old) return fail(TSE_ERR_ARG, "null pointer");
     rc = launch_l2(ctx, New, Old);
     if (rc) return rc;
     TSE_HIP_TRY(hipMemcpyAsync(ctx->red_host, ctx->red_out, 2 * sizeof(double),
                                hipMemcpyDeviceToHost, ctx->stream));
-    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    *out = ctx->red_host[0];
     return 0;
+}
+
+int tse_reduction_result(tse_ctx *ctx, double *out, int32_t n) {
+    if (!ctx) return fail(TSE_ERR_ARG, "null context");
+    if (!out || n < 1 || n > 3) return fail(TSE_ERR_ARG, "n must be 1..3");
+    TSE_HIP_TRY(hipSetDevice(ctx->device));
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < n; ++c) out[c] = ctx->red_host[c];
+    return 0;
+}
+
+int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
+                double *out) {
+    if (!out) return fail(TSE_ERR_ARG, "null pointer");
+    const int rc = tse_l2_norm_async(ctx, New, Old);
+    return rc ? rc : tse_reduction_result(ctx, out, 1);
 }
 
 double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,

@@ -310,9 +310,16 @@ int tse_linear_eos(tse_ctx *ctx, const double *C, const double *T, double *rho_f
 
 int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
                   const double *phi, const double *qx, double *out3) {
+    if (!out3) return fail(TSE_ERR_ARG, "null argument");
+    const int rc = tse_mass_sums_async(ctx, C, rho_f, phi, qx);
+    return rc ? rc : tse_reduction_result(ctx, out3, 3);
+}
+
+int tse_mass_sums_async(tse_ctx *ctx, const double *C, const double *rho_f,
+                        const double *phi, const double *qx) {
     int rc = check_ctx(ctx);
     if (rc) return rc;
-    if (!C || !rho_f || !phi || !qx || !out3) return fail(TSE_ERR_ARG, "null argument");
+    if (!C || !rho_f || !phi || !qx) return fail(TSE_ERR_ARG, "null argument");
     const Geo &G = ctx->geo;
     const long F = 2L * G.S;
     const int nb = (int)std::min<long>((F + kBlock - 1) / kBlock, 1024);
@@ -326,8 +333,6 @@ int tse_mass_sums(tse_ctx *ctx, const double *C, const double *rho_f,
     TSE_HIP_TRY(hipGetLastError());
     TSE_HIP_TRY(hipMemcpyAsync(ctx->red_host, ctx->red_out, 3 * sizeof(double),
                                hipMemcpyDeviceToHost, ctx->stream));
-    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-    for (int c = 0; c < 3; ++c) out3[c] = ctx->red_host[c];
     return 0;
 }
 

@@ -27,7 +27,7 @@ import numpy as np
 
 from .autotune import BATCH_PLANS32, PLANS64, TR_PLANS32, TR_PLANS64, plan_fits
 from .batch import KIND_P, KIND_TR, Batch
-from .scenario import DeviceScenario, ScenarioSpec
+from .scenario import DeviceScenario, ScenarioSpec, resume_all
 
 
 def ensemble_specs(n: int = 64, seed: int = 4, species: int = 1,
@@ -170,7 +170,8 @@ class Ensemble:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             gens = {i: s.steps(nsteps) for i, s in enumerate(self.scn)}
-            pend = {i: next(g) for i, g in gens.items()}
+            pend = {i: it for i, it in resume_all(gens, gens).items()
+                    if it is not None}
             rounds = solves = cycles = 0
             cell_sweeps = 0.0
             sweeps = {KIND_P: 0, KIND_TR: 0}
@@ -203,11 +204,11 @@ class Ensemble:
                         sweeps[kind] += st.sweeps
                         cell_sweeps += st.sweeps * self.scn[i].spec.cells
                         solves += 1
-                    for i in ids:
-                        try:
-                            pend[i] = next(gens[i])
-                        except StopIteration:
+                    for i, it in resume_all(gens, ids).items():
+                        if it is None:
                             del pend[i]
+                        else:
+                            pend[i] = it
             torch.cuda.synchronize()
             sec = time.perf_counter() - t0
         return EnsembleStats(sec, rounds, solves, cell_sweeps, sweeps[KIND_P],
@@ -226,7 +227,8 @@ class Ensemble:
             torch.cuda.synchronize()
             t0 = time.perf_counter()
             gens = {i: s.steps(nsteps) for i, s in enumerate(self.scn)}
-            pend = {i: next(g) for i, g in gens.items()}
+            pend = {i: it for i, it in resume_all(gens, gens).items()
+                    if it is not None}
 
             def enter(i, kind, sysm, rearm):
                 b = batches[kind]
@@ -253,6 +255,7 @@ class Ensemble:
                     b.cycle(graph=self.graph)
                 cycles += 1
                 rearm = {KIND_P: [], KIND_TR: []}
+                done = []
                 for i in list(pend):
                     kind = pend[i][0]
                     b = batches[kind]
@@ -269,12 +272,13 @@ class Ensemble:
                     sweeps[kind] += st.sweeps
                     cell_sweeps += st.sweeps * self.scn[i].spec.cells
                     solves += 1
-                    try:
-                        pend[i] = next(gens[i])
-                    except StopIteration:
+                    done.append(i)
+                for i, it in resume_all(gens, done).items():
+                    if it is None:
                         del pend[i]
                         continue
-                    enter(i, pend[i][0], pend[i][1], rearm)
+                    pend[i] = it
+                    enter(i, it[0], it[1], rearm)
                 for kind, lst in rearm.items():
                     batches[kind].rearm_many(lst)
             torch.cuda.synchronize()

@@ -37,6 +37,34 @@ from .workloads import hydrostatic_P, lognormal_K
 
 EOS = dict(rho0=1000.0, drho_dc=700.0, lf0=2.5, dlf_dt=0.002, t_ref=10.0)
 
+# DeviceScenario.steps() yields (KIND_WAIT, None) where it next reads a
+# device reduction; the solve kinds 0 (pressure) and 1 (transport) are
+# batch.KIND_P and batch.KIND_TR
+KIND_WAIT = 2
+
+
+def resume_all(gens: dict, ids) -> dict:
+    """next() on the step generators gens[i] for i in ids, resuming those
+    that stop at a reduction read (KIND_WAIT) until each yields a solve or
+    ends.  Every member's reduction is enqueued before the first read waits,
+    so the members share one wait.  Returns {i: (kind, system) or None}."""
+    out = {}
+    todo = list(ids)
+    while todo:
+        again = []
+        for i in todo:
+            try:
+                item = next(gens[i])
+            except StopIteration:
+                out[i] = None
+                continue
+            if item[0] == KIND_WAIT:
+                again.append(i)
+            else:
+                out[i] = item
+        todo = again
+    return out
+
 
 @dataclass
 class ScenarioSpec:
@@ -271,6 +299,25 @@ class DeviceScenario:
                                           self._p(self.qx), out), "tse_mass_sums")
         return tuple(out)
 
+    # the same two reductions enqueued without waiting; steps() yields
+    # (KIND_WAIT, None) after each, so a driver of many scenarios on one
+    # stream waits once for all of them, then reads them with _reduction
+    def _density_norm_start(self):
+        _lib.check(self.lib.tse_l2_norm_async(self.hip.ctx, self._p(self.rho),
+                                              self._p(self.rho_prev)),
+                   "tse_l2_norm_async")
+
+    def _mass_start(self):
+        _lib.check(self.lib.tse_mass_sums_async(self.hip.ctx, self._p(self.C[0]),
+                                                self._p(self.rho), self._p(self.u["phi"]),
+                                                self._p(self.qx)), "tse_mass_sums_async")
+
+    def _reduction(self, n):
+        out = (ctypes.c_double * n)()
+        _lib.check(self.lib.tse_reduction_result(self.hip.ctx, out, n),
+                   "tse_reduction_result")
+        return tuple(out)
+
     def _current(self, kind):
         """The iterate buffer a fast system's result is read from: graphs of
         the result conversions are kept per buffer."""
@@ -327,7 +374,9 @@ class DeviceScenario:
 
     def steps(self, nsteps: int):
         """Generator over ``nsteps`` SIA steps; yields (kind, system) for each
-        solve and resumes once the system holds its solution."""
+        solve and resumes once the system holds its solution, and
+        (KIND_WAIT, None) where it is about to read a reduction from the
+        device (resume at once, or after a wait that covers its stream)."""
         sp = self.spec
         for _ in range(nsteps):
             st = StepStats()
@@ -351,11 +400,15 @@ class DeviceScenario:
                     self._segment(f"finish_t{self._current(1)}",
                                   lambda: self.finish_tr(self.T))
                 self._segment("eos", self.eos_update)
-                diff = self.density_norm()
+                self._density_norm_start()
+                yield KIND_WAIT, None
+                diff = self._reduction(1)[0]
                 if diff < sp.sia_conv or it == sp.sia_maxit - 1:
                     break
                 self._segment("sia_reset", self._sia_reset)
-            st.mass = self.mass()
+            self._mass_start()
+            yield KIND_WAIT, None
+            st.mass = self._reduction(3)
             self.t += sp.dt
             self.n_steps += 1
             self.history.append(st)
@@ -371,6 +424,8 @@ class DeviceScenario:
         batched drivers use fixed cycles)."""
         out = []
         for kind, sysm in self.steps(nsteps):
+            if kind == KIND_WAIT:
+                continue
             if kind == 0:
                 om = self.omega_p.omega if sor_p is None else sor_p
                 r = sysm.solve(form, tol=tol, sor=om, inner=inner_p,

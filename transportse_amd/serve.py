@@ -29,7 +29,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .batch import KIND_P, KIND_TR, Batch
-from .scenario import DeviceScenario, ScenarioSpec, field_bytes
+from .scenario import DeviceScenario, ScenarioSpec, field_bytes, resume_all
 
 GRIDS = ((64, 64, 64), (128, 128, 128), (256, 256, 128))
 
@@ -161,7 +161,7 @@ class Server:
                     scn = DeviceScenario(r.spec, self.device, fields=r.fields)
                     r.fields = None
                     gen = scn.steps(r.steps)
-                    kind, sysm = next(gen)
+                    kind, sysm = resume_all({0: gen}, [0])[0]
                     slots = {KIND_P: None, KIND_TR: None}
                     self._enter(scn, kind, sysm, slots)
                     live[r.rid] = [r, scn, gen, kind, slots, need]
@@ -179,6 +179,7 @@ class Server:
                 # advance requests whose solve finished; their re-arms go to
                 # each batch at once after the pass
                 rearm = {KIND_P: [], KIND_TR: []}
+                fin = []
                 for rid in list(live):
                     r, scn, gen, kind, slots, need = live[rid]
                     b = self.batches[kind]
@@ -193,9 +194,12 @@ class Server:
                         scn.omega_p.report(st.sweeps)
                     r.solves += 1
                     r.sweeps += st.sweeps
-                    try:
-                        kind, sysm = next(gen)
-                    except StopIteration:
+                    fin.append(rid)
+                nxt_items = resume_all({rid: live[rid][2] for rid in fin}, fin)
+                for rid in fin:
+                    r, scn, gen, kind, slots, need = live[rid]
+                    it = nxt_items[rid]
+                    if it is None:
                         for k, sl in slots.items():
                             if sl is not None:
                                 self.batches[k].remove(sl)
@@ -208,6 +212,7 @@ class Server:
                         del live[rid]
                         done.append(r)
                         continue
+                    kind, sysm = it
                     self._enter(scn, kind, sysm, slots, rearm)
                     live[rid][3] = kind
                 for k, lst in rearm.items():
