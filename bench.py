@@ -99,7 +99,7 @@ def main(argv=None):
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--a-steps", type=int, default=100)
     ap.add_argument("--a-cpu-steps", type=int, default=10)
-    ap.add_argument("--b-steps", type=int, default=2)
+    ap.add_argument("--b-steps", type=int, default=20)
     ap.add_argument("--no-sia", action="store_true")
     ap.add_argument("--sia-steps", type=int, default=1)
     ap.add_argument("--no-ensemble", action="store_true")

@@ -1,9 +1,11 @@
 """GPU: the benchmark configs through the SIA driver, against the oracle.
 
-The salt config at its full size (64x32x32) and the salt + heat SIA physics
-at 64x64x32 (stretched z, slope, salt + heat, dispersion, SIA) run through
-the same driver twice -- GPU backend and oracle backend -- and every field of
-every step must be bit-identical, with equal iteration counts and times.
+The salt config at its full size (64x32x32, 25 of its 100 steps) and the
+salt + heat SIA physics at 64x64x32 (stretched z, slope, salt + heat,
+dispersion, SIA; the config's 20 time steps, about a minute of oracle time
+on 16 host threads) run through the same driver twice -- GPU backend and
+oracle backend -- and every field of every step must be bit-identical, with
+equal iteration counts and times.
 The salt + heat SIA config at full size (256x256x128) runs on the GPU alone:
 each step must converge, and the mass-balance output must be consistent.
 """
@@ -27,7 +29,7 @@ def _run(cfg, backend, steps, tmp):
 
 
 @pytest.mark.parametrize("cfg,steps", [("salt_64x32x32.py", 25),
-                                       ("salt_heat_sia_64x64x32.py", 2)])
+                                       ("salt_heat_sia_64x64x32.py", 20)])
 def test_config_matches_oracle(cfg, steps, tmp_path):
     from transportse_amd.driver import make_hip_backend
     th = _run(cfg, make_hip_backend, steps, tmp_path)
