@@ -60,8 +60,11 @@ class HipContext:
         self.nxh = (self.nx + 1) // 2
         self._staging = None
         self._up = None        # device staging of uploads
-        self._pin = None       # pinned host staging (both directions)
-        self._pin_free = None  # event: last copy reading / writing _pin done
+        # two pinned host staging buffers (both directions), used in turn,
+        # each with the event of the last copy that reads or writes it
+        self._pins = [None, None]
+        self._pin_ev = [None, None]
+        self._pin_turn = 0
 
     def close(self):
         if getattr(self, "ctx", None):
@@ -101,31 +104,38 @@ class HipContext:
         # contiguous block (e.g. a species slice of a C-order 4-D array)
         base, strides = _block_and_strides(a)
         n = base.size
-        pin = self._pinned(n)
-        np.copyto(pin.numpy()[:n], base)
         if self._up is None or self._up.numel() < n:
             self._up = torch.empty(n, dtype=torch.float64, device=self.device)
         if out is None:
             out = self.empty()
         self.sync_stream()
         stream = torch.cuda.current_stream(self.device)
-        self._up[:n].copy_(pin[:n], non_blocking=True)   # DMA from pinned memory
+        turn, pin = self._pinned(n)
+        pin_np = pin.numpy()
+        # chunked: the DMA of one chunk from pinned memory overlaps the host
+        # copy of the next into it
+        for lo, hi in _chunks(n, n):
+            np.copyto(pin_np[lo:hi], base[lo:hi])
+            self._up[lo:hi].copy_(pin[lo:hi], non_blocking=True)
         _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), self._up.data_ptr(),
                                           *strides), "tse_to_split")
-        self._pin_free = torch.cuda.Event()
-        self._pin_free.record(stream)
+        ev = torch.cuda.Event()
+        ev.record(stream)
+        self._pin_ev[turn] = ev
         return out
 
     def _pinned(self, n: int):
-        """The pinned host staging buffer (one per context, reused), once the
-        last copy that read it has finished."""
+        """The next pinned staging buffer of at least n doubles, once the
+        last copy that used it has finished: (index, buffer)."""
         torch = _torch()
-        if self._pin_free is not None:
-            self._pin_free.synchronize()
-            self._pin_free = None
-        if self._pin is None or self._pin.numel() < n:
-            self._pin = torch.empty(n, dtype=torch.float64, pin_memory=True)
-        return self._pin
+        turn = self._pin_turn
+        self._pin_turn ^= 1
+        if self._pin_ev[turn] is not None:
+            self._pin_ev[turn].synchronize()
+            self._pin_ev[turn] = None
+        if self._pins[turn] is None or self._pins[turn].numel() < n:
+            self._pins[turn] = torch.empty(n, dtype=torch.float64, pin_memory=True)
+        return turn, self._pins[turn]
 
     def download(self, field, out: np.ndarray | None = None) -> np.ndarray:
         """Copy a split field into a host (nx,ny,nz) array (new, or `out`)."""
@@ -138,17 +148,35 @@ class HipContext:
         _lib.check(self.lib.tse_from_split(self.ctx, st.data_ptr(),
                                             self.ny * self.nz, self.nz, 1,
                                             field.data_ptr()), "tse_from_split")
-        pin = self._pinned(n)
-        pin[:n].copy_(st, non_blocking=True)            # DMA into pinned memory
-        torch.cuda.current_stream(self.device).synchronize()
+        _, pin = self._pinned(n)
+        stream = torch.cuda.current_stream(self.device)
+        # chunks of whole i-planes: the DMA of one chunk into pinned memory
+        # overlaps the host copy of the one before out of it
+        plane = self.ny * self.nz
+        parts = []
+        for i0, i1 in _chunks(self.nx, n):
+            pin[i0 * plane:i1 * plane].copy_(st[i0 * plane:i1 * plane], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+            parts.append((i0, i1, ev))
         host = pin[:n].numpy().reshape(self.shape)
         if out is None:
-            return host.copy()
-        out[...] = host
+            out = np.empty(self.shape, dtype=np.float64)
+        for i0, i1, ev in parts:
+            ev.synchronize()
+            out[i0:i1] = host[i0:i1]
         return out
 
     def synchronize(self):
         _lib.check(self.lib.tse_ctx_synchronize(self.ctx), "tse_ctx_synchronize")
+
+
+def _chunks(m: int, n: int):
+    """Split range(m) into up to 8 consecutive parts, each standing for at
+    least 2^20 of the n elements moved (one part for small copies)."""
+    k = max(1, min(8, n >> 20, m))
+    step = -(-m // k)
+    return [(lo, min(lo + step, m)) for lo in range(0, m, step)]
 
 
 def _block_and_strides(a: np.ndarray):
