@@ -107,7 +107,7 @@ def main(argv=None):
     ap.add_argument("--ens-steps", type=int, default=50)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+",
-                    default=[5.0, 50.0, 200.0])
+                    default=[5.0, 10.0, 20.0, 50.0, 100.0, 200.0])
     ap.add_argument("--serve-requests", type=int, default=12)
     ap.add_argument("--serve-steps", type=int, default=10)
     args = ap.parse_args(argv)
