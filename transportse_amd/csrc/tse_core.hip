@@ -64,34 +64,68 @@ int ensure_mask(tse_ctx *ctx, size_t elems) {
 
 // ---- layout conversion ---------------------------------------------------------------------
 
-__global__ void __launch_bounds__(kBlock)
+// Host layout (element strides si, sj, sk; C order has k fastest) <-> split
+// layout (x fastest) as a transpose through LDS: a block takes a 64 (i) x 64
+// (k) tile of one y row j, so host-side accesses run along k and split-side
+// accesses along i (two colour streams of 32 consecutive slots per
+// wavefront).  i runs to 2*nxh: for odd nx, cell i = nx stands for the pad
+// slot of its colour, which holds 0.
+constexpr int kTT = 64;
+
+// Block b takes row j = b / (nti * ntk) and tile t = b % (nti * ntk) of
+// nti x ntk tiles (i major).
+__device__ inline void tile_origin(const Geo &G, int &i0, int &k0, int &j) {
+    const int nti = (2 * G.nxh + kTT - 1) / kTT, ntk = (G.nz + kTT - 1) / kTT;
+    const int b = blockIdx.x, t = b % (nti * ntk);
+    j = b / (nti * ntk);
+    i0 = (t / ntk) * kTT;
+    k0 = (t % ntk) * kTT;
+}
+
+__global__ void __launch_bounds__(256)
 to_split_kernel(Geo G, double *__restrict__ dst, const double *__restrict__ src,
                 long si, long sj, long sk) {
-    const long F = 2L * G.S;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
-         t += (long)gridDim.x * blockDim.x) {
-        const int c = t >= G.S ? 1 : 0;
-        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
-        dst[t] = q.valid ? src[q.i * si + q.j * sj + q.k * sk] : 0.0;
+    __shared__ double tile[kTT][kTT + 1];
+    int i0, k0, j;
+    tile_origin(G, i0, k0, j);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    for (int ii = ty; ii < kTT; ii += 4) {
+        const int i = i0 + ii, k = k0 + tx;
+        if (i < G.nx && k < G.nz) tile[ii][tx] = src[i * si + j * sj + k * sk];
+    }
+    __syncthreads();
+    const int i = i0 + tx;
+    if (i < 2 * G.nxh) {
+        for (int kk = ty; kk < kTT && k0 + kk < G.nz; kk += 4)
+            dst[slot(G, i, j, k0 + kk)] = i < G.nx ? tile[tx][kk] : 0.0;
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(256)
 from_split_kernel(Geo G, double *__restrict__ dst, long si, long sj, long sk,
                   const double *__restrict__ src) {
-    const long F = 2L * G.S;
-    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
-         t += (long)gridDim.x * blockDim.x) {
-        const int c = t >= G.S ? 1 : 0;
-        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
-        if (q.valid) dst[q.i * si + q.j * sj + q.k * sk] = src[t];
+    __shared__ double tile[kTT][kTT + 1];
+    int i0, k0, j;
+    tile_origin(G, i0, k0, j);
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i = i0 + tx;
+    if (i < G.nx) {
+        for (int kk = ty; kk < kTT && k0 + kk < G.nz; kk += 4)
+            tile[tx][kk] = src[slot(G, i, j, k0 + kk)];
+    }
+    __syncthreads();
+    for (int ii = ty; ii < kTT; ii += 4) {
+        const int ic = i0 + ii, k = k0 + tx;
+        if (ic < G.nx && k < G.nz) dst[ic * si + j * sj + k * sk] = tile[ii][tx];
     }
 }
 
-static inline int conv_grid(long n) {
-    long b = (n + kBlock - 1) / kBlock;
-    return (int)std::max(1L, std::min(b, 16384L));
+// Launch shape of the two permutation kernels.
+static inline dim3 perm_grid(const Geo &G) {
+    const long nti = (2L * G.nxh + kTT - 1) / kTT, ntk = (G.nz + kTT - 1) / kTT;
+    return dim3((unsigned)(nti * ntk * G.ny));
 }
+
 
 // ---- SOR driver loop shared by compute_P / compute_C / compute_T --------------------------
 
@@ -392,8 +426,7 @@ int tse_to_split(tse_ctx *ctx, double *dst_split, const double *src,
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!dst_split || !src) return fail(TSE_ERR_ARG, "null pointer");
-    const long F = 2L * ctx->geo.S;
-    hipLaunchKernelGGL(to_split_kernel, dim3(conv_grid(F)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(to_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
                        ctx->stream, ctx->geo, dst_split, src, (long)si,
                        (long)sj, (long)sk);
     TSE_HIP_TRY(hipGetLastError());
@@ -405,8 +438,7 @@ int tse_from_split(tse_ctx *ctx, double *dst, int64_t si, int64_t sj,
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!dst || !src_split) return fail(TSE_ERR_ARG, "null pointer");
-    const long F = 2L * ctx->geo.S;
-    hipLaunchKernelGGL(from_split_kernel, dim3(conv_grid(F)), dim3(kBlock), 0,
+    hipLaunchKernelGGL(from_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
                        ctx->stream, ctx->geo, dst, (long)si, (long)sj,
                        (long)sk, src_split);
     TSE_HIP_TRY(hipGetLastError());
