@@ -120,6 +120,33 @@ from_split_kernel(Geo G, double *__restrict__ dst, long si, long sj, long sk,
     }
 }
 
+// Small fields (a few tiles, mostly empty): one thread per slot instead.
+constexpr long kFlatPermSlots = 1L << 20;
+
+__global__ void __launch_bounds__(kBlock)
+to_split_flat_kernel(Geo G, double *__restrict__ dst, const double *__restrict__ src,
+                     long si, long sj, long sk) {
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        dst[t] = q.valid ? src[q.i * si + q.j * sj + q.k * sk] : 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+from_split_flat_kernel(Geo G, double *__restrict__ dst, long si, long sj, long sk,
+                       const double *__restrict__ src) {
+    const long F = 2L * G.S;
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < F;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        if (q.valid) dst[q.i * si + q.j * sj + q.k * sk] = src[t];
+    }
+}
+
 // Launch shape of the two permutation kernels.
 static inline dim3 perm_grid(const Geo &G) {
     const long nti = (2L * G.nxh + kTT - 1) / kTT, ntk = (G.nz + kTT - 1) / kTT;
@@ -426,9 +453,15 @@ int tse_to_split(tse_ctx *ctx, double *dst_split, const double *src,
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!dst_split || !src) return fail(TSE_ERR_ARG, "null pointer");
-    hipLaunchKernelGGL(to_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
-                       ctx->stream, ctx->geo, dst_split, src, (long)si,
-                       (long)sj, (long)sk);
+    const long F = 2L * ctx->geo.S;
+    if (F < kFlatPermSlots)
+        hipLaunchKernelGGL(to_split_flat_kernel, dim3((F + kBlock - 1) / kBlock),
+                           dim3(kBlock), 0, ctx->stream, ctx->geo, dst_split, src,
+                           (long)si, (long)sj, (long)sk);
+    else
+        hipLaunchKernelGGL(to_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
+                           ctx->stream, ctx->geo, dst_split, src, (long)si,
+                           (long)sj, (long)sk);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -438,9 +471,15 @@ int tse_from_split(tse_ctx *ctx, double *dst, int64_t si, int64_t sj,
     int rc = check_ctx(ctx);
     if (rc) return rc;
     if (!dst || !src_split) return fail(TSE_ERR_ARG, "null pointer");
-    hipLaunchKernelGGL(from_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
-                       ctx->stream, ctx->geo, dst, (long)si, (long)sj,
-                       (long)sk, src_split);
+    const long F = 2L * ctx->geo.S;
+    if (F < kFlatPermSlots)
+        hipLaunchKernelGGL(from_split_flat_kernel, dim3((F + kBlock - 1) / kBlock),
+                           dim3(kBlock), 0, ctx->stream, ctx->geo, dst, (long)si,
+                           (long)sj, (long)sk, src_split);
+    else
+        hipLaunchKernelGGL(from_split_kernel, perm_grid(ctx->geo), dim3(kTT, 4), 0,
+                           ctx->stream, ctx->geo, dst, (long)si, (long)sj,
+                           (long)sk, src_split);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
 }
