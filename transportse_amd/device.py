@@ -59,7 +59,7 @@ class HipContext:
         self.F = int(self.lib.tse_field_elems(self.ctx))
         self.nxh = (self.nx + 1) // 2
         self._staging = None
-        self._up = None        # device staging of uploads
+        self._up = [None, None]  # device staging of uploads, one per pinned buffer
         # two pinned host staging buffers (both directions), used in turn,
         # each with the event of the last copy that reads or writes it
         self._pins = [None, None]
@@ -104,20 +104,23 @@ class HipContext:
         # contiguous block (e.g. a species slice of a C-order 4-D array)
         base, strides = _block_and_strides(a)
         n = base.size
-        if self._up is None or self._up.numel() < n:
-            self._up = torch.empty(n, dtype=torch.float64, device=self.device)
         if out is None:
             out = self.empty()
         self.sync_stream()
         stream = torch.cuda.current_stream(self.device)
         turn, pin = self._pinned(n)
+        # the device staging of this pinned buffer: the event waited for in
+        # _pinned also covers the last tse_to_split that read it, on any stream
+        up = self._up[turn]
+        if up is None or up.numel() < n:
+            up = self._up[turn] = torch.empty(n, dtype=torch.float64, device=self.device)
         pin_np = pin.numpy()
         # chunked: the DMA of one chunk from pinned memory overlaps the host
         # copy of the next into it
         for lo, hi in _chunks(n, n):
             np.copyto(pin_np[lo:hi], base[lo:hi])
-            self._up[lo:hi].copy_(pin[lo:hi], non_blocking=True)
-        _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), self._up.data_ptr(),
+            up[lo:hi].copy_(pin[lo:hi], non_blocking=True)
+        _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), up.data_ptr(),
                                           *strides), "tse_to_split")
         ev = torch.cuda.Event()
         ev.record(stream)
