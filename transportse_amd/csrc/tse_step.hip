@@ -162,13 +162,30 @@ peclet_kernel(Geo G, const double *__restrict__ dxyz, const double *__restrict__
               const double *__restrict__ B, double *__restrict__ Pe,
               double *__restrict__ partials) {
     const double *dx = dxyz, *dy = dxyz + G.nx, *dz = dxyz + G.nx + G.ny;
-    const long N = (long)G.nx * G.ny * G.nz;
     double m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY;
-    for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < N;
-         f += (long)gridDim.x * blockDim.x) {
+    // peclet_T reads nine split-layout fields per cell and writes three host
+    // doubles: cells go in split-slot order (x fastest).  peclet_C reads
+    // 3*nspec host-layout D values per cell: host order (k fastest).  The
+    // maxima do not depend on the order.
+    const long nit = HEAT ? 2L * G.S : (long)G.nx * G.ny * G.nz;
+    for (long u = blockIdx.x * (long)blockDim.x + threadIdx.x; u < nit;
+         u += (long)gridDim.x * blockDim.x) {
         int i, j, k;
-        unflat(G, f, i, j, k);
-        const long t = slot(G, i, j, k);
+        long f, t;
+        if (HEAT) {
+            const int c = u >= G.S ? 1 : 0;
+            const Cell q = decode(G, c, (int)(u - (long)c * G.S));
+            if (!q.valid) continue;
+            i = q.i;
+            j = q.j;
+            k = q.k;
+            f = ((long)i * G.ny + j) * G.nz + k;
+            t = u;
+        } else {
+            f = u;
+            unflat(G, f, i, j, k);
+            t = slot(G, i, j, k);
+        }
         double v0, v1, v2;
         if (B[t] == 0) {
             if (HEAT) {
@@ -210,13 +227,16 @@ cfl_kernel(Geo G, const double *__restrict__ dxyz, const double *__restrict__ qx
            const double *__restrict__ qy, const double *__restrict__ qz,
            const double *__restrict__ phi, double *__restrict__ partials) {
     const double *dx = dxyz, *dy = dxyz + G.nx, *dz = dxyz + G.nx + G.ny;
-    const long N = (long)G.nx * G.ny * G.nz;
     double m0 = -INFINITY, m1 = -INFINITY, m2 = -INFINITY;
-    for (long f = blockIdx.x * (long)blockDim.x + threadIdx.x; f < N;
-         f += (long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        unflat(G, f, i, j, k);
-        const long t = slot(G, i, j, k);
+    // every operand is a split-layout field: cells in split-slot order, each
+    // with its host C-order index f for the reshaped spacing
+    for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < 2L * G.S;
+         t += (long)gridDim.x * blockDim.x) {
+        const int c = t >= G.S ? 1 : 0;
+        const Cell q = decode(G, c, (int)(t - (long)c * G.S));
+        if (!q.valid) continue;
+        const int k = q.k;
+        const long f = ((long)q.i * G.ny + q.j) * G.nz + k;
         m0 = max_nan(m0, fabs(qx[t]) / phi[t] / dx[f % G.nx]);
         m1 = max_nan(m1, fabs(qy[t]) / phi[t] / dy[f % G.ny]);
         m2 = max_nan(m2, fabs(qz[t]) / phi[t] / dz[k]);
