@@ -11,7 +11,8 @@ in T, lognormal permeability, geometric z stretch and slope angle beta.
 When the environment variable TSE_GOLDEN_DUMP names a directory, the
 ``dynamic_bc`` callback (called once per time step, tse.py:1449-1453) saves
 the solver state of the running ``__main__`` module there; it changes
-nothing otherwise.
+nothing otherwise, unless ``dyn_inplace`` asks it to scale phi and nue in
+place.
 """
 import os
 import sys
@@ -26,7 +27,8 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
           dt0=86400.0, dt_max=86400.0, beta=5.0, stretch=1.05,
           maxit_p=0, maxit_c=0, maxit_t=0, disp=(1.0, 0.1), wp_cell=True,
           plot_steps=0, c_f=4.5e-10, salt_patch=True, t_end=None,
-          smolar_c=False, smolar_t=False, nlte=False, bp_faces="x"):
+          smolar_c=False, smolar_t=False, nlte=False, bp_faces="x",
+          dyn_inplace=False):
     rng = numpy.random.default_rng(seed)
     S = species
     ns.update(dict(
@@ -158,6 +160,11 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
             for name in ('dt_P', 'dt_C', 'dt_T'):
                 rec[name] = getattr(m, name, numpy.nan)
             numpy.savez(os.path.join(out, 'step%05d.npz' % m.n_tsteps), **rec)
+        if dyn_inplace:
+            # arrays the callback receives but does not return, changed in
+            # place: a caller that caches device copies must see the change
+            phi *= 1.02
+            nue *= 0.99
         return bc_inc, P, T, C, BP, BT, BC, WP, WT, WC, Kx, Ky, Kz
 
     ns['fluid_properties'] = fluid_properties

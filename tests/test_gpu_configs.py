@@ -67,3 +67,25 @@ def test_config_b_full_size(tmp_path):
         assert abs(m[0] - m0) <= 0.05 * m0
     print("salt_heat_sia_256x256x128: P iterations", th[-1]["numitP"], "C",
           th[-1]["numitC"], "T", th[-1]["numitT"], flush=True)
+
+
+def test_dynamic_bc_in_place_changes_reach_the_device(tmp_path):
+    """A dynamic_bc that changes phi and nue in place (arrays it is handed
+    but does not return): the GPU backend re-uploads them and every field of
+    every step equals the oracle backend's, which keeps no device copies."""
+    from helpers import GOLDEN
+    from transportse_amd.driver import Simulation, make_hip_backend
+    cfg = os.path.join(GOLDEN, "configs", "x_dynbc_inplace.py")
+    runs = []
+    for be in (make_hip_backend, oracle_backend):
+        sim = Simulation(cfg, be, verbose=False, record_fields=True,
+                         out_dir=str(tmp_path))
+        runs.append(sim.run())
+    th, to = runs
+    assert len(th) == len(to) == 4
+    for s, (a, b) in enumerate(zip(th, to)):
+        for k in ("P", "C", "T", "qx", "qy", "qz", "e", "rho_f", "mass"):
+            assert np.array_equal(a[k], b[k]), (s, k)
+        for k in ("numitP", "numitC", "numitT", "t_sim", "dt"):
+            assert a[k] == b[k], (s, k)
+    print("dynamic_bc in place: bit-identical over 4 steps", flush=True)

@@ -54,3 +54,22 @@ def test_chunks_cover_the_range_once():
         assert all(lo < hi for lo, hi in parts)
         if n < 1 << 21:
             assert len(parts) == 1
+
+
+def test_strided_views_of_foreign_buffers_take_the_copy_path(tmp_path):
+    """A species slice of an array whose base chain ends in something other
+    than an ndarray (np.frombuffer over bytes, np.memmap over a file) is
+    gathered through a contiguous copy instead of failing on the walk."""
+    import numpy as np
+
+    from transportse_amd.device import _block_and_strides
+    shp = (4, 3, 5, 2)
+    ref = np.arange(np.prod(shp), dtype=np.float64).reshape(shp)
+    raw = np.frombuffer(ref.tobytes(), dtype=np.float64).reshape(shp)
+    mm = np.memmap(tmp_path / "f.bin", dtype=np.float64, mode="w+", shape=shp)
+    mm[...] = ref
+    for a in (raw, mm, ref):
+        view = a[:, :, :, 1]
+        flat, (si, sj, sk) = _block_and_strides(view)
+        i, j, k = np.indices(view.shape)
+        assert np.array_equal(flat[i * si + j * sj + k * sk], ref[:, :, :, 1])

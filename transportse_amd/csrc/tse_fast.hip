@@ -28,6 +28,7 @@
 #include <cmath>
 #include <vector>
 
+#include "tse_alloc.h"
 #include "tse_fast.h"
 
 namespace tse {
@@ -419,25 +420,40 @@ int ensure_sys(tse_ctx *ctx, FastSys *s) {
             hipHostMalloc((void **)&s->red_host, sizeof(double) * 2) != hipSuccess)
             return fail(TSE_ERR_NOMEM, "fast-form scratch allocation failed");
     }
-    if (s->N == N && s->buf64) return 0;
-    if (s->buf64) {
+    if (s->N == N && s->buf64 && s->buf32) return 0;
+    if (s->buf64 || s->buf32) {
         TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
-        TSE_HIP_TRY(hipFree(s->buf64));
-        TSE_HIP_TRY(hipFree(s->buf32));
-        s->buf64 = nullptr;
-        s->buf32 = nullptr;
+        if (s->buf64) TSE_HIP_TRY(hipFree(s->buf64));
+        if (s->buf32) TSE_HIP_TRY(hipFree(s->buf32));
     }
-    s->N = N;
+    // "not allocated" until both arrays exist: a failed allocation leaves
+    // N = 0 and both pointers null, so a retry allocates again
+    s->buf64 = nullptr;
+    s->buf32 = nullptr;
+    s->N = 0;
+    s->ld = 0;
+    s->ready = false;
     // the arrays are 1088 elements apart beyond a multiple of 64 (FP32:
     // 4.25 KiB, FP64: 8.5 KiB), so that equal offsets in different arrays
     // do not fall on the same HBM channel when N is a large power of two
     // (the 512x512x256 FP32 pressure sweep: 604 -> 577 us per launch)
-    s->ld = ((N + 63) / 64) * 64 + 1088;
-    if (hipMalloc((void **)&s->buf64, sizeof(double) * (s->n64() + 2) * s->ld) != hipSuccess)
-        return fail(TSE_ERR_NOMEM, "fast-form FP64 operator allocation failed");
-    if (hipMalloc((void **)&s->buf32, sizeof(float) * (s->n32() + 2) * s->ld) != hipSuccess)
-        return fail(TSE_ERR_NOMEM, "fast-form FP32 operator allocation failed");
-    s->ready = false;
+    const long ld = ((N + 63) / 64) * 64 + 1088;
+    void *b64 = nullptr, *b32 = nullptr;
+    const bool ok = alloc_both(
+        &b64, sizeof(double) * (s->n64() + 2) * ld, &b32,
+        sizeof(float) * (s->n32() + 2) * ld,
+        [](void **p, size_t n) {
+            if (hipMalloc(p, n) == hipSuccess) return true;
+            (void)hipGetLastError();
+            *p = nullptr;
+            return false;
+        },
+        [](void *p) { (void)hipFree(p); });
+    if (!ok) return fail(TSE_ERR_NOMEM, "fast-form operator allocation failed");
+    s->buf64 = (double *)b64;
+    s->buf32 = (float *)b32;
+    s->ld = ld;
+    s->N = N;
     return 0;
 }
 

@@ -192,8 +192,10 @@ def _block_and_strides(a: np.ndarray):
     """
     if a.flags["C_CONTIGUOUS"]:
         return a.reshape(-1), (a.shape[1] * a.shape[2], a.shape[2], 1)
+    # walk to the outermost ndarray; a chain that reaches another kind of
+    # buffer (memmap file, bytes, mmap) stops there and takes the copy path
     base = a.base
-    while base is not None and base.base is not None:
+    while isinstance(base, np.ndarray) and isinstance(base.base, np.ndarray):
         base = base.base
     if (isinstance(base, np.ndarray) and base.flags["C_CONTIGUOUS"]
             and base.dtype == np.float64 and all(s > 0 for s in a.strides)

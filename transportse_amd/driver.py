@@ -470,8 +470,10 @@ class Simulation:
                         st.BT, st.BC, st.WP, st.WT, st.WC, st.Kx, st.Ky, st.Kz,
                         st.rho_f, st.nue, st.c_f, st.cp_f, st.qx, st.phi,
                         st.l_f, st.l_r)
-                self._host_mutated([st.P, st.T, st.C, st.BP, st.BT, st.BC, st.WP,
-                                    st.WT, st.WC, st.Kx, st.Ky, st.Kz])
+                # tse.py:1451-1453 also hands rho_f, nue, c_f, cp_f, qx, phi,
+                # l_f and l_r to the callback, which may change any of them in
+                # place: every device copy is stale from here on
+                self._host_mutated(None)
                 if bc_inc_ != bc_inc and c.FLUIDPROP:
                     self._host_mutated(self.update_fluid_properties())
 

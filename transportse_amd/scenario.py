@@ -130,19 +130,44 @@ def host_fields(spec: ScenarioSpec) -> dict:
     return f
 
 
-# uniform fields, shared by every scenario of the same grid shape
+# uniform fields, shared by every open scenario of the same grid shape on a
+# device: key -> [tensor, number of scenarios holding it]; the last scenario
+# of a shape to close drops it (shared_bytes counts what is held)
 UNIFORM = dict(phi=0.2, c_f=4.5e-10, c_r=1e-10, nue=1e-3, cp_f=4150.0,
                l_r=2.0, cp_r=1000.0, rho_r=2500.0, R=1.0, D_f=1e-9, zero=0.0)
 _shared: dict = {}
 
 
-def _uniform(hip: HipContext, name: str, value: float):
+def _acquire_uniform(hip: HipContext, name: str, value: float):
+    """(key, tensor) of a shared uniform field; the caller holds one
+    reference until it calls _release_uniform(key)."""
     key = (hip.device_index, hip.shape, name, float(value))
-    t = _shared.get(key)
-    if t is None:
-        t = hip.upload(np.full(hip.shape, float(value)))
-        _shared[key] = t
-    return t
+    ent = _shared.get(key)
+    if ent is None:
+        ent = _shared[key] = [hip.upload(np.full(hip.shape, float(value))), 0]
+    ent[1] += 1
+    return key, ent[0]
+
+
+def _release_uniform(key) -> None:
+    ent = _shared.get(key)
+    if ent is None:
+        return
+    ent[1] -= 1
+    if ent[1] <= 0:
+        del _shared[key]
+
+
+def shared_bytes() -> int:
+    """Device bytes held by the shared uniform fields."""
+    return sum(ent[0].numel() * ent[0].element_size() for ent in _shared.values())
+
+
+def uniform_bytes(spec: "ScenarioSpec") -> int:
+    """Upper bound of the shared uniform fields a scenario of this grid
+    adds when it is the first of its shape (UNIFORM plus the two
+    dispersivities)."""
+    return (len(UNIFORM) + 2) * 16 * ((spec.nx + 1) // 2) * spec.ny * spec.nz
 
 
 def field_bytes(spec: ScenarioSpec) -> int:
@@ -184,7 +209,8 @@ class DeviceScenario:
                               spec.beta, device)
         hip = self.hip
         hf = fields if fields is not None else host_fields(spec)
-        u = {k: _uniform(hip, k, v) for k, v in UNIFORM.items()}
+        self._ukeys = {}
+        u = {k: self._uniform(k, v) for k, v in UNIFORM.items()}
         self.u = u
         self.K = hip.upload(hf["K"])
         self.P = hip.upload(hf["P"])
@@ -272,8 +298,15 @@ class DeviceScenario:
         _lib.check(self.lib.tse_fast_t_setup(self.hip.ctx, ctypes.byref(f), sp.dt,
                                              sp.omega), "tse_fast_t_setup")
 
+    def _uniform(self, name, value):
+        """A shared uniform field, referenced once per scenario."""
+        k = (name, float(value))
+        if k not in self._ukeys:
+            self._ukeys[k] = _acquire_uniform(self.hip, name, value)
+        return self._ukeys[k][1]
+
     def _disp(self, n):
-        return _uniform(self.hip, f"disp{n}", self.spec.disp[n])
+        return self._uniform(f"disp{n}", self.spec.disp[n])
 
     def finish_tr(self, dst):
         _lib.check(self.lib.tse_fast_tr_result(self.hip.ctx, self._p(dst)),
@@ -512,4 +545,9 @@ class DeviceScenario:
         for g in self._graphs.values():
             self.lib.tse_graph_destroy(g)
         self._graphs.clear()
+        self.hip.sync_stream()
+        self.u = {}
+        for key, _ in self._ukeys.values():
+            _release_uniform(key)
+        self._ukeys.clear()
         self.hip.close()
