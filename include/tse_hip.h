@@ -135,6 +135,20 @@ int tse_from_split(tse_ctx *ctx, double *dst, int64_t si, int64_t sj,
 int tse_compute_p(tse_ctx *ctx, const tse_p_fields *f,
                   const tse_sor_params *prm, tse_result *out);
 
+/* compute_P as tse_compute_p (faithful form only), for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  It
+ * calls no hipMalloc / hipFree: the coefficient records and cell mask live
+ * in a fixed device array of 64 MiB that the library holds on each GPU it
+ * runs on until the process ends (grids up to about 6.4e5 cells: 104 B of
+ * records + 1 mask byte per cell; larger grids return TSE_ERR_ARG).  The
+ * context must have been created and its grid set beforehand (those calls
+ * allocate).  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU, from any context or thread, race and must not be
+ * issued.  Results are those of tse_compute_p, bit for bit. */
+int tse_compute_p_noalloc(tse_ctx *ctx, const tse_p_fields *f,
+                          const tse_sor_params *prm, tse_result *out);
+
 /* compute_q (tse.py:744-780): writes qx, qy, qz in place; the last index in
  * each direction keeps its previous value, as in the reference. */
 int tse_compute_q(tse_ctx *ctx, const double *P, const double *BP,

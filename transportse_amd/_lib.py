@@ -101,6 +101,8 @@ SIGNATURES = {
     "tse_from_split": (_I, [_V, _dp, _L, _L, _L, _dp]),
     "tse_compute_p": (_I, [_V, ctypes.POINTER(PFields),
                            ctypes.POINTER(SorParams), ctypes.POINTER(Result)]),
+    "tse_compute_p_noalloc": (_I, [_V, ctypes.POINTER(PFields),
+                                   ctypes.POINTER(SorParams), ctypes.POINTER(Result)]),
     "tse_compute_q": (_I, [_V] + [_dp] * 10),
     "tse_flow_error": (_I, [_V] + [_dp] * 10 + [_D, _dp]),
     "tse_compute_c": (_I, [_V, ctypes.POINTER(CFields),

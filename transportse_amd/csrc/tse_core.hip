@@ -13,10 +13,53 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "tse_internal.h"
 
 namespace tse {
+
+// ---- fixed scratch of tse_compute_p_noalloc -------------------------------------------------
+// The coefficient records and the cell mask of the faithful compute_P,
+// which tse_compute_p keeps in context buffers grown with hipMalloc, live
+// for tse_compute_p_noalloc in this file-scope device array: each GPU the
+// library runs on holds one copy, at its own address, until the process
+// ends.  Host code never uses the array's name as an address: tse_ctx_create
+// looks up the device's copy once per device with hipGetSymbolAddress (with
+// that device current) and keeps it in g_noalloc_ptr.
+constexpr size_t kNoallocScratchBytes = 64ull << 20;
+constexpr int kNoallocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+static void *g_noalloc_ptr[kNoallocMaxDevices];
+static std::mutex g_noalloc_mu;
+
+// Called with `device` current.  A failed lookup leaves the pointer null:
+// the context still works, and tse_compute_p_noalloc reports the error.
+static void noalloc_init(int device) {
+    if (device < 0 || device >= kNoallocMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) == hipSuccess && p)
+        g_noalloc_ptr[device] = p;
+    else
+        (void)hipGetLastError();
+}
+
+static void *noalloc_ptr(int device) {
+    if (device < 0 || device >= kNoallocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
+// Bytes of fixed scratch a grid needs: records, then the mask at the next
+// 256-byte boundary.
+static size_t noalloc_bytes(const Geo &G, size_t *mask_off) {
+    const size_t rec = rec_buffer_elems(G, kPRec) * sizeof(double);
+    const size_t off = (rec + 255) / 256 * 256;
+    *mask_off = off;
+    return off + mask_bytes(G);
+}
 
 static thread_local std::string g_err;
 
@@ -304,6 +347,7 @@ int tse_ctx_create(tse_ctx **out, int device, void *stream) {
     if (device < 0 || device >= ndev)
         return fail(TSE_ERR_ARG, "device index out of range");
     TSE_HIP_TRY(hipSetDevice(device));
+    noalloc_init(device);
     tse_ctx *ctx = new tse_ctx();
     ctx->device = device;
     ctx->stream = (hipStream_t)stream;
@@ -494,6 +538,52 @@ int tse_compute_p(tse_ctx *ctx, const tse_p_fields *f,
         return fail(TSE_ERR_ARG, "null field pointer");
     if (prm->form != TSE_FORM_FAITHFUL64)
         return fail(TSE_ERR_ARG, "tse_compute_p: unsupported form");
+    SolveHooks hk{p_assemble_hook, f, f->P, f->Pn, false};
+    return sor_solve(ctx, hk, prm, out);
+}
+
+int tse_compute_p_noalloc(tse_ctx *ctx, const tse_p_fields *f,
+                          const tse_sor_params *prm, tse_result *out) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    if (!f || !prm || !out) return fail(TSE_ERR_ARG, "null argument");
+    if (!f->P || !f->Pn || !f->c_f || !f->nue || !f->rho_f || !f->phi ||
+        !f->Kx || !f->Ky || !f->Kz || !f->BP || !f->WP || !f->c_r)
+        return fail(TSE_ERR_ARG, "null field pointer");
+    if (prm->form != TSE_FORM_FAITHFUL64)
+        return fail(TSE_ERR_ARG, "tse_compute_p_noalloc: only TSE_FORM_FAITHFUL64");
+    size_t mask_off = 0;
+    const size_t need = noalloc_bytes(ctx->geo, &mask_off);
+    if (need > kNoallocScratchBytes)
+        return fail(TSE_ERR_ARG,
+                    "tse_compute_p_noalloc: the grid needs " + std::to_string(need) +
+                        " bytes of scratch, the fixed limit is " +
+                        std::to_string(kNoallocScratchBytes) +
+                        " (about 6.4e5 cells); use tse_compute_p");
+    unsigned char *base = (unsigned char *)noalloc_ptr(ctx->device);
+    if (!base)
+        return fail(TSE_ERR_STATE,
+                    "tse_compute_p_noalloc: no scratch address for this device");
+    // The solve runs on the context's record and mask slots pointed at the
+    // fixed scratch, sized so that the on-demand growth never triggers; the
+    // context's own buffers come back afterwards, whatever the outcome.
+    struct Swap {
+        tse_ctx *c;
+        double *rec;
+        size_t rec_elems;
+        uint8_t *mask;
+        size_t mask_elems;
+        ~Swap() {
+            c->rec = rec;
+            c->rec_elems = rec_elems;
+            c->mask = mask;
+            c->mask_elems = mask_elems;
+        }
+    } keep{ctx, ctx->rec, ctx->rec_elems, ctx->mask, ctx->mask_elems};
+    ctx->rec = (double *)base;
+    ctx->rec_elems = rec_buffer_elems(ctx->geo, kPRec);
+    ctx->mask = (uint8_t *)(base + mask_off);
+    ctx->mask_elems = mask_bytes(ctx->geo);
     SolveHooks hk{p_assemble_hook, f, f->P, f->Pn, false};
     return sor_solve(ctx, hk, prm, out);
 }
