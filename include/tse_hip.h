@@ -38,10 +38,23 @@ enum tse_status {
     TSE_ERR_NOMEM = -4    /* device allocation failed           */
 };
 
-/* Solver forms for tse_compute_p. */
+/* Solver forms for tse_compute_p, tse_compute_c and tse_compute_t.
+ * FAITHFUL64 is the reference's iteration operation for operation (its
+ * update-norm stop test, bit-identical results).  PLAIN64 and MIXED solve
+ * the fixed-point system of the same update (the operator of the
+ * performance forms below) to the relative residual ||b - A x|| / ||b||
+ * <= conv, with FP64 sweeps (PLAIN64) or FP32 sweeps inside FP64 defect
+ * correction (MIXED); max_it bounds the sweeps of one attempt, and the
+ * time-step control around the attempts is the reference's: accept and
+ * grow dt by inc_it (up to max_dt) when an attempt took fewer than max_it/2
+ * sweeps, else halve dt and reset the unknown to its entry value, stop with
+ * converged = 0 once dt <= min_dt (tse.py:724-740, 896-914, 1065-1079);
+ * transport with omega == 0 is accepted after its first sweep launch.  For
+ * these forms num_it counts sweeps over all attempts and last_diff is the
+ * last relative residual. */
 enum tse_p_form {
     TSE_FORM_FAITHFUL64 = 0, /* reference arithmetic, bit-identical      */
-    TSE_FORM_PLAIN64 = 1,    /* precomputed symmetric operator, FP64     */
+    TSE_FORM_PLAIN64 = 1,    /* precomputed operator, FP64 sweeps        */
     TSE_FORM_MIXED = 2       /* FP32 sweeps in FP64 defect correction    */
 };
 
@@ -72,7 +85,7 @@ typedef struct {
     double min_dt;
     double conv;
     int32_t max_it;
-    int32_t form;      /* enum tse_p_form (compute_P only)               */
+    int32_t form;      /* enum tse_p_form                                */
 } tse_sor_params;
 
 /* Inputs of compute_P (tse.py:649). P is updated in place; Pn receives P at
@@ -260,7 +273,10 @@ typedef struct {
 /* Assembles the operator from compute_P's inputs at time step dt (Pn = the
  * storage-term reference pressure; NULL uses P) and loads P as the iterate. */
 int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt);
-/* Solves to prm->tol (or prm->max_sweeps) from the current iterate. */
+/* Solves to prm->tol (or prm->max_sweeps) from the current iterate; a
+ * cycle that would run past max_sweeps is shortened to end there (rounded
+ * up to the plan's ns).  Batched members instead stop at the first cycle end
+ * at or beyond max_sweeps. */
 int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
                      tse_fast_result *out);
 /* Relative residual of the current iterate. */

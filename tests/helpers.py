@@ -1,5 +1,6 @@
 """Shared test helpers: seeded field sets and backend factories."""
 import math
+import types
 import os
 
 import numpy as np
@@ -86,3 +87,63 @@ class Fields:
     def grid_args(self):
         return (self.nx, self.ny, self.nz, self.dx, self.dy, self.dz, self.g,
                 self.beta)
+
+
+def oracle_sia_steps(spec, nsteps, dts=None):
+    """The SIA steps of a ScenarioSpec (transportse_amd.scenario) run with the
+    oracle's reference functions in tse.py order (compute_P, compute_q,
+    flow_error, compute_C per species, compute_T, linear EOS, density norm,
+    reset on a rejected Picard iteration), each solve converged to an update
+    norm of 1e-15 at the step's dt (dts[s], default the spec's dt), i.e. the
+    FP64 fixed point of every solve."""
+    from oracle import Oracle
+    from transportse_amd.scenario import EOS, UNIFORM, host_fields
+    hf = host_fields(spec)
+    shp = (spec.nx, spec.ny, spec.nz)
+    U = {k: np.full(shp, v) for k, v in UNIFORM.items()}
+    disp_l, disp_t = np.full(shp, spec.disp[0]), np.full(shp, spec.disp[1])
+    frozen = types.SimpleNamespace(c_r=U["c_r"], l_r=U["l_r"], cp_r=U["cp_r"],
+                                   rho_r=U["rho_r"], lf_l=disp_l, lf_t=disp_t,
+                                   lt_l=disp_l, lt_t=disp_t)
+    d = np.full(spec.nx, spec.spacing)
+    o = Oracle(spec.nx, spec.ny, spec.nz, d, np.full(spec.ny, spec.spacing),
+               np.full(spec.nz, spec.spacing), 9.80665, spec.beta,
+               frozen_source=frozen, dt_max=spec.dt, c_imblflo=spec.imblflo,
+               t_imblflo=spec.imblflo)
+    K, P, T = hf["K"], hf["P"].copy(), hf["T"].copy()
+    C = [c.copy() for c in hf["C"]]
+    rho, l_f = hf["rho_f"].copy(), hf["l_f"].copy()
+    rho_prev = np.zeros(shp)
+    qx, qy, qz = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+    for step in range(nsteps):
+        dt = spec.dt if dts is None else dts[step]
+        Pn0, Cn0, Tn0 = P.copy(), [c.copy() for c in C], T.copy()
+        for it in range(spec.sia_maxit):
+            r = o.compute_P(P, U["c_f"], U["nue"], rho, U["phi"], K, K, K, qx, qy,
+                            qz, dt, hf["BP"], U["zero"], SOR=1.8, max_it=10 ** 6,
+                            inc_it=1.25, max_dt=dt, P_conv=1e-15)
+            assert r[4]
+            P, Pn = r[2], r[3]
+            o.compute_q(P, hf["BP"], K, K, K, qx, qy, qz, U["nue"], rho)
+            e = o.flow_error(P, Pn, qx, qy, qz, U["zero"], U["phi"], U["c_f"],
+                             U["c_r"], rho, dt)
+            for s in range(spec.species):
+                r = o.compute_C(C[s], U["D_f"], U["D_f"], U["D_f"], U["phi"], rho,
+                                qx, qy, qz, dt, hf["BC"], U["zero"], U["R"], e,
+                                omega=spec.omega, SOR=1.0, max_it=10 ** 5,
+                                max_dt=dt, C_conv=1e-15)
+                assert r[3]
+                C[s] = r[2]
+            r = o.compute_T(T, l_f, U["cp_f"], rho, U["phi"], qx, qy, qz, dt,
+                            hf["BT"], U["zero"], e, omega=spec.omega, SOR=1.0,
+                            max_it=10 ** 5, max_dt=dt, T_conv=1e-15)
+            assert r[3]
+            T = r[2]
+            rho = EOS["rho0"] + EOS["drho_dc"] * C[0]
+            l_f = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
+            diff = o.l2_norm(rho, rho_prev)
+            if diff < spec.sia_conv or it == spec.sia_maxit - 1:
+                break
+            rho_prev = rho.copy()
+            P, T, C = Pn0.copy(), Tn0.copy(), [c.copy() for c in Cn0]
+    return {"P": P, "T": T, "C": C, "rho_f": rho, "l_f": l_f}

@@ -1,12 +1,10 @@
-"""GPU, opt-in: the salt + heat SIA config at its full size (256x256x128)
-for one step through the driver with the GPU backend and with the oracle
-backend; every field must be bit-identical, with equal iteration counts.
+"""GPU: the salt + heat SIA config at its full size (256x256x128) for one
+step through the driver with the GPU backend and with the oracle backend;
+every field must be bit-identical, with equal iteration counts.
 
-The oracle needs about four minutes for the step on 16 host threads, so the
-test runs only with TSE_FULL_B=1, and with pytest's -s so that the progress
-line printed after every oracle solver call is seen:
-
-    TSE_FULL_B=1 python -m pytest -s tests/test_gpu_config_b_oracle.py
+The oracle needs about four minutes for the step on 16 host threads; it
+prints a progress line after every solver call (seen with pytest's -s).
+TSE_SKIP_FULL_B=1 skips it for quick local runs.
 """
 import os
 import time
@@ -17,8 +15,8 @@ import pytest
 from helpers import ROOT, oracle_backend
 
 pytestmark = [pytest.mark.gpu,
-              pytest.mark.skipif(not os.environ.get("TSE_FULL_B"),
-                                 reason="opt-in (TSE_FULL_B=1): ~4 min of oracle time")]
+              pytest.mark.skipif(bool(os.environ.get("TSE_SKIP_FULL_B")),
+                                 reason="TSE_SKIP_FULL_B=1: ~4 min of oracle time")]
 
 CFG = os.path.join(ROOT, "transportse_amd", "configs", "salt_heat_sia_256x256x128.py")
 

@@ -117,7 +117,7 @@ def test_ensemble_tuner_picks_a_plan():
 
 def test_serving_alone_equals_mix():
     from transportse_amd.serve import Server, make_requests, prepare
-    reqs = make_requests(6, rate=500.0, steps=2, species=2, grids=SMALL)
+    reqs = make_requests(6, rate=500.0, steps=2, species=3, grids=SMALL)
     for r in reqs:
         r.spec.dt = 43200.0
     prepare(reqs)

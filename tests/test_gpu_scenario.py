@@ -17,6 +17,8 @@ import types
 import numpy as np
 import pytest
 
+from helpers import oracle_sia_steps
+
 pytestmark = pytest.mark.gpu
 
 
@@ -25,60 +27,6 @@ def _spec():
     return ScenarioSpec(24, 20, 16, spacing=5.0, beta=3.0, sigma=1.5, corr=4,
                         seed=7, species=2, heat=True, salt_patch=3,
                         dt=43200.0, sia_maxit=2, sia_conv=1e-12)
-
-
-def _oracle_steps(spec, nsteps):
-    from oracle import Oracle
-    from transportse_amd.scenario import EOS, UNIFORM, host_fields
-    hf = host_fields(spec)
-    shp = (spec.nx, spec.ny, spec.nz)
-    U = {k: np.full(shp, v) for k, v in UNIFORM.items()}
-    disp_l, disp_t = np.full(shp, spec.disp[0]), np.full(shp, spec.disp[1])
-    frozen = types.SimpleNamespace(c_r=U["c_r"], l_r=U["l_r"], cp_r=U["cp_r"],
-                                   rho_r=U["rho_r"], lf_l=disp_l, lf_t=disp_t,
-                                   lt_l=disp_l, lt_t=disp_t)
-    d = np.full(spec.nx, spec.spacing)
-    o = Oracle(spec.nx, spec.ny, spec.nz, d, np.full(spec.ny, spec.spacing),
-               np.full(spec.nz, spec.spacing), 9.80665, spec.beta,
-               frozen_source=frozen, dt_max=spec.dt, c_imblflo=spec.imblflo,
-               t_imblflo=spec.imblflo)
-    K, P, T = hf["K"], hf["P"].copy(), hf["T"].copy()
-    C = [c.copy() for c in hf["C"]]
-    rho, l_f = hf["rho_f"].copy(), hf["l_f"].copy()
-    rho_prev = np.zeros(shp)
-    qx, qy, qz = np.zeros(shp), np.zeros(shp), np.zeros(shp)
-    dt = spec.dt
-    for _ in range(nsteps):
-        Pn0, Cn0, Tn0 = P.copy(), [c.copy() for c in C], T.copy()
-        for it in range(spec.sia_maxit):
-            r = o.compute_P(P, U["c_f"], U["nue"], rho, U["phi"], K, K, K, qx, qy,
-                            qz, dt, hf["BP"], U["zero"], SOR=1.8, max_it=10 ** 6,
-                            inc_it=1.25, max_dt=dt, P_conv=1e-15)
-            assert r[4]
-            P, Pn = r[2], r[3]
-            o.compute_q(P, hf["BP"], K, K, K, qx, qy, qz, U["nue"], rho)
-            e = o.flow_error(P, Pn, qx, qy, qz, U["zero"], U["phi"], U["c_f"],
-                             U["c_r"], rho, dt)
-            for s in range(spec.species):
-                r = o.compute_C(C[s], U["D_f"], U["D_f"], U["D_f"], U["phi"], rho,
-                                qx, qy, qz, dt, hf["BC"], U["zero"], U["R"], e,
-                                omega=spec.omega, SOR=1.0, max_it=10 ** 5,
-                                max_dt=dt, C_conv=1e-15)
-                assert r[3]
-                C[s] = r[2]
-            r = o.compute_T(T, l_f, U["cp_f"], rho, U["phi"], qx, qy, qz, dt,
-                            hf["BT"], U["zero"], e, omega=spec.omega, SOR=1.0,
-                            max_it=10 ** 5, max_dt=dt, T_conv=1e-15)
-            assert r[3]
-            T = r[2]
-            rho = EOS["rho0"] + EOS["drho_dc"] * C[0]
-            l_f = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
-            diff = o.l2_norm(rho, rho_prev)
-            if diff < spec.sia_conv or it == spec.sia_maxit - 1:
-                break
-            rho_prev = rho.copy()
-            P, T, C = Pn0.copy(), Tn0.copy(), [c.copy() for c in Cn0]
-    return {"P": P, "T": T, "C": C, "rho_f": rho, "l_f": l_f}
 
 
 def _rel(a, b):
@@ -90,7 +38,7 @@ def _rel(a, b):
 def test_scenario_steps_match_oracle(form, tol, bp, bx):
     from transportse_amd.scenario import DeviceScenario
     spec = _spec()
-    ref = _oracle_steps(spec, 2)
+    ref = oracle_sia_steps(spec, 2)
     scn = DeviceScenario(spec)
     scn.run(2, form=form, tol=tol, plan_p=(2, 16, 1, 0), plan_tr=(2, 16, 1, 0),
             sor_p=1.9, sor_tr=1.2, inner_p=20, inner_tr=4)
@@ -180,9 +128,54 @@ def test_scenario_segment_graphs_bit_identical():
     eager, replay = out
     print("captured segments:", replay["graphs"], flush=True)
     assert not eager["graphs"]
-    assert {"setup_p", "setup_c0", "setup_t", "eos"} <= set(replay["graphs"])
+    # segments that depend on dt are kept per dt value
+    dt = repr(spec.dt)
+    assert {f"setup_p@{dt}", f"setup_c0@{dt}", f"setup_t@{dt}", "eos"} \
+        <= set(replay["graphs"])
     assert replay["sweeps"] == eager["sweeps"]
     for k in ("P", "T", "rho_f", "qx", "e"):
         assert np.array_equal(replay[k], eager[k]), k
     for a, b in zip(replay["C"], eager["C"]):
         assert np.array_equal(a, b)
+
+
+def test_scenario_halves_dt_on_a_failed_solve():
+    """The first pressure solve is given too few sweeps to converge: the
+    step halves dt and repeats it from the same entry state, the rest of the
+    step runs at dt/2, and the next step's dt is the smallest of the solvers'
+    next dt, min(dt/2 * 1.25, dt/2 * 1.05) (tse.py:724-740, 1486-1492).  The
+    fields then match the oracle's SIA steps at that dt sequence within the
+    mixed form's tolerances (P 1e-9, C and T 1e-7 relative L2)."""
+    from transportse_amd.scenario import KIND_WAIT, DeviceScenario
+    spec = _spec()
+    scn = DeviceScenario(spec)
+    big = 10 ** 6
+    first = True
+    for kind, sysm in scn.steps(2):
+        if kind == KIND_WAIT:
+            continue
+        if kind == 0:
+            budget = 3 if first else big
+            r = sysm.solve("mixed", tol=1e-10, sor=1.9, inner=3, plan=(3, 16, 1, 0),
+                           max_sweeps=budget)
+            if first:
+                assert not r.converged
+            first = False
+        else:
+            r = sysm.solve("mixed", tol=1e-10, sor=1.2, inner=4, plan=(2, 16, 1, 0),
+                           max_sweeps=big)
+        scn.solved(r.converged, r.sweeps, budget if kind == 0 else big)
+    dts = [spec.dt / 2, spec.dt / 2 * 1.05]
+    assert scn.halvings == 1
+    assert [h.dt for h in scn.history] == dts
+    assert scn.t == dts[0] + dts[1]
+    ref = oracle_sia_steps(spec, 2, dts=dts)
+    got = scn.download()
+    errs = {"P": _rel(got["P"], ref["P"]), "T": _rel(got["T"], ref["T"])}
+    for s in range(spec.species):
+        errs[f"C{s}"] = _rel(got["C"][s], ref["C"][s])
+    print("halving: relative L2 error vs oracle", errs, flush=True)
+    assert errs["P"] <= 1e-9
+    for k, v in errs.items():
+        assert v <= 1e-7, k
+    scn.close()

@@ -54,6 +54,7 @@ class Batch:
         ns = plan[0]
         inner = max(ns, (int(inner) // ns) * ns)
         self.form, self.plan, self.inner = form, tuple(plan), inner
+        self.max_sweeps = int(max_sweeps)
         prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*plan),
                               inner=inner, max_sweeps=int(max_sweeps),
                               sor=float(sor), tol=float(tol))

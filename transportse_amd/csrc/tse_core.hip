@@ -536,8 +536,7 @@ int tse_compute_p(tse_ctx *ctx, const tse_p_fields *f,
     if (!f->P || !f->Pn || !f->c_f || !f->nue || !f->rho_f || !f->phi ||
         !f->Kx || !f->Ky || !f->Kz || !f->BP || !f->WP || !f->c_r)
         return fail(TSE_ERR_ARG, "null field pointer");
-    if (prm->form != TSE_FORM_FAITHFUL64)
-        return fail(TSE_ERR_ARG, "tse_compute_p: unsupported form");
+    if (prm->form != TSE_FORM_FAITHFUL64) return fast_compute(ctx, 0, f, prm, out);
     SolveHooks hk{p_assemble_hook, f, f->P, f->Pn, false};
     return sor_solve(ctx, hk, prm, out);
 }
@@ -597,6 +596,7 @@ int tse_compute_c(tse_ctx *ctx, const tse_c_fields *f,
         !f->rho_f || !f->qx || !f->qy || !f->qz || !f->BC || !f->WC ||
         !f->RC || !f->e || !f->lf_l || !f->lf_t)
         return fail(TSE_ERR_ARG, "null field pointer");
+    if (prm->form != TSE_FORM_FAITHFUL64) return fast_compute(ctx, 1, f, prm, out);
     SolveHooks hk{c_assemble_hook, f, f->C, f->Cn, true};
     return sor_solve(ctx, hk, prm, out);
 }
@@ -610,6 +610,7 @@ int tse_compute_t(tse_ctx *ctx, const tse_t_fields *f,
         !f->qx || !f->qy || !f->qz || !f->BT || !f->WT || !f->e || !f->l_r ||
         !f->cp_r || !f->rho_r || !f->lt_l || !f->lt_t)
         return fail(TSE_ERR_ARG, "null field pointer");
+    if (prm->form != TSE_FORM_FAITHFUL64) return fast_compute(ctx, 2, f, prm, out);
     SolveHooks hk{t_assemble_hook, f, f->T, f->Tn, true};
     return sor_solve(ctx, hk, prm, out);
 }

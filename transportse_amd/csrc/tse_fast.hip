@@ -739,21 +739,24 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     double rel = rn / bs;
     int sweeps = 0, cycles = 0;
     while (rel > prm->tol && sweeps < prm->max_sweeps) {
+        // the last cycle ends at max_sweeps (rounded up to the plan's ns)
+        const int left = (prm->max_sweeps - sweeps + pl.ns - 1) / pl.ns * pl.ns;
+        const int run = std::min(chunk, left);
         if (fp32) {
             int e;
-            rc = correction_sweeps(ctx, s, zg, pl, chunk, (float)prm->sor, &e);
+            rc = correction_sweeps(ctx, s, zg, pl, run, (float)prm->sor, &e);
             if (rc) return rc;
             rc = residual<1>(ctx, s, s->p(s->cur), s->e(e), s->p(1 - s->cur),
                              &rn, &bn);
             if (rc) return rc;
             s->cur = 1 - s->cur;
         } else {
-            rc = plain_sweeps(ctx, s, zg, pl, chunk, prm->sor);
+            rc = plain_sweeps(ctx, s, zg, pl, run, prm->sor);
             if (rc) return rc;
             rc = residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
             if (rc) return rc;
         }
-        sweeps += chunk;
+        sweeps += run;
         cycles += 1;
         const double rel_prev = rel;
         rel = rn / bs;
@@ -762,7 +765,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             // sweeps still needed at the last cycle's rate, clamped
             long next = inner;
             if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
-                const double rate = std::log(rel / rel_prev) / chunk;   // < 0
+                const double rate = std::log(rel / rel_prev) / run;     // < 0
                 const double need = std::log(prm->tol / rel) / rate;     // > 0
                 next = (long)std::ceil(need * 1.02);
             }
@@ -918,3 +921,116 @@ int tse_fast_tr_sweeps(tse_ctx *ctx, const tse_sweep_plan *plan,
 }
 
 }  // extern "C"
+
+// ---- the performance forms behind compute_P / compute_C / compute_T -----------------------
+// tse_compute_p / _c / _t with form TSE_FORM_PLAIN64 or TSE_FORM_MIXED: one
+// attempt solves the fixed-point system of the reference update at dt to
+// the relative residual prm->conv within prm->max_it sweeps; the time-step
+// control around the attempts is the reference's (tse.py:724-740 for
+// compute_P, 896-914 compute_C, 1065-1079 compute_T):
+//  * converged: t += dt, and if the attempt took fewer than max_it/2 sweeps,
+//    dt = min(dt * inc_it, max_dt);
+//  * not converged within max_it sweeps (or the residual turned NaN, which
+//    the reference's norm reports as 1 until max_it): dt /= 2 and the unknown
+//    is reset to its value at entry; dt <= min_dt ends the call with
+//    converged = 0.
+// Transport with omega == 0 is accepted after its first launch, as the
+// reference accepts after one iteration.  num_it counts sweeps over all
+// attempts; last_diff is the last relative residual.
+namespace tse {
+
+// ns sweeps per launch; explicit: one sweep per launch (transport, omega 0)
+static tse_sweep_plan default_fast_plan(const tse_ctx *ctx, int kind, int fp32,
+                                        bool explicit_step) {
+    const Geo &G = ctx->geo;
+    const long N = (long)G.nx * G.ny * G.nz;
+    tse_sweep_plan pl{explicit_step ? 1 : 2, 16, 1, 0, 2};
+    if (kind == SYS_P && fp32) {
+        pl = tse_sweep_plan{3, 32, 2, 0, 3};
+        if (!fast_plan_grid_ok(pl, G.nx)) pl.variant = 2;
+    }
+    if (!fast_plan_fits(pl, N, fp32)) {  // fields over 1 GiB: no buffer loads
+        pl = tse_sweep_plan{explicit_step ? 1 : 2, 16, 1, 0, 0};
+    }
+    return pl;
+}
+
+int fast_compute(tse_ctx *ctx, int kind, const void *fields,
+                 const tse_sor_params *prm, tse_result *out) {
+    const int fp32 = prm->form == TSE_FORM_MIXED;
+    if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
+        return fail(TSE_ERR_ARG, "form must be FAITHFUL64, PLAIN64 or MIXED");
+    if (prm->max_it < 1) return fail(TSE_ERR_ARG, "max_it must be >= 1");
+    if (!(prm->dt > 0.0)) return fail(TSE_ERR_ARG, "dt must be > 0");
+    const int which = kind;  // 0 pressure, 1 compute_C, 2 compute_T
+    const int sys_kind = which == 0 ? SYS_P : SYS_TR;
+    const Geo &G = ctx->geo;
+    const size_t bytes = 2ull * G.S * sizeof(double);
+    const tse_p_fields *fp = which == 0 ? (const tse_p_fields *)fields : nullptr;
+    const tse_c_fields *fc = which == 1 ? (const tse_c_fields *)fields : nullptr;
+    const tse_t_fields *ft = which == 2 ? (const tse_t_fields *)fields : nullptr;
+    double *X = fp ? fp->P : (fc ? fc->C : ft->T);
+    double *Xn = fp ? fp->Pn : (fc ? fc->Cn : ft->Tn);
+    // the unknown at entry (compute_P's Pn, tse.py:656; the transport setups
+    // copy it themselves)
+    TSE_HIP_TRY(hipMemcpyAsync(Xn, X, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool explicit_step = which != 0 && prm->omega == 0.0;
+    // omega == 0: one red+black sweep, as the reference's single iteration
+    const tse_sweep_plan pl = default_fast_plan(ctx, sys_kind, fp32, explicit_step);
+    tse_fast_params fpm{};
+    fpm.form = prm->form;
+    fpm.plan = pl;
+    // first cycle about max_it/8 sweeps (at most 64), then sized from the
+    // observed rate (adaptive cycles), all multiples of the plan's ns
+    const int first = std::max(1, std::min(64, prm->max_it / 8));
+    fpm.inner = std::max(pl.ns, first / pl.ns * pl.ns);
+    fpm.cycle_mode = TSE_CYCLE_ADAPTIVE;
+    fpm.sor = prm->sor;
+    fpm.tol = explicit_step ? 0.0 : prm->conv;
+    fpm.max_sweeps = explicit_step ? pl.ns : prm->max_it;
+    double dt = prm->dt, t = 0.0, rel = 0.0;
+    int total = 0, halvings = 0, status = FAILED;
+    FastSys *s = system_of(ctx, sys_kind);
+    for (;;) {
+        int rc;
+        if (fp) {
+            rc = tse_fast_p_setup(ctx, fp, dt);
+        } else if (fc) {
+            rc = tse_fast_c_setup(ctx, fc, dt, prm->omega);
+        } else {
+            rc = tse_fast_t_setup(ctx, ft, dt, prm->omega);
+        }
+        if (rc) return rc;
+        tse_fast_result r{};
+        rc = sys_solve(ctx, s, &fpm, &r);
+        if (rc) return rc;
+        total += r.sweeps;
+        rel = r.rel_residual;
+        if (r.converged || explicit_step) {
+            rc = sys_result(ctx, s, X);
+            if (rc) return rc;
+            t += dt;
+            if ((double)r.sweeps < prm->max_it / 2.0) dt = std::min(dt * prm->inc_it, prm->max_dt);
+            status = CONVERGED;
+            break;
+        }
+        dt = dt / 2;  // tse.py:733-738: halve, reset the unknown
+        halvings += 1;
+        TSE_HIP_TRY(hipMemcpyAsync(X, Xn, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+        if (dt <= prm->min_dt) {
+            status = FAILED;
+            break;
+        }
+    }
+    TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
+    out->t = t;
+    out->dt = dt;
+    out->converged = status == CONVERGED ? 1 : 0;
+    out->num_it = total;
+    out->num_halvings = halvings;
+    out->status = status;
+    out->last_diff = rel;
+    return 0;
+}
+
+}  // namespace tse

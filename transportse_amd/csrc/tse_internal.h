@@ -216,4 +216,8 @@ int launch_smolarkiewicz(tse_ctx *ctx, const double *C, const double *qx,
                          double Sc);
 // implemented in tse_fast.hip
 void fast_free(tse_ctx *ctx);
+// compute_P / compute_C / compute_T (kind 0 / 1 / 2) in a performance form,
+// with the reference's time-step control around each attempt
+int fast_compute(tse_ctx *ctx, int kind, const void *fields,
+                 const tse_sor_params *prm, tse_result *out);
 }  // namespace tse

@@ -22,6 +22,7 @@ properties then act on layer 0, tse.py:207-226) follow the reference.
 """
 from __future__ import annotations
 
+import functools
 import os
 import sys
 import time
@@ -543,8 +544,12 @@ class Simulation:
                 st.Ky[l], st.Kz[l] = r[0], r[1], r[2], r[3], r[4], r[5], r[6], r[7]
 
 
-def make_hip_backend(sim: Simulation, p_form: str = "faithful64", device: int = 0):
-    """GPU backend for a Simulation (frozen globals read from its state)."""
+def make_hip_backend(sim: Simulation, p_form: str = "faithful64", device: int = 0,
+                     c_form: str = "faithful64", t_form: str = "faithful64"):
+    """GPU backend for a Simulation (frozen globals read from its state).
+    Forms as HipSolvers: faithful64 is bit-identical to the reference;
+    plain64 / mixed read the config's CONV_P / CONV_C / CONV_T as relative
+    residuals."""
     from .solvers import HipSolvers
     st, c = sim.st, sim.config
     return HipSolvers(st.nx, st.ny, st.nz, st.dx, st.dy, st.dz, st.g,
@@ -552,7 +557,8 @@ def make_hip_backend(sim: Simulation, p_form: str = "faithful64", device: int = 
                       maxit_c=sim.MAXIT_C, maxit_t=sim.MAXIT_T,
                       dt_max=c.dt_max, c_imblflo=c.C_IMBLFLO,
                       t_imblflo=c.T_IMBLFLO, device=device,
-                      auto_invalidate=False, verbose=sim.verbose, p_form=p_form)
+                      auto_invalidate=False, verbose=sim.verbose, p_form=p_form,
+                      c_form=c_form, t_form=t_form)
 
 
 def main(argv=None):
@@ -561,13 +567,20 @@ def main(argv=None):
     import argparse
     ap = argparse.ArgumentParser(prog="python -m transportse_amd.driver",
                                  description="Run a TRANSPORTSE config on the GPU "
-                                             "(faithful FP64 solvers).")
+                                             "(faithful FP64 solvers by default).")
     ap.add_argument("config", help="config module (path or importable name)")
     ap.add_argument("--max-steps", type=int, default=None,
                     help="stop after this many time steps")
     ap.add_argument("--quiet", action="store_true", help="no per-step printout")
+    forms = ("faithful64", "plain64", "mixed")
+    for what in ("p", "c", "t"):
+        ap.add_argument(f"--{what}-form", choices=forms, default="faithful64",
+                        help=f"compute_{what.upper()} form (plain64 / mixed read "
+                             f"CONV_{what.upper()} as a relative residual)")
     args = ap.parse_args(sys.argv[1:] if argv is None else argv[1:])
-    sim = Simulation(args.config, make_hip_backend, verbose=not args.quiet,
+    backend = functools.partial(make_hip_backend, p_form=args.p_form,
+                                c_form=args.c_form, t_form=args.t_form)
+    sim = Simulation(args.config, backend, verbose=not args.quiet,
                      max_steps=args.max_steps)
     try:
         sim.run()

@@ -194,15 +194,16 @@ class Ensemble:
                     cycles += b.run(graph=self.graph)
                     for i in ids:
                         st = b.status(slots[kind][i])
+                        # a solve that did not converge halves the
+                        # realization's dt and comes back as the same stage
+                        self.scn[i].solved(st.converged, st.sweeps, self.max_sweeps)
+                        sweeps[kind] += st.sweeps
+                        cell_sweeps += st.sweeps * self.scn[i].spec.cells
                         if not st.converged:
-                            raise RuntimeError(
-                                f"realization {i} kind {kind} did not converge:"
-                                f" rel {st.rel_residual:.3e} after {st.sweeps} sweeps")
+                            continue
                         b.writeback(slots[kind][i])
                         if kind == KIND_P and self.tune_omega:
                             self.scn[i].omega_p.report(st.sweeps)
-                        sweeps[kind] += st.sweeps
-                        cell_sweeps += st.sweeps * self.scn[i].spec.cells
                         solves += 1
                     for i, it in resume_all(gens, ids).items():
                         if it is None:
@@ -262,17 +263,16 @@ class Ensemble:
                     st = b.status(slots[kind][i])
                     if st.active:
                         continue
-                    if not st.converged:
-                        raise RuntimeError(
-                            f"realization {i} kind {kind} did not converge:"
-                            f" rel {st.rel_residual:.3e} after {st.sweeps} sweeps")
+                    self.scn[i].solved(st.converged, st.sweeps, self.max_sweeps)
+                    sweeps[kind] += st.sweeps
+                    cell_sweeps += st.sweeps * self.scn[i].spec.cells
+                    done.append(i)
+                    if not st.converged:   # repeated at half the dt
+                        continue
                     b.writeback(slots[kind][i])
                     if kind == KIND_P and self.tune_omega:
                         self.scn[i].omega_p.report(st.sweeps)
-                    sweeps[kind] += st.sweeps
-                    cell_sweeps += st.sweeps * self.scn[i].spec.cells
                     solves += 1
-                    done.append(i)
                 for i, it in resume_all(gens, done).items():
                     if it is None:
                         del pend[i]

@@ -13,12 +13,19 @@ is in.
 
 Solver form: the pressure and transport systems are solved to a relative
 residual (FP32 sweeps in FP64 defect correction, or plain FP64) instead of
-the reference's update-norm / dt-halving control, so dt stays fixed (the
-reference keeps dt = dt_max too when every solve converges and SIA is on:
-tse.py:1262-1271, 1486-1492).  Everything else follows tse.py: the solver
-call order, the time level of each input, rhof_prev starting at zero and
-persisting across steps (tse.py:1179, 1430), accept on SIA_conv or the last
-iteration (tse.py:1429).
+the reference's update norm.  The time-step control is the reference's: the
+driver reports each solve's outcome (``solved``); a solve that does not
+converge within its sweep budget halves dt and runs again from its entry
+state, and the rest of the step uses the halved dt (tse.py:730-740, the
+returned t_X feeding the next call); the next step's dt is the smallest of
+the solvers' next dt, each grown by its inc_it (1.25 pressure, 1.05
+transport) when its solve took fewer than half its sweeps and capped at the
+scenario's dt (tse.py:724-729, 1486-1492).  When every solve converges dt
+stays at the scenario's dt, as in the reference with SIA on
+(tse.py:1262-1271).  Everything else follows tse.py: the solver call order,
+the time level of each input, rhof_prev starting at zero and persisting
+across steps (tse.py:1179, 1430), accept on SIA_conv or the last iteration
+(tse.py:1429).
 """
 from __future__ import annotations
 
@@ -185,6 +192,7 @@ def field_bytes(spec: ScenarioSpec) -> int:
 @dataclass
 class StepStats:
     sia_iters: int = 0
+    dt: float = 0.0
     sweeps: dict = field(default_factory=dict)   # stage -> sweeps
     mass: tuple = ()
 
@@ -236,6 +244,17 @@ class DeviceScenario:
         self.fp.assemble = self.setup_p
         self.omega_p = OmegaTuner(max(spec.nx, spec.ny, spec.nz))
         self.t = 0.0
+        # time-step control (tse.py:724-740, 896-914, 1065-1079, 1486-1492):
+        # dt is the running time step of the current step (a solve that does
+        # not converge halves it and is repeated from its entry state), dt_max
+        # caps its growth, dt_step is the dt of the last completed step
+        self.dt = float(spec.dt)
+        self.dt_max = float(spec.dt)
+        self.dt_step = float(spec.dt)
+        self.min_dt = 1e-8
+        self.halvings = 0
+        self._next_dt = {}
+        self._outcome = None
         self.n_steps = 0
         self.history: list[StepStats] = []
         self.eos = _lib.EosParams(**EOS)
@@ -255,7 +274,7 @@ class DeviceScenario:
                          BP=self._p(self.BP), WP=self._p(u["zero"]),
                          c_r=self._p(u["c_r"]))
         _lib.check(self.lib.tse_fast_p_setup(self.hip.ctx, ctypes.byref(f),
-                                             self.spec.dt), "tse_fast_p_setup")
+                                             self.dt), "tse_fast_p_setup")
 
     def finish_p(self):
         _lib.check(self.lib.tse_fast_p_result(self.hip.ctx, self._p(self.P)),
@@ -272,7 +291,7 @@ class DeviceScenario:
         _lib.check(L.tse_flow_error(self.hip.ctx, p(self.P), p(self.Pn), p(self.qx),
                                     p(self.qy), p(self.qz), p(u["zero"]),
                                     p(u["phi"]), p(u["c_f"]), p(u["c_r"]),
-                                    p(self.rho), self.spec.dt, p(self.e)),
+                                    p(self.rho), self.dt, p(self.e)),
                    "tse_flow_error")
 
     def setup_c(self, s):
@@ -283,7 +302,7 @@ class DeviceScenario:
                          qz=p(self.qz), BC=p(self.BC), WC=p(u["zero"]),
                          RC=p(u["R"]), e=p(self.e), lf_l=p(self._disp(0)),
                          lf_t=p(self._disp(1)), imblflo=int(sp.imblflo))
-        _lib.check(self.lib.tse_fast_c_setup(self.hip.ctx, ctypes.byref(f), sp.dt,
+        _lib.check(self.lib.tse_fast_c_setup(self.hip.ctx, ctypes.byref(f), self.dt,
                                              sp.omega), "tse_fast_c_setup")
 
     def setup_t(self):
@@ -295,7 +314,7 @@ class DeviceScenario:
                          l_r=p(u["l_r"]), cp_r=p(u["cp_r"]), rho_r=p(u["rho_r"]),
                          lt_l=p(self._disp(0)), lt_t=p(self._disp(1)),
                          imblflo=int(sp.imblflo))
-        _lib.check(self.lib.tse_fast_t_setup(self.hip.ctx, ctypes.byref(f), sp.dt,
+        _lib.check(self.lib.tse_fast_t_setup(self.hip.ctx, ctypes.byref(f), self.dt,
                                              sp.omega), "tse_fast_t_setup")
 
     def _uniform(self, name, value):
@@ -405,31 +424,66 @@ class DeviceScenario:
         for s in range(self.spec.species):
             self.C[s].copy_(self.Cn[s])
 
+    def solved(self, converged: bool, sweeps: int, max_sweeps: int) -> None:
+        """The driver's report on the solve the step generator last yielded,
+        given before resuming it: whether it reached its tolerance, and its
+        sweeps out of the max_sweeps it was allowed."""
+        self._outcome = (bool(converged), int(sweeps), int(max_sweeps))
+
+    def _solve(self, kind, name, setup, system, loaded, inc_it):
+        """Yield one solve (kind, system) until it converges: a solve that
+        does not halves dt and runs again from the same entry state (the
+        unknown is only written back after a converged solve), as the
+        reference's solvers do (tse.py:730-740); dt <= min_dt fails the run.
+        On success the solver's next dt grows by inc_it when the solve took
+        fewer than half its sweeps (tse.py:724-729)."""
+        while True:
+            self._segment(f"{name}@{self.dt!r}", setup, loaded)
+            self._outcome = None
+            yield kind, system
+            if self._outcome is None:
+                raise RuntimeError("the driver of DeviceScenario.steps must call"
+                                   " solved() after every solve")
+            ok, sweeps, max_sweeps = self._outcome
+            if ok:
+                nxt = self.dt
+                if sweeps < max_sweeps / 2:
+                    nxt = min(self.dt * inc_it, self.dt_max)
+                self._next_dt[name] = nxt
+                return
+            self.dt = self.dt / 2
+            self.halvings += 1
+            if self.dt <= self.min_dt:
+                raise RuntimeError(f"{name}: dt fell to {self.dt:.3e} s without"
+                                   " convergence")
+
     def steps(self, nsteps: int):
         """Generator over ``nsteps`` SIA steps; yields (kind, system) for each
-        solve and resumes once the system holds its solution, and
-        (KIND_WAIT, None) where it is about to read a reduction from the
-        device (resume at once, or after a wait that covers its stream)."""
+        solve and resumes once the system holds its solution (the driver
+        reports each outcome with ``solved`` first), and (KIND_WAIT, None)
+        where it is about to read a reduction from the device (resume at
+        once, or after a wait that covers its stream)."""
         sp = self.spec
         for _ in range(nsteps):
             st = StepStats()
+            self._next_dt = {}
             self._segment("step_begin", self._save_step_start)
             for it in range(sp.sia_maxit):
                 st.sia_iters = it + 1
-                self._segment("setup_p", self.setup_p, (0,))
-                yield 0, self.fp
-                self._segment(f"post_p{self._current(0)}",
+                yield from self._solve(0, "setup_p", self.setup_p, self.fp, (0,), 1.25)
+                self._segment(f"post_p{self._current(0)}@{self.dt!r}",
                               lambda: (self.finish_p(), self.flow()))
                 for s in range(sp.species):
-                    self._segment(f"setup_c{s}", lambda s=s: self.setup_c(s), (1,))
                     self.ft.assemble = lambda s=s: self.setup_c(s)
-                    yield 1, self.ft
+                    yield from self._solve(1, f"setup_c{s}",
+                                           lambda s=s: self.setup_c(s), self.ft,
+                                           (1,), 1.05)
                     self._segment(f"finish_c{s}.{self._current(1)}",
                                   lambda s=s: self.finish_tr(self.C[s]))
                 if sp.heat:
-                    self._segment("setup_t", self.setup_t, (1,))
                     self.ft.assemble = self.setup_t
-                    yield 1, self.ft
+                    yield from self._solve(1, "setup_t", self.setup_t, self.ft,
+                                           (1,), 1.05)
                     self._segment(f"finish_t{self._current(1)}",
                                   lambda: self.finish_tr(self.T))
                 self._segment("eos", self.eos_update)
@@ -442,7 +496,11 @@ class DeviceScenario:
             self._mass_start()
             yield KIND_WAIT, None
             st.mass = self._reduction(3)
-            self.t += sp.dt
+            st.dt = self.dt
+            self.t += self.dt
+            self.dt_step = self.dt
+            # next dt: the smallest of the solvers' next dt (tse.py:1486-1492)
+            self.dt = min(self._next_dt.values()) if self._next_dt else self.dt
             self.n_steps += 1
             self.history.append(st)
 
@@ -451,7 +509,8 @@ class DeviceScenario:
             max_sweeps=400000, strict=True, progress=None, adaptive=False):
         """Drive ``steps`` with single-system solves; returns per-solve stats.
         sor_p None: the pressure factor comes from the scenario's tuner.
-        strict: raise if a solve stops at max_sweeps above its tolerance.
+        strict: raise if a solve stops at max_sweeps above its tolerance;
+        else the step halves dt and repeats the solve (tse.py:730-740).
         progress: called with (kind, stats) after every solve.
         adaptive: pressure cycles sized from the convergence rate (the
         batched drivers use fixed cycles)."""
@@ -472,6 +531,7 @@ class DeviceScenario:
                 raise RuntimeError(f"{'P' if kind == 0 else 'transport'} solve did"
                                    f" not converge: rel {r.rel_residual:.3e}"
                                    f" after {r.sweeps} sweeps")
+            self.solved(r.converged, r.sweeps, max_sweeps)
             if progress:
                 progress(kind, r)
             out.append((kind, r))
@@ -514,7 +574,7 @@ class DeviceScenario:
         done = torch.cuda.Event()
         done.record(self._copy_stream)
         stage.record_stream(self._copy_stream)
-        t_sim, dt = self.t, sp.dt
+        t_sim, dt = self.t, self.dt_step
         mass = np.array(self.history[-1].mass if self.history else (0.0, 0.0, 0.0))
         dxs = np.full(nx, float(sp.spacing))
         dys = np.full(ny, float(sp.spacing))

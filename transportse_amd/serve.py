@@ -5,8 +5,10 @@ permeability, species) that runs a number of SIA steps to a relative
 residual in mixed precision.  The server
 
 * admits waiting requests in arrival order while their HBM footprint fits
-  the budget (``scenario.field_bytes``) -- fields are uploaded once and stay
-  device-resident across the request's time steps;
+  the budget (``scenario.field_bytes``, plus ``scenario.uniform_bytes`` for
+  the first live request of a grid shape, whose shared uniform fields stay
+  until the last request of that shape finishes) -- fields are uploaded
+  once and stay device-resident across the request's time steps;
 * every tick runs one cycle (``inner`` sweeps + residual + stop test) of the
   pressure batch and one of the transport batch over all requests that are
   in a solve right now, whichever step or Picard iteration each is in;
@@ -29,7 +31,8 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from .batch import KIND_P, KIND_TR, Batch
-from .scenario import DeviceScenario, ScenarioSpec, field_bytes, resume_all
+from .scenario import (DeviceScenario, ScenarioSpec, field_bytes, resume_all,
+                       uniform_bytes)
 
 GRIDS = ((64, 64, 64), (128, 128, 128), (256, 256, 128))
 
@@ -133,7 +136,8 @@ class Server:
                 form, tol, inner, sor, max_sweeps = self.p_args
                 bp.configure(form, plan, inner, tol, sor, max_sweeps)
         waiting: deque = deque()
-        live = {}            # rid -> [request, scenario, generator, kind, slot]
+        live = {}            # rid -> [request, scenario, generator, kind, slots, need]
+        shapes = {}          # grid shape -> live requests of that shape
         used = 0
         peak = 0
         max_conc = 0
@@ -151,10 +155,15 @@ class Server:
                 # admission by HBM footprint, in arrival order
                 while waiting:
                     r = waiting[0]
-                    need = field_bytes(r.spec)
+                    shp = (r.spec.nx, r.spec.ny, r.spec.nz)
+                    # the first live request of a shape also brings the
+                    # shape's shared uniform fields
+                    need = field_bytes(r.spec) + (0 if shapes.get(shp) else
+                                                  uniform_bytes(r.spec))
                     if not admissible(need, used, self.budget, len(live)):
                         break
                     waiting.popleft()
+                    shapes[shp] = shapes.get(shp, 0) + 1
                     used += need
                     peak = max(peak, used)
                     r.admitted = now()
@@ -186,15 +195,17 @@ class Server:
                     st = b.status(slots[kind])
                     if st.active:
                         continue
+                    # a solve that did not converge halves the request's dt
+                    # and comes back as the same stage
+                    scn.solved(st.converged, st.sweeps, b.max_sweeps)
+                    r.sweeps += st.sweeps
+                    fin.append(rid)
                     if not st.converged:
-                        raise RuntimeError(f"request {rid}: solve did not converge"
-                                           f" (rel {st.rel_residual:.3e})")
+                        continue
                     b.writeback(slots[kind])
                     if kind == KIND_P:
                         scn.omega_p.report(st.sweeps)
                     r.solves += 1
-                    r.sweeps += st.sweeps
-                    fin.append(rid)
                 nxt_items = resume_all({rid: live[rid][2] for rid in fin}, fin)
                 for rid in fin:
                     r, scn, gen, kind, slots, need = live[rid]
@@ -208,7 +219,12 @@ class Server:
                         if self.keep:
                             r.result = scn.download()
                         scn.close()
-                        used -= need
+                        used -= field_bytes(r.spec)
+                        shp = (r.spec.nx, r.spec.ny, r.spec.nz)
+                        shapes[shp] -= 1
+                        if shapes[shp] == 0:     # its shared fields go too
+                            del shapes[shp]
+                            used -= uniform_bytes(r.spec)
                         del live[rid]
                         done.append(r)
                         continue

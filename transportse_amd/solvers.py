@@ -21,6 +21,13 @@ reads ``c_r`` in compute_P, ``lf_l``/``lf_t`` in compute_C and ``l_r``,
 solvers reproduce that: they read those arrays from ``frozen_source`` at the
 first call of the function that uses them and keep the device copy.
 
+Forms (``p_form``, ``c_form``, ``t_form``): ``"faithful64"`` runs the
+reference's iteration operation for operation (bit-identical); ``"plain64"``
+and ``"mixed"`` solve the fixed point of the same update to the relative
+residual given as P_conv / C_conv / T_conv (FP64 sweeps, or FP32 sweeps in
+FP64 defect correction), with the reference's accept / grow / halve / reset
+time-step control around each attempt and max_it bounding its sweeps.
+
 Device residency: inputs are cached on the device per argument role. A
 cached copy is reused while the same host array (same buffer, strides and
 shape) is passed and no host mutation was signalled; ``invalidate()``
@@ -52,7 +59,8 @@ class HipSolvers:
     def __init__(self, nx, ny, nz, dx, dy, dz, g, beta_deg, frozen_source=None,
                  maxit_p=20000, maxit_c=100, maxit_t=100, dt_max=1e99,
                  c_imblflo=False, t_imblflo=False, device=0,
-                 auto_invalidate=True, verbose=True, p_form="faithful64"):
+                 auto_invalidate=True, verbose=True, p_form="faithful64",
+                 c_form="faithful64", t_form="faithful64"):
         self.hip = HipContext(nx, ny, nz, dx, dy, dz, g, beta_deg, device)
         self.lib = self.hip.lib
         self.shape = self.hip.shape
@@ -62,7 +70,14 @@ class HipSolvers:
         self.c_imblflo, self.t_imblflo = bool(c_imblflo), bool(t_imblflo)
         self.auto_invalidate = auto_invalidate
         self.verbose = verbose
-        self.p_form = p_form
+        for name, form in (("p_form", p_form), ("c_form", c_form),
+                           ("t_form", t_form)):
+            if form not in _FORMS:
+                raise ValueError(f"{name} must be one of {sorted(_FORMS)}")
+        # faithful64: the reference's iteration, bit-identical; plain64 /
+        # mixed: the same update's fixed point to the relative residual
+        # P_conv / C_conv / T_conv, with the reference's dt control
+        self.p_form, self.c_form, self.t_form = p_form, c_form, t_form
         self._gen = 0
         self._cache = {}   # role -> [key, gen, tensor, host_ref]
         self._frozen = {}  # name -> tensor
@@ -237,7 +252,7 @@ class HipSolvers:
         prm = _lib.SorParams(dt=float(dt_C), sor=float(SOR), omega=float(omega),
                              inc_it=float(inc_it), max_dt=float(max_dt),
                              min_dt=float(min_dt), conv=float(C_conv),
-                             max_it=int(max_it), form=0)
+                             max_it=int(max_it), form=_FORMS[self.c_form])
         res = _lib.Result()
         _lib.check(self.lib.tse_compute_c(self.hip.ctx, ctypes.byref(f),
                                           ctypes.byref(prm), ctypes.byref(res)),
@@ -267,7 +282,7 @@ class HipSolvers:
         prm = _lib.SorParams(dt=float(dt_T), sor=float(SOR), omega=float(omega),
                              inc_it=float(inc_it), max_dt=float(max_dt),
                              min_dt=float(min_dt), conv=float(T_conv),
-                             max_it=int(max_it), form=0)
+                             max_it=int(max_it), form=_FORMS[self.t_form])
         res = _lib.Result()
         _lib.check(self.lib.tse_compute_t(self.hip.ctx, ctypes.byref(f),
                                           ctypes.byref(prm), ctypes.byref(res)),
