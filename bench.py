@@ -11,10 +11,17 @@ sweeps, FP64 residual passes and write-back included; inputs resident in HBM).
 
 Reported (one JSON line on stdout):
   value        SOR cell-updates/s over the whole timed steps
-  roofline     dominant kernel (FP32 fused red/black sweep), algorithmic
-               28 B per cell update per sweep, timed with HIP events on the
-               library's stream
-  cpu_baseline the oracle's OpenMP red/black sweeps on the host cores
+  roofline     dominant kernel (FP32 fused red/black sweep, NS sweeps per
+               launch), timed with HIP events on the library's stream:
+               achieved = algorithmic bytes per launch (one pass of the
+               operands, 28 B per cell) / mean launch time, frac = achieved /
+               8 TB/s; traffic = HBM bytes per launch measured in this run by
+               two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE, each
+               scaled by its reading for a 1 GiB copy); the per-sweep figure
+               (every fused sweep at 28 B per cell) is effective_per_sweep_*
+  cpu_baseline the oracle's OpenMP red/black sweeps on the host cores, and
+               the time to solution of one compute_P call (oracle, faithful
+               and mixed GPU forms) on the 64x32x32 problem
   extra keys   time to solution, solves/s, sweeps, the plain FP64 form's time
                to solution on the same problem and the speedup over it, and
                the mixed solution's deviation from the FP64 solution.
@@ -22,9 +29,9 @@ Reported (one JSON line on stdout):
 Further sections of the same line (each bounded; --no-configs / --no-sia /
 --no-ensemble / --no-serve skip them):
   configs_faithful64  the salt 64x32x32 config (100 steps) and the salt +
-               heat SIA 256x256x128 config (2 steps) through the SIA driver
+               heat SIA 256x256x128 config (20 steps) through the SIA driver
                with the faithful FP64 solvers; the 64x32x32 one also with the
-               oracle on the host cores (10 steps)
+               oracle on the host cores (100 steps)
   sia          device-resident SIA steps (heat + salt + tracer) on the
                512x512x256 problem in the mixed form and in plain FP64 solved
                to 1e-13, and the relative L2 / max error of P, T and each C
@@ -34,9 +41,11 @@ Further sections of the same line (each bounded; --no-configs / --no-sia /
                batched solves with hipGraph-replayed persistent tile-queue
                cycles and autotuned plans: cell-updates/s, s per step
   serving      Poisson request arrivals (grids 64^3 / 128^3 / 256x256x128,
-               heat + 3 species, 10 SIA steps each, mixed to 1e-10) under a
-               200 GB HBM admission budget: p50 / p99 time to solution and
-               solves/s at the given arrival rate(s)
+               heat + 3 species, 10 SIA steps each, mixed to 1e-10) over a
+               3 s window per rate (8..24 requests) under a 200 GB HBM
+               admission budget, plus the top rate under a 12 GB budget that
+               makes admission bind: p50 / p99 time to solution split into
+               queueing and service, solves/s, concurrency
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
        python bench.py --profile-kernel N   (N launches of the sweep kernel
@@ -59,20 +68,10 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 BYTES_FP64_SWEEP = 56
-# HBM bytes per launch of the FP32 sweep kernel at 512x512x256 for the plans
-# the autotuner picks there (variant 3, and variant 2 before it), with the
-# operator arrays at their padded distance: TCC_EA0_RDREQ_128B x 128 +
-# WRITE_SIZE, each calibrated on a 1 GiB copy
-# (profiles/rb_zmarch_x2_fp32_pressure_512x512x256.md,
-# profiles/rb_zmarch_xr_fp32_pressure_512x512x256.md).  Other plans report
-# traffic null.
-TRAFFIC_PER_LAUNCH = {(512, 512, 256, 3, 32, 2, 0, 2): 2.652e9,
-                      (512, 512, 256, 3, 32, 2, 0, 3): 2.509e9}
-# sweep kernel and its profile summary per plan variant
-VARIANT_KERNEL = {0: ("rb_zmarch_kernel<float>", "rb_zmarch"),
-                  1: ("rb_zmarch_xl_kernel<float>", "rb_zmarch_xl"),
-                  2: ("rb_zmarch_xr_kernel<float>", "rb_zmarch_xr"),
-                  3: ("rb_zmarch_x2_kernel<float>", "rb_zmarch_x2")}
+# sweep kernel (name as rocprofv3 reports it, without template arguments)
+# per plan variant
+VARIANT_KERNEL = {0: "rb_zmarch_kernel", 1: "rb_zmarch_xl_kernel",
+                  2: "rb_zmarch_xr_kernel", 3: "rb_zmarch_x2_kernel"}
 
 
 def log(*a):
@@ -92,13 +91,16 @@ def main(argv=None):
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sweeps", type=int, default=8)
     ap.add_argument("--profile-kernel", type=int, default=0)
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the rocprofv3 --pmc runs that measure the sweep "
+                         "kernel's HBM traffic")
     ap.add_argument("--profile-faithful", type=int, default=0)
     ap.add_argument("--plan", type=int, nargs="+", default=None,
                     metavar="N", help="FP32 sweep plan NS BY RY ZCHUNK [VARIANT]"
                     " instead of autotuning")
     ap.add_argument("--no-configs", action="store_true")
     ap.add_argument("--a-steps", type=int, default=100)
-    ap.add_argument("--a-cpu-steps", type=int, default=10)
+    ap.add_argument("--a-cpu-steps", type=int, default=100)
     ap.add_argument("--b-steps", type=int, default=20)
     ap.add_argument("--no-sia", action="store_true")
     ap.add_argument("--sia-steps", type=int, default=1)
@@ -108,7 +110,12 @@ def main(argv=None):
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-rates", type=float, nargs="+",
                     default=[5.0, 10.0, 20.0, 50.0, 100.0, 200.0])
-    ap.add_argument("--serve-requests", type=int, default=12)
+    ap.add_argument("--serve-window", type=float, default=3.0,
+                    help="seconds over which a rate's requests arrive")
+    ap.add_argument("--serve-min-requests", type=int, default=8)
+    ap.add_argument("--serve-max-requests", type=int, default=24)
+    ap.add_argument("--serve-tight-gb", type=float, default=12.0,
+                    help="admission budget (GB) of the extra run at the top rate")
     ap.add_argument("--serve-steps", type=int, default=10)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -194,6 +201,7 @@ def main(argv=None):
     P_mixed = fp.result()
 
     # dominant kernel: the FP32 fused sweep, HIP events on the library stream
+    # (the context issues on torch's current stream, where the events go)
     stream = torch.cuda.current_stream()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
@@ -205,10 +213,14 @@ def main(argv=None):
     ev1.record(stream)
     torch.cuda.synchronize()
     launch_s = ev0.elapsed_time(ev1) * 1e-3 / launches
-    bytes_launch = BYTES_FP32_SWEEP * N * ns
+    # algorithmic bytes of one launch: every operand once (28 B per cell); the
+    # ns sweeps a launch fuses share that one pass
+    bytes_launch = BYTES_FP32_SWEEP * N
     achieved = bytes_launch / launch_s / 1e9
+    per_sweep = BYTES_FP32_SWEEP * N * ns / launch_s / 1e9
     log(f"sweep kernel: {launch_s * 1e6:.1f} us/launch ({ns} sweeps), "
-        f"{achieved:.0f} GB/s algorithmic")
+        f"{achieved:.0f} GB/s one-pass ({achieved / HBM_PEAK_GBS:.3f} of peak), "
+        f"{per_sweep:.0f} GB/s counting every sweep at 28 B/cell")
 
     plain = None
     if not args.no_compare:
@@ -259,22 +271,28 @@ def main(argv=None):
         orc.set_threads(threads)
         log(f"cpu 1 thread {cpu['one_thread']['value']:.3e}; other grids "
             + ", ".join(f"{k} {v['value']:.3e}" for k, v in cpu["other_grids"].items()))
+        cpu["compute_P_time_to_solution"] = compute_p_tts(orc, threads)
+        orc.set_threads(threads)
 
     del fp
     torch.cuda.empty_cache()
+    traffic = None if args.no_traffic else measure_traffic(args, plan32)
     cfgs = None if args.no_configs else configs_section(args)
     sia = None if args.no_sia else sia_section(args, nx, ny, nz)
     ens = None if args.no_ensemble else ensemble_section(args)
     serving = None if args.no_serve else serve_section(args)
 
-    traffic = TRAFFIC_PER_LAUNCH.get((nx, ny, nz) + tuple(plan32) + (0,) * (5 - len(plan32)))
     variant = plan32[4] if len(plan32) > 4 else 0
+    tbytes = traffic.get("bytes_per_launch") if traffic else None
     srv0 = serving["runs"][0] if serving and serving.get("runs") else None
     line = {
         "metric": "SOR cell-updates/s & HBM roofline %; solves/s + p99 at rel. "
                   "residual 1e-10",
         "value_is": "SOR cell-updates/s of the 512x512x256 mixed solve",
         "hbm_roofline_pct": 100.0 * achieved / HBM_PEAK_GBS,
+        "hbm_roofline_pct_definition": "one pass of the sweep operands "
+                                       "(28 B x cells) per launch / launch time "
+                                       "/ 8 TB/s",
         "serving_solves_per_s": srv0["solves_per_s"] if srv0 else None,
         "serving_p99_s": srv0["p99_s"] if srv0 else None,
         "serving_rate_per_s": srv0["rate_per_s"] if srv0 else None,
@@ -304,20 +322,24 @@ def main(argv=None):
         "plain64": plain,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "traffic_source": "rocprofv3 --pmc of the same plan, profiles/"
-                                       + VARIANT_KERNEL[variant][1]
-                                       + "_fp32_pressure_512x512x256.md"
-                                       if traffic else None,
-                     "kernel": VARIANT_KERNEL[variant][0],
-                     "bytes_per_update": BYTES_FP32_SWEEP,
+                     "traffic": tbytes,
+                     "kernel": VARIANT_KERNEL[variant],
+                     "algorithmic_bytes_per_launch": bytes_launch,
+                     "algorithmic_bytes_definition": "28 B per cell (3 face "
+                         "transmissibilities, 1/diag, rhs, unknown in and out, "
+                         "FP32), once per launch",
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6,
-                     # achieved counts every sweep at 28 B per cell; a launch
-                     # fuses ns sweeps, so the HBM bytes it really moves
-                     # (traffic) over its time is the memory system's rate
-                     "hbm_measured_gbs": traffic / launch_s / 1e9 if traffic else None,
-                     "hbm_measured_frac": (traffic / launch_s / 1e9 / HBM_PEAK_GBS
-                                           if traffic else None)},
+                     # every fused sweep counted at 28 B per cell: the rate a
+                     # one-sweep-per-launch kernel would need for the same
+                     # sweep throughput (temporal blocking puts it above peak)
+                     "effective_per_sweep_gbs": per_sweep,
+                     "effective_per_sweep_frac": per_sweep / HBM_PEAK_GBS,
+                     "traffic_over_algorithmic": (tbytes / bytes_launch
+                                                  if tbytes else None),
+                     "hbm_measured_gbs": tbytes / launch_s / 1e9 if tbytes else None,
+                     "hbm_measured_frac": (tbytes / launch_s / 1e9 / HBM_PEAK_GBS
+                                           if tbytes else None),
+                     "traffic_measurement": traffic},
         "cpu_baseline": cpu,
         "configs_faithful64": cfgs,
         "sia": sia,
@@ -326,6 +348,84 @@ def main(argv=None):
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def _pmc_per_dispatch(outdir, counter):
+    """{kernel name: [value per dispatch]} from a rocprofv3 --pmc CSV run."""
+    import csv
+    import glob
+    files = glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"),
+                      recursive=True)
+    if not files:
+        raise RuntimeError(f"no counter_collection.csv under {outdir}")
+    per = {}
+    for fn in files:
+        with open(fn, newline="") as fh:
+            for row in csv.DictReader(fh):
+                if row.get("Counter_Name") != counter:
+                    continue
+                key = (row.get("Dispatch_Id"), row.get("Kernel_Name", ""))
+                per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    out = {}
+    for (_, name), v in per.items():
+        out.setdefault(name, []).append(v)
+    return out
+
+
+def measure_traffic(args, plan, launches=6):
+    """HBM bytes per launch of the FP32 sweep kernel with `plan` on this
+    problem, measured in this run: two rocprofv3 --pmc child runs of
+    ``bench.py --profile-kernel`` (FETCH_SIZE, then WRITE_SIZE, in KiB per
+    dispatch), each counter scaled by what it reads for the 1 GiB device
+    copy the same child makes (2^30 bytes read and 2^30 written).  Returns
+    the figures, or {"error": ...} (also printed) if a run fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    name = VARIANT_KERNEL[plan[4] if len(plan) > 4 else 0]
+    child = [sys.executable, os.path.join(ROOT, "bench.py"), "--profile-kernel",
+             str(launches), "--grid", *map(str, args.grid), "--plan", *map(str, plan)]
+    res = {"kernel": name, "launches": launches, "plan": list(plan),
+           "method": "rocprofv3 --pmc FETCH_SIZE, then WRITE_SIZE, over "
+                     + " ".join(["bench.py"] + child[2:])
+                     + "; each scaled by the same counter on a 1 GiB device copy"}
+    env = dict(os.environ, TMPDIR="/tmp")
+    t0 = time.time()
+    try:
+        for counter, key in (("FETCH_SIZE", "read"), ("WRITE_SIZE", "write")):
+            d = tempfile.mkdtemp(prefix="tse_pmc_", dir="/tmp")
+            cmd = ["timeout", "-s", "KILL", "300", prof, "--pmc", counter,
+                   "--output-format", "csv", "-d", d, "-o", "p", "--", *child]
+            p = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True,
+                               text=True)
+            try:
+                vals = _pmc_per_dispatch(d, counter)
+            except RuntimeError:
+                raise RuntimeError(f"{counter} run rc {p.returncode}: "
+                                   f"{p.stderr[-800:]}")
+            finally:
+                shutil.rmtree(d, ignore_errors=True)
+            sweep = [v for n, v in vals.items() if name + "<" in n or n.endswith(name)]
+            copy = [v for n, v in vals.items() if "copy" in n.lower()]
+            if not sweep or not copy:
+                raise RuntimeError(f"{counter}: kernels seen {sorted(vals)[:6]}")
+            k_sweep = [x for v in sweep for x in v]
+            k_copy = max(x for v in copy for x in v)
+            factor = float(2 ** 30) / (k_copy * 1024.0)
+            mean = sum(k_sweep) / len(k_sweep)
+            res[key] = {"counter": counter, "kib_per_dispatch": mean,
+                        "dispatches": len(k_sweep), "copy_kib": k_copy,
+                        "copy_factor": factor, "bytes": mean * 1024.0 * factor}
+        res["bytes_per_launch"] = res["read"]["bytes"] + res["write"]["bytes"]
+        log(f"traffic of {name}: {res['bytes_per_launch'] / 1e9:.3f} GB per launch "
+            f"(read factor {res['read']['copy_factor']:.2f}, write factor "
+            f"{res['write']['copy_factor']:.2f}) in {time.time() - t0:.0f}s")
+    except Exception as e:   # reported in the JSON line, never silently
+        res["error"] = str(e)
+        res["bytes_per_launch"] = None
+        log(f"TRAFFIC MEASUREMENT FAILED: {e}")
+    return res
 
 
 def cpu_sweep_rate(orc, prob, dt, omega, threads, sweeps):
@@ -344,6 +444,50 @@ def cpu_sweep_rate(orc, prob, dt, omega, threads, sweeps):
     n = prob.nx * prob.ny * prob.nz
     return {"value": n * sweeps / sec, "unit": "cell-updates/s", "cores": threads,
             "sweeps": sweeps, "seconds": sec}
+
+
+def compute_p_tts(orc, threads):
+    """Time to solution of one compute_P call (tse.py:647-742, the drop-in
+    signature) on the 64x32x32 salt problem from its hydrostatic state (dt
+    1 day, SOR 1.8, inc_it 1.25, P_conv 1e-8): the oracle on the host cores,
+    the faithful GPU form (bit-identical, same iterations) and the mixed GPU
+    form (P_conv read as relative residual); host arrays in and out, so the
+    GPU times include the transfers."""
+    import numpy as np
+    import torch
+
+    from transportse_amd.solvers import HipSolvers
+    from transportse_amd.workloads import PressureProblem
+    prob = PressureProblem(64, 32, 32, spacing=10.0, sigma=1.0, corr_len=5, seed=1)
+    args_ = lambda: (prob.c_f, prob.nue, prob.rho_f, prob.phi, prob.Kx, prob.Ky,
+                     prob.Kz, prob.qx, prob.qy, prob.qz, 86400.0, prob.BP, prob.WP)
+    kw = dict(SOR=1.8, inc_it=1.25, P_conv=1e-8, max_it=20000)
+    out = {"grid": [64, 32, 32], "P_conv": 1e-8, "SOR": 1.8}
+    orc.set_threads(threads)
+    o = orc.Oracle(*prob.grid_args(), frozen_source=prob)
+    P = prob.P.copy()
+    t0 = time.perf_counter()
+    r = o.compute_P(P, *args_(), **kw)
+    out["oracle"] = {"seconds": time.perf_counter() - t0, "iterations": r[5],
+                     "cores": threads}
+    P_ref = P
+    for form in ("faithful64", "mixed"):
+        h = HipSolvers(*prob.grid_args(), frozen_source=prob, verbose=False,
+                       p_form=form)
+        for rep in range(2):          # the second call is timed
+            P = prob.P.copy()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = h.compute_P(P, *args_(), **kw)
+            torch.cuda.synchronize()
+            sec = time.perf_counter() - t0
+        out[form] = {"seconds": sec, "iterations": r[5], "converged": r[4],
+                     "max_abs_dev_from_oracle_Pa": float(np.max(np.abs(P - P_ref)))}
+        del h
+    log("compute_P time to solution 64x32x32: oracle "
+        f"{out['oracle']['seconds']:.3f}s ({out['oracle']['iterations']} it), GPU "
+        f"faithful {out['faithful64']['seconds']:.3f}s, mixed {out['mixed']['seconds']:.3f}s")
+    return out
 
 
 def profile_faithful(args, prob, dt):
@@ -536,31 +680,60 @@ def ensemble_section(args):
 
 
 def serve_section(args):
+    """Serving at each arrival rate: requests arrive over a window of
+    --serve-window seconds (at least --serve-min-requests, at most
+    --serve-max-requests of them), the same request list (grids, seeds) at
+    every rate, with the 200 GB admission budget; one more run at the
+    highest rate with a budget of --serve-tight-gb so that admission binds.
+    Each run reports latency (arrival to completion) split into queueing
+    (arrival to admission) and service (admission to completion)."""
+    import math
+
     import torch
 
     from transportse_amd.serve import Server, make_requests, prepare
+    count = lambda rate: int(min(args.serve_max_requests,
+                                 max(args.serve_min_requests,
+                                     math.ceil(rate * args.serve_window))))
+    cache = {}
+    t0 = time.time()
+    prepare(make_requests(max(count(r) for r in args.serve_rates), 1.0,
+                          steps=args.serve_steps), cache)
+    log(f"serving: {len(cache)} request payloads prepared in {time.time() - t0:.1f}s")
     out = []
-    for rate in args.serve_rates:
-        reqs = make_requests(args.serve_requests, rate, steps=args.serve_steps)
-        t0 = time.time()
-        prepare(reqs)
-        log(f"serving: {len(reqs)} requests at {rate}/s prepared in "
-            f"{time.time() - t0:.1f}s")
-        srv = Server(budget_bytes=200e9)
+    runs = [(rate, 200e9) for rate in args.serve_rates]
+    runs.append((max(args.serve_rates), args.serve_tight_gb * 1e9))
+    for rate, budget in runs:
+        reqs = make_requests(count(rate), rate, steps=args.serve_steps)
+        prepare(reqs, cache)
+        srv = Server(budget_bytes=budget)
         st = srv.run(reqs)
         srv.close()
         del srv, reqs
         torch.cuda.empty_cache()
-        log(f"serving {rate}/s: p50 {st.p50:.2f}s p99 {st.p99:.2f}s, "
-            f"{st.solves_per_s:.2f} solves/s, max {st.max_concurrent} concurrent")
-        out.append({"rate_per_s": rate, "requests": st.completed,
+        log(f"serving {rate}/s budget {budget / 1e9:.0f} GB: {st.completed} requests,"
+            f" p50 {st.p50:.2f}s p99 {st.p99:.2f}s (queue p99 {st.queue_p99:.2f}s,"
+            f" service p99 {st.service_p99:.2f}s), {st.solves_per_s:.2f} solves/s,"
+            f" max {st.max_concurrent} concurrent, {st.waited} waited")
+        out.append({"rate_per_s": rate, "budget_bytes": budget,
+                    "requests": st.completed,
+                    "arrival_window_s": count(rate) / rate,
                     "p50_s": st.p50, "p99_s": st.p99, "mean_s": st.mean,
+                    "queue_p50_s": st.queue_p50, "queue_p99_s": st.queue_p99,
+                    "service_p50_s": st.service_p50,
+                    "service_p99_s": st.service_p99,
+                    "waited_for_admission": st.waited,
                     "solves_per_s": st.solves_per_s, "seconds": st.seconds,
                     "peak_admitted_bytes": st.peak_bytes,
                     "max_concurrent": st.max_concurrent})
+    cap = max(r["solves_per_s"] for r in out)
     return {"budget_bytes": 200e9, "steps_per_request": args.serve_steps,
             "grids": [[64, 64, 64], [128, 128, 128], [256, 256, 128]],
             "species": 3, "heat": True, "tol": 1e-10, "form": "mixed",
+            "capacity_solves_per_s": cap,
+            "note": "every offered rate above capacity_solves_per_s overloads the "
+                    "server: the queue grows over the arrival window, so latency "
+                    "grows with the rate and the window (no steady state)",
             "runs": out}
 
 

@@ -51,12 +51,21 @@ class Request:
     result: dict | None = None
 
 
-def prepare(requests) -> None:
-    """Build each request's input fields (the payload a client sends)."""
+def prepare(requests, cache: dict | None = None) -> None:
+    """Build each request's input fields (the payload a client sends).  With
+    `cache`, payloads are kept by scenario (grid, seeds, physics) and reused
+    by requests of the same scenario; the server only reads them."""
     from .scenario import host_fields
     for r in requests:
-        if r.fields is None:
+        if r.fields is not None:
+            continue
+        if cache is None:
             r.fields = host_fields(r.spec)
+            continue
+        key = repr(r.spec)
+        if key not in cache:
+            cache[key] = host_fields(r.spec)
+        r.fields = cache[key]
 
 
 def make_requests(n: int, rate: float, steps: int = 10, species: int = 3,
@@ -99,6 +108,12 @@ class ServeStats:
     peak_bytes: int
     max_concurrent: int
     latencies: list = field(default_factory=list)
+    # latency = queueing (arrival -> admission) + service (admission -> done)
+    queue_p50: float = float("nan")
+    queue_p99: float = float("nan")
+    service_p50: float = float("nan")
+    service_p99: float = float("nan")
+    waited: int = 0           # requests that waited for admission (> 1 ms)
 
 
 class Server:
@@ -235,13 +250,18 @@ class Server:
                     self.batches[k].rearm_many(lst)
             sec = now()
         lat = np.array([r.done - r.arrival * time_scale for r in done])
+        wait = np.array([r.admitted - r.arrival * time_scale for r in done])
+        serv = np.array([r.done - r.admitted for r in done])
+        pct = lambda a, q: float(np.percentile(a, q)) if len(a) else float("nan")
         return ServeStats(
             rate=0.0, completed=len(done), seconds=sec,
-            p50=float(np.percentile(lat, 50)) if len(lat) else float("nan"),
-            p99=float(np.percentile(lat, 99)) if len(lat) else float("nan"),
+            p50=pct(lat, 50), p99=pct(lat, 99),
             mean=float(lat.mean()) if len(lat) else float("nan"),
             solves_per_s=len(done) / sec if sec > 0 else 0.0, peak_bytes=peak,
-            max_concurrent=max_conc, latencies=lat.tolist())
+            max_concurrent=max_conc, latencies=lat.tolist(),
+            queue_p50=pct(wait, 50), queue_p99=pct(wait, 99),
+            service_p50=pct(serv, 50), service_p99=pct(serv, 99),
+            waited=int(np.sum(wait > 1e-3)))
 
     def _enter(self, scn, kind, sysm, slots, rearm=None):
         """Put a scenario's set-up system into the batch of its kind (a slot
