@@ -28,7 +28,7 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
           maxit_p=0, maxit_c=0, maxit_t=0, disp=(1.0, 0.1), wp_cell=True,
           plot_steps=0, c_f=4.5e-10, salt_patch=True, t_end=None,
           smolar_c=False, smolar_t=False, nlte=False, bp_faces="x",
-          dyn_inplace=False):
+          dyn_inplace=False, plot_times=None):
     rng = numpy.random.default_rng(seed)
     S = species
     ns.update(dict(
@@ -45,8 +45,11 @@ def build(ns, nx=7, ny=6, nz=5, seed=11, heat=False, species=1, sptrans=(0,),
         SPTRANS=numpy.array(sptrans),
         dt=dt0, dt_max=dt_max, OMEGA_C=omega_c, OMEGA_T=omega_t,
         SOR_P=sor_p, SOR_C=sor_c, SOR_T=sor_t, CONV_P=conv_p,
-        CONV_C=conv_c, CONV_T=conv_t, PRINTOUT=1, PLOTSAVE=False,
-        PLOTTIME=numpy.array([0.0]), MAXIT_P=maxit_p, MAXIT_C=maxit_c,
+        CONV_C=conv_c, CONV_T=conv_t, PRINTOUT=1,
+        PLOTSAVE=plot_times is not None,
+        PLOTTIME=numpy.array([0.0] if plot_times is None else plot_times,
+                             dtype=float),
+        MAXIT_P=maxit_p, MAXIT_C=maxit_c,
         MAXIT_T=maxit_t))
     ns['lx'] = float(ns['dx'].sum())
     ns['ly'] = float(ns['dy'].sum())

@@ -8,7 +8,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
 CASES = ("g1_salt", "g2_sia_heat", "g3_halving", "g4_smolar", "g5_nlte",
-         "g6_xz", "g7_column", "g8_xy")
+         "g6_xz", "g7_column", "g8_xy", "g9_plot")
 
 
 def oracle_backend(sim):
@@ -147,3 +147,25 @@ def oracle_sia_steps(spec, nsteps, dts=None):
             rho_prev = rho.copy()
             P, T, C = Pn0.copy(), Tn0.copy(), [c.copy() for c in Cn0]
     return {"P": P, "T": T, "C": C, "rho_f": rho, "l_f": l_f}
+
+
+def check_plot_files(out_dir, case):
+    """The 34-tuple files a driver wrote at PLOTTIME (tse.py:312-329) against
+    the reference's own tuples in tests/golden/<case>.npz: the same number
+    of files, names <case>-<index>.p, and every element equal (arrays bit
+    for bit).  The files are this build's own output, read back with
+    joblib."""
+    from joblib import load
+    g = np.load(f"{GOLDEN}/{case}.npz")
+    count = int(g["plot_count"])
+    files = sorted(f for f in os.listdir(out_dir) if f.endswith(".p"))
+    assert files == [f"{case}-{i:05d}.p" for i in range(count)], files
+    for n, fn in enumerate(files):
+        tup = load(os.path.join(out_dir, fn))
+        assert len(tup) == 34
+        for e, v in enumerate(tup):
+            ref = g[f"plot{n}_e{e:02d}"]
+            got = np.asarray(v)
+            assert got.shape == ref.shape and np.array_equal(got, ref), (fn, e)
+    return count
+

@@ -258,3 +258,18 @@ def test_driver_matches_reference_trajectory(case, tmp_path):
             assert r[k] == int(g[k][s]), (case, s, k)
         assert r["t_sim"] == g["t_end"][s]
     print("driver ok", case, flush=True)
+
+
+def test_driver_output_tuples_match_reference(tmp_path):
+    """PLOTSAVE on the GPU backend (golden g9): the output clamp's dt
+    sequence and every element of the 34-tuples written at PLOTTIME equal
+    the reference's (tse.py:312-329)."""
+    from helpers import check_plot_files
+    from transportse_amd.driver import Simulation, make_hip_backend
+    sim = Simulation(f"{GOLDEN}/configs/g9_plot.py", make_hip_backend,
+                     verbose=False, record_fields=True, out_dir=str(tmp_path))
+    traj = sim.run()
+    g = np.load(f"{GOLDEN}/g9_plot.npz")
+    assert [r["dt"] for r in traj] == g["dt"].tolist()
+    assert check_plot_files(str(tmp_path), "g9_plot") == 3
+    print("output tuples ok: 3 files, dt", [r["dt"] for r in traj], flush=True)

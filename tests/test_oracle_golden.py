@@ -11,7 +11,7 @@ compute_C / compute_T / dispersivity / l2_norm and the driver's control flow
 import numpy as np
 import pytest
 
-from helpers import CASES, GOLDEN, oracle_backend
+from helpers import CASES, GOLDEN, check_plot_files, oracle_backend
 
 
 @pytest.mark.parametrize("case", CASES)
@@ -45,3 +45,19 @@ def test_golden_cases_exercise_the_control_paths():
     g5 = np.load(f"{GOLDEN}/g5_nlte.npz")
     assert g5["T"].ndim == 5 and g5["T"].shape[-1] == 2
     assert not np.array_equal(g5["T"][-1, ..., 0], g5["T"][-1, ..., 1])
+
+
+def test_oracle_driver_output_tuples_match_reference(tmp_path, oracle_mod):
+    """PLOTSAVE with PLOTTIME points between steps (golden g9): the output
+    clamp shortens dt (tse.py:316-321), and the 34-tuples written at each
+    PLOTTIME equal the reference's element by element.  The reference skips
+    PLOTTIME points that a step oversteps (only 3 of its 5 are written), and
+    so does the driver."""
+    from transportse_amd.driver import Simulation
+    sim = Simulation(f"{GOLDEN}/configs/g9_plot.py", oracle_backend,
+                     verbose=False, record_fields=True, out_dir=str(tmp_path))
+    traj = sim.run()
+    g = np.load(f"{GOLDEN}/g9_plot.npz")
+    assert [r["dt"] for r in traj] == g["dt"].tolist()
+    assert check_plot_files(str(tmp_path), "g9_plot") == 3
+
