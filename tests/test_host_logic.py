@@ -73,3 +73,27 @@ def test_strided_views_of_foreign_buffers_take_the_copy_path(tmp_path):
         flat, (si, sj, sk) = _block_and_strides(view)
         i, j, k = np.indices(view.shape)
         assert np.array_equal(flat[i * si + j * sj + k * sk], ref[:, :, :, 1])
+
+
+def test_weak_view_references_follow_their_owner():
+    """The solvers' cache and pending list hold host arrays weakly; a species
+    slice is held through its owner, and rebuilt as the same strided view
+    while the owner lives."""
+    import gc
+
+    import numpy as np
+
+    from transportse_amd.solvers import _held, _holder, _key
+    C = np.arange(4 * 3 * 5 * 2, dtype=np.float64).reshape(4, 3, 5, 2)
+    v = C[:, :, :, 1]
+    h = _holder(v)
+    w = _held(h)
+    assert _key(w) == _key(v) and np.array_equal(w, v)
+    w[0, 0, 0] = -1.0                   # a write through it reaches C
+    assert C[0, 0, 0, 1] == -1.0
+    P = np.zeros((3, 3, 3))
+    hp = _holder(P)
+    assert _held(hp) is P
+    del C, v, w, P
+    gc.collect()
+    assert _held(h) is None and _held(hp) is None

@@ -39,13 +39,14 @@ def test_host_sequential_norm_matches_oracle(oracle_mod):
     from transportse_amd import _lib
     lib = _lib.load()
     rng = np.random.default_rng(0)
-    a, b = rng.standard_normal(1000), rng.standard_normal(1000)
+    o = Oracle(1, 1, 1, np.ones(1), np.ones(1), np.ones(1), 9.8, 0.0)
     dp = ctypes.POINTER(ctypes.c_double)
     flag = ctypes.c_int32()
-    v = lib.tse_host_l2_norm_seq(a.ctypes.data_as(dp), b.ctypes.data_as(dp),
-                                 a.size, ctypes.byref(flag))
-    o = Oracle(1, 1, 1, np.ones(1), np.ones(1), np.ones(1), 9.8, 0.0)
-    assert v == o.l2_norm(a, b)
+    for n in (1000, 3 << 20):   # the second sum runs on a thread above 2^20
+        a, b = rng.standard_normal(n), rng.standard_normal(n)
+        v = lib.tse_host_l2_norm_seq(a.ctypes.data_as(dp), b.ctypes.data_as(dp),
+                                     a.size, ctypes.byref(flag))
+        assert v == o.l2_norm(a, b) and flag.value == 0
 
 
 def test_no_cpu_fallback_without_gpu():

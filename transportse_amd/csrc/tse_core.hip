@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <mutex>
+#include <thread>
 
 #include "tse_internal.h"
 
@@ -679,12 +680,23 @@ int tse_l2_norm(tse_ctx *ctx, const double *New, const double *Old,
 
 double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
                             int32_t *is_nan) {
+    // the two sums are independent: each keeps its sequential order, and on
+    // large arrays the second runs on a thread of its own
     double s1 = 0.0, s2 = 0.0;
+    auto sum2 = [&]() {
+        for (int64_t p = 0; p < n; ++p) s2 += Old[p] * Old[p];
+    };
+    std::thread other;
+    const bool split = n >= (1 << 20);
+    if (split) other = std::thread(sum2);
     for (int64_t p = 0; p < n; ++p) {
         const double d = New[p] - Old[p];
         s1 += d * d;
     }
-    for (int64_t p = 0; p < n; ++p) s2 += Old[p] * Old[p];
+    if (split)
+        other.join();
+    else
+        sum2();
     const double l2 = std::sqrt(s1) / std::sqrt(s2 + 1e-200);
     if (std::isnan(l2)) {
         if (is_nan) *is_nan = 1;

@@ -115,10 +115,18 @@ class HipContext:
         if up is None or up.numel() < n:
             up = self._up[turn] = torch.empty(n, dtype=torch.float64, device=self.device)
         pin_np = pin.numpy()
-        # chunked: the DMA of one chunk from pinned memory overlaps the host
-        # copy of the next into it
-        for lo, hi in _chunks(n, n):
-            np.copyto(pin_np[lo:hi], base[lo:hi])
+        # chunked: host copies into pinned memory run on the copy threads,
+        # and each chunk's DMA starts as soon as its copy is done
+        parts = _chunks(n, n)
+        if len(parts) > 1:
+            futs = [_copier().submit(np.copyto, pin_np[lo:hi], base[lo:hi])
+                    for lo, hi in parts]
+        else:
+            np.copyto(pin_np[:n], base[:n])
+            futs = [None]
+        for f, (lo, hi) in zip(futs, parts):
+            if f is not None:
+                f.result()
             up[lo:hi].copy_(pin[lo:hi], non_blocking=True)
         _lib.check(self.lib.tse_to_split(self.ctx, out.data_ptr(), up.data_ptr(),
                                           *strides), "tse_to_split")
@@ -165,13 +173,56 @@ class HipContext:
         host = pin[:n].numpy().reshape(self.shape)
         if out is None:
             out = np.empty(self.shape, dtype=np.float64)
+        if len(parts) == 1:
+            parts[0][2].synchronize()
+            out[...] = host
+            return out
+        # each chunk is copied out on the copy threads once its DMA is done
+        futs = []
         for i0, i1, ev in parts:
             ev.synchronize()
-            out[i0:i1] = host[i0:i1]
+            futs.append(_copier().submit(np.copyto, out[i0:i1], host[i0:i1]))
+        for f in futs:
+            f.result()
         return out
 
     def synchronize(self):
         _lib.check(self.lib.tse_ctx_synchronize(self.ctx), "tse_ctx_synchronize")
+
+
+_COPY_POOL = None
+
+
+def _copier():
+    """Threads for the host-side copies of the transfers (numpy releases the
+    GIL while it copies)."""
+    global _COPY_POOL
+    if _COPY_POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _COPY_POOL = ThreadPoolExecutor(max_workers=4)
+    return _COPY_POOL
+
+
+def pcopyto(dst: np.ndarray, src) -> None:
+    """``dst[...] = src`` for large arrays, in slabs along the first axis on
+    the copy threads (src broadcast to dst's shape)."""
+    src = np.broadcast_to(np.asarray(src), dst.shape)
+    if dst.ndim == 0 or dst.shape[0] < 4 or dst.size < (1 << 21):
+        dst[...] = src
+        return
+    k = min(4, dst.shape[0])
+    step = -(-dst.shape[0] // k)
+    futs = [_copier().submit(np.copyto, dst[i:i + step], src[i:i + step])
+            for i in range(0, dst.shape[0], step)]
+    for f in futs:
+        f.result()
+
+
+def pcopy(a: np.ndarray) -> np.ndarray:
+    """``a.copy()`` (C order) through pcopyto."""
+    out = np.empty(a.shape, dtype=a.dtype)
+    pcopyto(out, a)
+    return out
 
 
 def _chunks(m: int, n: int):

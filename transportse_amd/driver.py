@@ -31,6 +31,7 @@ import types
 import numpy
 
 from .config_loader import load_config
+from .device import pcopy, pcopyto
 from .snapshot import PickleWriter
 
 # config attributes the reference binds at start-up (tse.py:81-151)
@@ -51,7 +52,8 @@ class Simulation:
     """One TRANSPORTSE run of a config module with an injected backend."""
 
     def __init__(self, config, backend_factory, out_dir=None, verbose=True,
-                 record_fields=False, max_steps=None, eos_mode="auto"):
+                 record_fields=False, max_steps=None, eos_mode="auto",
+                 lazy_host=True):
         if isinstance(config, str):
             self.cfile = os.path.splitext(os.path.basename(config))[0]
             self.cdir = os.path.dirname(os.path.abspath(config))
@@ -99,8 +101,15 @@ class Simulation:
         self.solvers = backend_factory(self)
         if hasattr(self.solvers, "pe_writeback"):
             self.solvers.pe_writeback = False
+        # deferred host copies of what the solvers write (backends that keep
+        # fields on a device): the loop syncs an array before host code
+        # reads it -- a callback gets its arguments current, output and the
+        # recorded trajectory everything -- and copies fields on the device
+        if lazy_host and hasattr(self.solvers, "lazy"):
+            self.solvers.lazy = True
         self.trajectory = []
         self.writer = PickleWriter()
+        self._wc_cache = {}
 
     # ---- EOS (tse.py:178-229) -------------------------------------------------
     def _fp_arrays(self):
@@ -163,7 +172,7 @@ class Simulation:
         changed = []
         for a, v, same in zip(self._fp_arrays(), res, self._eos_same):
             if not same:       # the callback returned its argument unchanged
-                a[...] = v
+                pcopyto(a, v)
                 changed.append(a)
         return changed
 
@@ -206,6 +215,7 @@ class Simulation:
                 self.dt_prev = self.dt
                 self.dt = min(self.dt, t_plot - self.t_sim)
             elif self.t_sim == t_plot:
+                self._sync()
                 path = os.path.join(self.out_dir, "{:s}-{:05d}.p".format(
                     self.cfile, self.PLOTINDEX))
                 # the 34-tuple of tse.py:322-327; arrays are snapshot-copied so
@@ -221,6 +231,46 @@ class Simulation:
                 self.writer.submit(path, snap)
                 self.dt = self.dt_prev
                 self.PLOTINDEX += 1
+
+    def _sync(self, arrays=None):
+        """Make host arrays current (all, or those overlapping `arrays`)."""
+        f = getattr(self.solvers, "sync", None)
+        if f is not None:
+            f(arrays)
+
+    def _copy(self, a):
+        """``a.copy()``, on the backend's device where it holds the value."""
+        f = getattr(self.solvers, "copy", None)
+        return f(a) if f is not None else a.copy()
+
+    def _nonzero_sum(self, a) -> bool:
+        """``numpy.sum(a) != 0.0`` (tse.py:1300-1301)."""
+        f = getattr(self.solvers, "nonzero_sum", None)
+        return f(a) if f is not None else bool(numpy.sum(a) != 0.0)
+
+    def _wc_nonzero(self, n) -> bool:
+        """``numpy.sum(WC[:, :, :, n]) != 0.0`` (tse.py:1301), kept until a
+        callback that receives WC (mass_balance, dynamic_bc) or one that may
+        replace it runs."""
+        k = (id(self.st.WC), n)
+        v = self._wc_cache.get(k)
+        if v is None:
+            v = self._wc_cache[k] = bool(numpy.sum(self.st.WC[:, :, :, n]) != 0.0)
+        return v
+
+    def _store(self, dst, n, val):
+        """``dst[:, :, :, n] = val`` (tse.py:1304, 1345-1353); nothing to do
+        when the solver returned that very slice."""
+        target = dst[:, :, :, n]
+        v = numpy.asarray(val)
+        if (v.__array_interface__["data"][0] == target.__array_interface__["data"][0]
+                and v.strides == target.strides and v.shape == target.shape):
+            return
+        self._sync([v])
+        target[...] = v
+
+    def _lazy(self) -> bool:
+        return bool(getattr(self.solvers, "lazy", False))
 
     def _host_mutated(self, arrays=None):
         """Tell the backend that host code changed `arrays` (None: any)."""
@@ -239,6 +289,7 @@ class Simulation:
         try:
             return self._run()
         finally:
+            self._sync()
             self.writer.flush()
 
     def _run(self):
@@ -275,6 +326,7 @@ class Simulation:
                 if c.FLUIDFLOW:
                     while not Co_Filled:
                         t0 = time.perf_counter()
+                        Pn = None    # replaced below; the backend need not keep it
                         self.dt, dt_P, st.P, Pn, converged_P, num_it_P = \
                             sv.compute_P(st.P, st.c_f, st.nue, st.rho_f,
                                          st.phi, st.Kx, st.Ky, st.Kz, st.qx,
@@ -289,6 +341,7 @@ class Simulation:
                             st.P, st.BP, st.Kx, st.Ky, st.Kz, st.qx, st.qy,
                             st.qz, st.nue, st.rho_f)
                         if c.HEATTRANS or c.SPECTRANS:
+                            st.e = None
                             st.e = sv.flow_error(st.P, Pn, st.qx, st.qy, st.qz,
                                                  st.WP, st.phi, st.c_f, st.c_r,
                                                  st.rho_f, self.dt)
@@ -316,6 +369,7 @@ class Simulation:
                         else:
                             st.dt_max = min(dt_CFL, dt_max_C, dt_max_T, dt_max_P)
                         if n_tsteps % c.PRINTOUT == 0 and self.verbose:
+                            self._sync([st.P, st.T, st.C])
                             if c.OMEGA_C + c.OMEGA_T < 2:
                                 print(' [dt_NeC] %8.2e [dt_NeT] %8.2e' % (dt_max_C, dt_max_T))
                             print(' [CFL] %6.2f%% | Pe_T %.1e/%.1e/%.1e | Pe_C %.1e/%.1e/%.1e'
@@ -335,18 +389,19 @@ class Simulation:
 
                 num_it_C = 0
                 if c.SPECTRANS:
-                    Cn = st.C.copy()
+                    Cn = self._copy(st.C)
                     for n in c.SPTRANS:
-                        if numpy.sum(st.C[:, :, :, n]) != 0.0 \
-                                or numpy.sum(st.WC[:, :, :, n]) != 0.0:
+                        if self._nonzero_sum(st.C[:, :, :, n]) \
+                                or self._wc_nonzero(n):
                             t0 = time.perf_counter()
-                            self.dt, dt_C, st.C[:, :, :, n], converged_C, num_itC = \
+                            self.dt, dt_C, res_C, converged_C, num_itC = \
                                 sv.compute_C(st.C[:, :, :, n], st.D_fx[:, :, :, n],
                                              st.D_fy[:, :, :, n], st.D_fz[:, :, :, n],
                                              st.phi, st.rho_f, st.qx, st.qy, st.qz,
                                              self.dt, st.BC, st.WC[:, :, :, n], st.R,
                                              st.e, omega=c.OMEGA_C, SOR=c.SOR_C,
                                              max_it=self.MAXIT_C, C_conv=c.CONV_C)
+                            self._store(st.C, n, res_C)
                             solve_time += time.perf_counter() - t0
                             num_it_C += num_itC
                             if not converged_C:
@@ -357,7 +412,7 @@ class Simulation:
                                 st.vx, st.vy, st.vz = sv.smolarkiewicz(
                                     st.C[:, :, :, n], st.qx, st.qy, st.qz, st.vx,
                                     st.vy, st.vz, self.dt, eps=1e-15, Sc=1.08)
-                                self.dt, dt_C, st.C[:, :, :, n], converged_C, num_itC = \
+                                self.dt, dt_C, res_C, converged_C, num_itC = \
                                     sv.compute_C(st.C[:, :, :, n], st.D_fx[:, :, :, n],
                                                  st.D_fy[:, :, :, n], st.D_fz[:, :, :, n],
                                                  st.phi, st.rho_f, st.vx, st.vy, st.vz,
@@ -365,6 +420,7 @@ class Simulation:
                                                  st.ZEROS, omega=c.OMEGA_C,
                                                  max_it=self.MAXIT_C, SOR=c.SOR_C,
                                                  C_conv=c.CONV_C)
+                                self._store(st.C, n, res_C)
                                 solve_time += time.perf_counter() - t0
                                 num_it_C += num_itC
                                 if not converged_C:
@@ -373,23 +429,25 @@ class Simulation:
 
                 num_it_T = 0
                 if c.HEATTRANS:
-                    Tn = st.T.copy()
+                    Tn = self._copy(st.T)
                     t0 = time.perf_counter()
                     if self.NLTE:  # tse.py:1343-1354
                         # fluid layer: unit porosity, no source
-                        self.dt, dt_T, st.T[:, :, :, 0], converged_T, num_itT1 = \
+                        self.dt, dt_T, res_T, converged_T, num_itT1 = \
                             sv.compute_T(st.T[:, :, :, 0], st.l_f, st.cp_f, st.rho_f,
                                          st.ONES, st.qx, st.qy, st.qz, self.dt,
                                          st.BT, st.ZEROS, st.e, omega=c.OMEGA_T,
                                          max_it=self.MAXIT_T, SOR=c.SOR_T,
                                          T_conv=c.CONV_T)
+                        self._store(st.T, 0, res_T)
                         # rock layer: zero fluid properties
-                        self.dt, dt_T, st.T[:, :, :, 1], converged_T, num_itT2 = \
+                        self.dt, dt_T, res_T1, converged_T, num_itT2 = \
                             sv.compute_T(st.T[:, :, :, 1], st.ZEROS, st.ZEROS,
                                          st.ZEROS, st.ZEROS, st.qx, st.qy, st.qz,
                                          self.dt, st.BT, st.WT, st.e,
                                          omega=c.OMEGA_T, max_it=self.MAXIT_T,
                                          SOR=c.SOR_T, T_conv=c.CONV_T)
+                        self._store(st.T, 1, res_T1)
                         num_itT = num_itT1 + num_itT2
                     else:
                         self.dt, dt_T, st.T, converged_T, num_itT = \
@@ -432,11 +490,14 @@ class Simulation:
                                  for j in range(st.ny) for i in range(st.nx)
                                  if st.BC[i, j, k] != 1]
                     if chem_list != [] and chem_inc == 0:
+                        self._sync()
+                        self._wc_cache.clear()
                         self._chemistry(chem_list, dt_chem)
                         self._host_mutated([st.C, st.P, st.T, st.rho_f, st.phi,
                                             st.Kx, st.Ky, st.Kz])
 
                 if c.FLUIDPROP:
+                    self._sync(self._fp_arrays())
                     self._host_mutated(self.update_fluid_properties())
 
                 if c.SIA:
@@ -446,15 +507,17 @@ class Simulation:
                     if SIA_diff < c.SIA_conv or SIA_iter == c.SIA_maxit - 1:
                         SIA_filled = True
                     else:
-                        rhof_prev = st.rho_f.copy()
-                        st.P = Pn.copy()
+                        rhof_prev = pcopy(st.rho_f)
+                        st.P = self._copy(Pn)
                         if c.HEATTRANS:
-                            st.T = Tn.copy()
+                            st.T = self._copy(Tn)
                         if c.SPECTRANS:
-                            st.C = Cn.copy()
+                            st.C = self._copy(Cn)
                         SIA_iter += 1
-                        # new arrays: the backend's cache keys no longer match
-                        self._host_mutated([st.P, st.T, st.C])
+                        if not self._lazy():
+                            # new arrays: the backend's cache keys no longer
+                            # match (a lazy backend tracks the copies itself)
+                            self._host_mutated([st.P, st.T, st.C])
                 else:
                     SIA_filled = True
                     if n_tsteps % c.PRINTOUT == 0:
@@ -464,6 +527,8 @@ class Simulation:
                 numitT += num_it_T
 
             if c.DYNAMICBC:
+                self._sync()
+                self._wc_cache.clear()
                 bc_inc_ = bc_inc
                 bc_inc, st.P, st.T, st.C, st.BP, st.BT, st.BC, st.WP, st.WT, \
                     st.WC, st.Kx, st.Ky, st.Kz = c.dynamic_bc(
@@ -479,6 +544,9 @@ class Simulation:
                     self._host_mutated(self.update_fluid_properties())
 
             if c.MASSBALAN:
+                self._sync([st.C, st.qx, st.qy, st.qz, st.WP, st.WC, st.WT,
+                            st.D_fx, st.D_fy, st.D_fz])
+                self._wc_cache.clear()
                 st.mass = c.mass_balance(st.C, st.qx, st.qy, st.qz, st.WP,
                                          st.WC, st.WT, st.D_fx, st.D_fy,
                                          st.D_fz, self.dt, st.mass)
@@ -492,6 +560,7 @@ class Simulation:
                        dt_P=dt_P, dt_C=dt_C, dt_T=dt_T,
                        mass=numpy.array(st.mass, dtype=float).copy())
             if self.record_fields:
+                self._sync()
                 rec.update(P=st.P.copy(), C=st.C.copy(), T=st.T.copy(),
                            qx=st.qx.copy(), qy=st.qy.copy(), qz=st.qz.copy(),
                            e=st.e.copy(), rho_f=st.rho_f.copy())

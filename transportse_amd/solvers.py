@@ -28,6 +28,15 @@ residual given as P_conv / C_conv / T_conv (FP64 sweeps, or FP32 sweeps in
 FP64 defect correction), with the reference's accept / grow / halve / reset
 time-step control around each attempt and max_it bounding its sweeps.
 
+Deferred downloads (``lazy = True``, off by default; the SIA driver turns
+it on): the arrays a solver writes (P and the fresh Pn of compute_P, qx, qy,
+qz, the fresh e of flow_error, C, T, the Smolarkiewicz velocities) keep
+their values on the device, and the host arrays are filled only by
+``sync`` -- which the caller invokes before host code reads them -- or when
+the device buffer holding them is about to be reused.  ``copy`` copies such
+an array on the device, ``nonzero_sum`` answers ``numpy.sum(a) != 0`` for
+it.  Passing a pending array back to a solver uses the device copy.
+
 Device residency: inputs are cached on the device per argument role. A
 cached copy is reused while the same host array (same buffer, strides and
 shape) is passed and no host mutation was signalled; ``invalidate()``
@@ -38,6 +47,7 @@ invalidates after every host callback.
 from __future__ import annotations
 
 import ctypes
+import weakref
 
 import numpy as np
 
@@ -51,6 +61,51 @@ _FORMS = {"faithful64": _lib.FORM_FAITHFUL64, "plain64": _lib.FORM_PLAIN64,
 def _key(a: np.ndarray):
     ai = a.__array_interface__
     return (ai["data"][0], a.strides, a.shape, a.dtype.str)
+
+
+class _ViewRef:
+    """A weak reference to a strided view: the owning array weakly, plus the
+    view's offset, shape and strides (callers make a new view object at
+    every use, e.g. a species slice C[:, :, :, n], and holding the view would
+    keep its owner alive)."""
+    __slots__ = ("owner", "offset", "shape", "strides")
+
+    def __init__(self, owner, a):
+        self.owner = weakref.ref(owner)
+        self.offset = (a.__array_interface__["data"][0]
+                       - owner.__array_interface__["data"][0])
+        self.shape, self.strides = a.shape, a.strides
+
+    def __call__(self):
+        o = self.owner()
+        if o is None:
+            return None
+        flat = o.reshape(-1)
+        return np.lib.stride_tricks.as_strided(flat[self.offset // flat.itemsize:],
+                                               self.shape, self.strides)
+
+
+def _holder(a: np.ndarray):
+    """What the cache and the pending list keep of a host array: a weak
+    reference (they must not keep a caller's array alive); for a view of a
+    C-contiguous ndarray, a weak reference to that owner; any other view is
+    held itself."""
+    if a.base is None:
+        return weakref.ref(a)
+    o = a.base
+    while isinstance(o, np.ndarray) and isinstance(o.base, np.ndarray):
+        o = o.base
+    if isinstance(o, np.ndarray) and o.base is None and o.flags["C_CONTIGUOUS"] \
+            and o.dtype == a.dtype:
+        return _ViewRef(o, a)
+    return a
+
+
+def _held(h):
+    """The host array of a cache or pending entry (None if it is gone)."""
+    if h is None or isinstance(h, np.ndarray):
+        return h
+    return h()
 
 
 class HipSolvers:
@@ -87,31 +142,174 @@ class HipSolvers:
         # device copy stays the only one
         self.pe_writeback = True
         self.last = None
-        self.stats = {"uploads": 0, "downloads": 0}
+        self.stats = {"uploads": 0, "downloads": 0, "device_copies": 0}
+        self.lazy = False
+        # key -> [holder, tensor]: host arrays whose values live on the
+        # device only (weakly held, see _holder)
+        self._pending = {}
 
     # residency -------------------------------------------------------------------
     def invalidate(self, arrays=None):
         """Signal that host code may have changed input arrays: all of them
-        (arrays None), or those sharing memory with any of `arrays`."""
+        (arrays None), or those sharing memory with any of `arrays`.  Host
+        code that writes an array first syncs it; its pending device value,
+        if any, is dropped here (the host now holds the value)."""
         if arrays is None:
             self._gen += 1
+            self._pending.clear()
             return
         arrays = [np.asarray(a) for a in arrays]
         for ent in self._cache.values():
-            ref = ent[3]
+            ref = _held(ent[3])
             if ref is not None and any(np.may_share_memory(ref, a) for a in arrays):
                 ent[1] = -1
+        for pid, a, _, _ in self._pending_items():
+            if any(np.may_share_memory(a, b) for b in arrays):
+                del self._pending[pid]
+
+    # deferred downloads ------------------------------------------------------------
+    def _pending_items(self):
+        """Live pending entries as (key, host array, tensor, key)."""
+        out = []
+        for pk, (h, t) in list(self._pending.items()):
+            a = _held(h)
+            if a is None:
+                del self._pending[pk]
+                continue
+            out.append((pk, a, t, pk))
+        return out
+
+    def _pend(self, host, tensor):
+        """Record that `host`'s value is `tensor`'s (not yet downloaded)."""
+        a = np.asarray(host)
+        self._pending[_key(a)] = [_holder(a), tensor]
+
+    def _pending_exact(self, a):
+        k = _key(a)
+        ent = self._pending.get(k)
+        if ent is None:
+            return None
+        if _held(ent[0]) is None:
+            del self._pending[k]
+            return None
+        return k, ent[1]
+
+    def sync(self, arrays=None):
+        """Fill the host arrays whose values are still on the device: all of
+        them (arrays None), or those sharing memory with any of `arrays`."""
+        sel = None if arrays is None else [np.asarray(a) for a in arrays]
+        for pid, a, t, _ in self._pending_items():
+            if sel is not None and not any(np.may_share_memory(a, b) for b in sel):
+                continue
+            del self._pending[pid]
+            self.hip.download(t, out=a)
+            self.stats["downloads"] += 1
+
+    def _release(self, tensor, keep=None):
+        """Before `tensor` is overwritten: download the pending host arrays
+        it holds (except `keep`, whose value the write produces)."""
+        kk = None if keep is None else _key(np.asarray(keep))
+        for pid, a, t, k in self._pending_items():
+            if t is tensor and k != kk:
+                del self._pending[pid]
+                self.hip.download(t, out=a)
+                self.stats["downloads"] += 1
+
+    def _out(self, role, host, tensor):
+        """A solver wrote `tensor`, the value of host array `host` (None: a
+        fresh array): download it, or with lazy leave it pending."""
+        if host is None:
+            host = np.empty(self.shape, dtype=np.float64)
+        if self.lazy:
+            self._pend(host, tensor)
+        else:
+            self.hip.download(tensor, out=host)
+            self.stats["downloads"] += 1
+        self._mark(role, host, tensor)
+        return host
+
+    def copy(self, arr):
+        """``arr.copy()``; where arr's value (or, for a 4-D array, a species
+        slice's value) is on the device -- pending or a current input copy --
+        the copy is made there and the new array is pending."""
+        a = np.asarray(arr)
+        if not self.lazy:
+            return a.copy()
+        if a.ndim == 4:
+            new = np.empty_like(a)
+            for s_ in range(a.shape[-1]):
+                src = self._device_value(a[..., s_])
+                if src is None:
+                    new[..., s_] = a[..., s_]
+                else:
+                    self._pend(new[..., s_], self._clone(src))
+            return new
+        src = self._device_value(a)
+        if src is None:
+            return a.copy()
+        new = np.empty_like(a)
+        self._pend(new, self._clone(src))
+        return new
+
+    def _device_value(self, a):
+        """A device tensor holding a's current value, or None."""
+        ex = self._pending_exact(a)
+        if ex is not None:
+            return ex[1]
+        self.sync([a])     # a partly pending through another view
+        k = _key(a)
+        for ent in self._cache.values():
+            if ent[0] == k and ent[1] == self._gen and _held(ent[3]) is not None:
+                return ent[2]
+        return None
+
+    def _clone(self, t):
+        c = self.hip.empty()
+        c.copy_(t)
+        self.stats["device_copies"] += 1
+        return c
+
+    def nonzero_sum(self, arr) -> bool:
+        """``numpy.sum(arr) != 0.0`` (tse.py:1300): for an array whose value
+        is on the device (pending, or a current input copy) from the device
+        values when they decide it (all zero: False; no negative and some
+        positive value: True), else from the host sum."""
+        a = np.asarray(arr)
+        t = self._device_value(a) if self.lazy else None
+        if t is not None:
+            lo, hi = (float(v) for v in t.aminmax())
+            if lo == 0.0 and hi == 0.0:
+                return False
+            if lo >= 0.0 and hi > 0.0:
+                return True
+            self.sync([a])
+        return bool(np.sum(a) != 0.0)
 
     def _dev(self, role: str, arr):
         a = np.asarray(arr)
         k = _key(a)
         ent = self._cache.get(role)
-        if ent is not None and ent[0] == k and ent[1] == self._gen:
+        ex = self._pending_exact(a) if self._pending else None
+        # a cache hit needs the same array, unchanged; a pending array's
+        # value is its pending tensor (a new array may reuse a freed one's
+        # address, and with it the key of an old cache entry)
+        if ent is not None and ent[0] == k and ent[1] == self._gen and \
+                _held(ent[3]) is not None and (ex is None or ex[1] is ent[2]):
             return ent[2]
         out = ent[2] if ent is not None else None
-        t = self.hip.upload(a, out=out)
-        self.stats["uploads"] += 1
-        self._cache[role] = [k, self._gen, t, a]
+        if out is not None:
+            self._release(out, keep=a if ex is not None and ex[1] is out else None)
+        if ex is not None and ex[1] is out:
+            pass                 # the role's buffer holds the value already
+        elif ex is not None:     # the value is on the device already
+            t = out if out is not None else self.hip.empty()
+            t.copy_(ex[1])
+            self.stats["device_copies"] += 1
+        else:
+            self.sync([a])       # a view of it may be pending
+            t = self.hip.upload(a, out=out)
+            self.stats["uploads"] += 1
+        self._cache[role] = [k, self._gen, t, _holder(a)]
         return t
 
     def _raw(self, role: str, arr):
@@ -121,12 +319,14 @@ class HipSolvers:
         a = np.asarray(arr)
         k = _key(a)
         ent = self._cache.get(role)
-        if ent is not None and ent[0] == k and ent[1] == self._gen:
+        if ent is not None and ent[0] == k and ent[1] == self._gen and \
+                _held(ent[3]) is not None:
             return ent[2]
+        self.sync([a])
         t = torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).to(
             self.hip.device)
         self.stats["uploads"] += 1
-        self._cache[role] = [k, self._gen, t, a]
+        self._cache[role] = [k, self._gen, t, _holder(a)]
         return t
 
     def _pe_buffer(self, name: str, host):
@@ -145,7 +345,7 @@ class HipSolvers:
     def _mark(self, role: str, arr, tensor):
         """Record that `tensor` now equals host `arr` (after a download)."""
         a = np.asarray(arr)
-        self._cache[role] = [_key(a), self._gen, tensor, a]
+        self._cache[role] = [_key(a), self._gen, tensor, _holder(a)]
 
     def _scratch(self, role: str):
         ent = self._cache.get(role)
@@ -167,6 +367,7 @@ class HipSolvers:
 
     def _begin(self):
         if self.auto_invalidate:
+            self.sync()
             self.invalidate()
         self.hip.sync_stream()
 
@@ -183,6 +384,8 @@ class HipSolvers:
             ("WP", WP))}
         d["Pn"] = self._scratch("Pn")
         d["c_r"] = self._frozen_field("c_r")
+        self._release(d["P"], keep=P)
+        self._release(d["Pn"])
         f = _lib.PFields(**{n: d[n].data_ptr() for n in _lib.P_FIELDS})
         prm = _lib.SorParams(dt=float(dt_P), sor=float(SOR), omega=1.0,
                              inc_it=float(inc_it), max_dt=float(max_dt),
@@ -193,11 +396,8 @@ class HipSolvers:
                                           ctypes.byref(prm), ctypes.byref(res)),
                    "tse_compute_p")
         self._report_halvings("P", res, dt_P)
-        self.hip.download(d["P"], out=P)
-        Pn = self.hip.download(d["Pn"])
-        self.stats["downloads"] += 2
-        self._mark("P", P, d["P"])
-        self._mark("Pn", Pn, d["Pn"])
+        self._out("P", P, d["P"])
+        Pn = self._out("Pn", None, d["Pn"])
         self.last = res
         return res.t, res.dt, P, Pn, bool(res.converged), int(res.num_it)
 
@@ -207,14 +407,14 @@ class HipSolvers:
             ("P", P), ("BP", BP), ("Kx", Kx), ("Ky", Ky), ("Kz", Kz),
             ("qx", qx), ("qy", qy), ("qz", qz), ("nue", nue),
             ("rho_f", rho_f))}
+        for n, a in (("qx", qx), ("qy", qy), ("qz", qz)):
+            self._release(d[n], keep=a)
         _lib.check(self.lib.tse_compute_q(
             self.hip.ctx, *[d[n].data_ptr() for n in
                             ("P", "BP", "Kx", "Ky", "Kz", "qx", "qy", "qz",
                              "nue", "rho_f")]), "tse_compute_q")
         for n, a in (("qx", qx), ("qy", qy), ("qz", qz)):
-            self.hip.download(d[n], out=a)
-            self._mark(n, a, d[n])
-        self.stats["downloads"] += 3
+            self._out(n, a, d[n])
         return qx, qy, qz
 
     def flow_error(self, P, Pn, qx, qy, qz, WP, phi, c_f, c_r, rho_f, dt_P):
@@ -224,15 +424,13 @@ class HipSolvers:
             ("WP", WP), ("phi", phi), ("c_f", c_f), ("c_r_live", c_r),
             ("rho_f", rho_f))}
         e_dev = self._scratch("e")
+        self._release(e_dev)
         _lib.check(self.lib.tse_flow_error(
             self.hip.ctx, *[d[n].data_ptr() for n in
                             ("P", "Pn", "qx", "qy", "qz", "WP", "phi", "c_f",
                              "c_r_live", "rho_f")], float(dt_P),
             e_dev.data_ptr()), "tse_flow_error")
-        e = self.hip.download(e_dev)
-        self.stats["downloads"] += 1
-        self._mark("e", e, e_dev)
-        return e
+        return self._out("e", None, e_dev)
 
     def compute_C(self, C, D_fx, D_fy, D_fz, phi, rho_f, qx, qy, qz, dt_C, BC,
                   WC, RC, e, omega=1, SOR=1.0, max_it=None, inc_it=1.05,
@@ -245,6 +443,7 @@ class HipSolvers:
             ("phi", phi), ("rho_f", rho_f), ("qx", qx), ("qy", qy),
             ("qz", qz), ("BC", BC), ("WC", WC), ("RC", RC), ("e", e))}
         d["Cn"] = self._scratch("Cn")
+        self._release(d["C"], keep=C)
         d["lf_l"] = self._frozen_field("lf_l")
         d["lf_t"] = self._frozen_field("lf_t")
         f = _lib.CFields(**{n: d[n].data_ptr() for n in _lib.C_FIELDS},
@@ -258,9 +457,7 @@ class HipSolvers:
                                           ctypes.byref(prm), ctypes.byref(res)),
                    "tse_compute_c")
         self._report_halvings("C", res, dt_C)
-        self.hip.download(d["C"], out=C)
-        self.stats["downloads"] += 1
-        self._mark("C", C, d["C"])
+        self._out("C", C, d["C"])
         self.last = res
         return res.t, res.dt, C, bool(res.converged), int(res.num_it)
 
@@ -275,6 +472,7 @@ class HipSolvers:
             ("phi", phi), ("qx", qx), ("qy", qy), ("qz", qz), ("BT", BT),
             ("WT", WT), ("e", e))}
         d["Tn"] = self._scratch("Tn")
+        self._release(d["T"], keep=T)
         for n in ("l_r", "cp_r", "rho_r", "lt_l", "lt_t"):
             d[n] = self._frozen_field(n)
         f = _lib.TFields(**{n: d[n].data_ptr() for n in _lib.T_FIELDS},
@@ -288,9 +486,7 @@ class HipSolvers:
                                           ctypes.byref(prm), ctypes.byref(res)),
                    "tse_compute_t")
         self._report_halvings("T", res, dt_T)
-        self.hip.download(d["T"], out=T)
-        self.stats["downloads"] += 1
-        self._mark("T", T, d["T"])
+        self._out("T", T, d["T"])
         self.last = res
         return res.t, res.dt, T, bool(res.converged), int(res.num_it)
 
@@ -301,14 +497,14 @@ class HipSolvers:
         d = {n: self._dev(n, a) for n, a in (
             ("smol_C", C), ("qx", qx), ("qy", qy), ("qz", qz),
             ("vx", vx), ("vy", vy), ("vz", vz))}
+        for n, a in (("vx", vx), ("vy", vy), ("vz", vz)):
+            self._release(d[n], keep=a)
         _lib.check(self.lib.tse_smolarkiewicz(
             self.hip.ctx, *[d[n].data_ptr() for n in
                             ("smol_C", "qx", "qy", "qz", "vx", "vy", "vz")],
             float(dt), float(eps), float(Sc)), "tse_smolarkiewicz")
         for n, a in (("vx", vx), ("vy", vy), ("vz", vz)):
-            self.hip.download(d[n], out=a)
-            self._mark(n, a, d[n])
-        self.stats["downloads"] += 3
+            self._out(n, a, d[n])
         return vx, vy, vz
 
     # diagnostics of the SIA driver --------------------------------------------------
