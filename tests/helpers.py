@@ -89,13 +89,15 @@ class Fields:
                 self.beta)
 
 
-def oracle_sia_steps(spec, nsteps, dts=None):
+def oracle_sia_steps(spec, nsteps, dts=None, eos=None):
     """The SIA steps of a ScenarioSpec (transportse_amd.scenario) run with the
     oracle's reference functions in tse.py order (compute_P, compute_q,
     flow_error, compute_C per species, compute_T, linear EOS, density norm,
     reset on a rejected Picard iteration), each solve converged to an update
     norm of 1e-15 at the step's dt (dts[s], default the spec's dt), i.e. the
-    FP64 fixed point of every solve."""
+    FP64 fixed point of every solve.  eos: a fluid_properties callable
+    (tse.py:178-229 contract, whole fields, species axis first) instead of
+    the linear EOS; it may change rho_f, l_f, nue, c_f and cp_f."""
     from oracle import Oracle
     from transportse_amd.scenario import EOS, UNIFORM, host_fields
     hf = host_fields(spec)
@@ -139,8 +141,17 @@ def oracle_sia_steps(spec, nsteps, dts=None):
                             max_it=10 ** 5, max_dt=dt, T_conv=1e-15)
             assert r[3]
             T = r[2]
-            rho = EOS["rho0"] + EOS["drho_dc"] * C[0]
-            l_f = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
+            if eos is None:
+                rho = EOS["rho0"] + EOS["drho_dc"] * C[0]
+                l_f = EOS["lf0"] * (1.0 + EOS["dlf_dt"] * (T - EOS["t_ref"]))
+            else:
+                D = np.broadcast_to(U["D_f"], (spec.species,) + shp)
+                out = eos(np.stack(C), T, P, U["nue"], rho, U["c_f"], U["cp_f"],
+                          l_f, U["cp_r"], U["l_r"], U["phi"], K, K, K, D, D, D,
+                          U["rho_r"])
+                U["nue"], rho, U["c_f"], U["cp_f"], l_f = (
+                    np.broadcast_to(np.asarray(out[n], np.float64), shp).copy()
+                    for n in (3, 4, 5, 6, 7))
             diff = o.l2_norm(rho, rho_prev)
             if diff < spec.sia_conv or it == spec.sia_maxit - 1:
                 break

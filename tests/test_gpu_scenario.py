@@ -179,3 +179,60 @@ def test_scenario_halves_dt_on_a_failed_solve():
     for k, v in errs.items():
         assert v <= 1e-7, k
     scn.close()
+
+
+def _linear_fp(C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+               D_fx, D_fy, D_fz, rho_r):
+    """The scenario's linear EOS written as a config's fluid_properties."""
+    rho_f = 1000.0 + 700.0 * C[0]
+    l_f = 2.5 * (1.0 + 0.002 * (T - 10.0))
+    return (C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+            D_fx, D_fy, D_fz, rho_r)
+
+
+def _nonlinear_fp(C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky,
+                  Kz, D_fx, D_fy, D_fz, rho_r):
+    """Density with thermal expansion and salt, temperature-dependent
+    conductivity and viscosity."""
+    rho_f = 1000.0 + 700.0 * C[0] - 0.2 * (T - 10.0) - 0.004 * (T - 10.0) ** 2
+    l_f = 0.56 + 0.0018 * T
+    nue = 1e-3 * 2.0 ** (-(T - 10.0) / 30.0)
+    return (C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+            D_fx, D_fy, D_fz, rho_r)
+
+
+def test_scenario_host_fluid_properties():
+    """A config-style fluid_properties drives the Picard update of the
+    device-resident SIA step on the host.  Written as the linear EOS it
+    gives the device EOS's fields bit for bit; a nonlinear one (density of
+    salt and temperature, conductivity and viscosity of temperature) matches
+    the oracle's SIA steps with the same callback within the mixed form's
+    tolerances (P 1e-9, C and T 1e-7 relative L2)."""
+    from transportse_amd.scenario import DeviceScenario
+    spec = _spec()
+    kw = dict(form="mixed", tol=1e-10, plan_p=(2, 16, 1, 0), plan_tr=(2, 16, 1, 0),
+              sor_p=1.9, sor_tr=1.2, inner_p=20, inner_tr=4)
+    a = DeviceScenario(spec)
+    a.run(2, **kw)
+    b = DeviceScenario(spec, fluid_properties=_linear_fp)
+    b.run(2, **kw)
+    ga, gb = a.download(), b.download()
+    for k in ("P", "T", "rho_f"):
+        assert np.array_equal(ga[k], gb[k]), k
+    for x, y in zip(ga["C"], gb["C"]):
+        assert np.array_equal(x, y)
+    a.close()
+    b.close()
+    c = DeviceScenario(spec, fluid_properties=_nonlinear_fp)
+    c.run(2, **kw)
+    got = c.download()
+    ref = oracle_sia_steps(spec, 2, eos=_nonlinear_fp)
+    errs = {"P": _rel(got["P"], ref["P"]), "T": _rel(got["T"], ref["T"]),
+            "rho_f": _rel(got["rho_f"], ref["rho_f"])}
+    for s in range(spec.species):
+        errs[f"C{s}"] = _rel(got["C"][s], ref["C"][s])
+    print("host EOS: relative L2 error vs oracle", errs, flush=True)
+    assert errs["P"] <= 1e-9
+    for k, v in errs.items():
+        assert v <= 1e-7, k
+    c.close()

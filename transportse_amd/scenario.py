@@ -201,9 +201,17 @@ class DeviceScenario:
     """One scenario's fields in HBM and its SIA step as a generator."""
 
     def __init__(self, spec: ScenarioSpec, device: int = 0, fields=None,
-                 use_graphs: bool = False):
+                 use_graphs: bool = False, fluid_properties=None):
+        """fluid_properties: None for the closed-form linear EOS on the
+        device, or a callable with the contract of a config's
+        fluid_properties (tse.py:178-229), called on whole fields as the SIA
+        driver calls it (species axis first for C and D_f*): it may change
+        rho_f, l_f, nue, c_f and cp_f; every other argument must come back
+        unchanged.  It runs on the host: the Picard update then downloads
+        C, T and P and uploads what changed."""
         import torch
         self.spec = spec
+        self.fluid_properties = fluid_properties
         # device work between solves as hipGraphs: each segment runs eagerly
         # once (allocations happen there), is captured on its second use and
         # replayed from then on (the fields never move)
@@ -219,6 +227,11 @@ class DeviceScenario:
         hf = fields if fields is not None else host_fields(spec)
         self._ukeys = {}
         u = {k: self._uniform(k, v) for k, v in UNIFORM.items()}
+        if fluid_properties is not None:
+            # fields a host EOS may change: this scenario's own copies, made
+            # now so that their addresses never change (captured graphs)
+            for k in ("nue", "c_f", "cp_f"):
+                u[k] = hip.upload(np.full((spec.nx, spec.ny, spec.nz), UNIFORM[k]))
         self.u = u
         self.K = hip.upload(hf["K"])
         self.P = hip.upload(hf["P"])
@@ -332,10 +345,60 @@ class DeviceScenario:
                    "tse_fast_tr_result")
 
     def eos_update(self):
+        if self.fluid_properties is not None:
+            return self._host_eos()
         _lib.check(self.lib.tse_linear_eos(self.hip.ctx, self._p(self.C[0]),
                                            self._p(self.T), self._p(self.rho),
                                            self._p(self.l_f), ctypes.byref(self.eos)),
                    "tse_linear_eos")
+
+    # tse.py:178-229 argument order; the fluid properties a host EOS may set
+    _FP_ARGS = ("C", "T", "P", "nue", "rho_f", "c_f", "cp_f", "l_f", "cp_r",
+                "l_r", "phi", "Kx", "Ky", "Kz", "D_fx", "D_fy", "D_fz", "rho_r")
+    _FP_SET = ("nue", "rho_f", "c_f", "cp_f", "l_f")
+
+    def _host_eos(self):
+        """The Picard density update through a host fluid_properties: the
+        current C, T, P and fluid properties go to the callback as whole
+        fields, what it changes comes back to the device."""
+        sp, h = self.spec, self.hip
+        shp = (sp.nx, sp.ny, sp.nz)
+        if not hasattr(self, "_fp_host"):
+            const = lambda v: np.broadcast_to(np.float64(v), shp)
+            self._fp_host = {k: h.download(t) for k, t in (
+                ("nue", self.u["nue"]), ("c_f", self.u["c_f"]),
+                ("cp_f", self.u["cp_f"]))}
+            self._fp_host.update(cp_r=const(UNIFORM["cp_r"]), l_r=const(UNIFORM["l_r"]),
+                                 phi=const(UNIFORM["phi"]),
+                                 rho_r=const(UNIFORM["rho_r"]))
+            K = h.download(self.K)
+            K.flags.writeable = False
+            D = np.broadcast_to(np.float64(UNIFORM["D_f"]), (sp.species,) + shp)
+            self._fp_host.update(Kx=K, Ky=K, Kz=K, D_fx=D, D_fy=D, D_fz=D)
+        a = dict(self._fp_host)
+        a["C"] = np.stack([h.download(c) for c in self.C])
+        a["T"] = h.download(self.T)
+        a["P"] = h.download(self.P)
+        a["rho_f"] = h.download(self.rho)
+        a["l_f"] = h.download(self.l_f)
+        args = [a[k] for k in self._FP_ARGS]
+        for v in args:
+            if v.flags.writeable:
+                v.flags.writeable = False   # changes come back as new arrays
+        out = self.fluid_properties(*args)
+        dst = {"rho_f": self.rho, "l_f": self.l_f, "nue": self.u["nue"],
+               "c_f": self.u["c_f"], "cp_f": self.u["cp_f"]}
+        for k, old, new in zip(self._FP_ARGS, args, out):
+            if new is old:
+                continue
+            if k not in self._FP_SET:
+                raise NotImplementedError(
+                    f"fluid_properties changed {k}; the device-resident SIA step "
+                    f"takes changes of {', '.join(self._FP_SET)} only")
+            v = np.ascontiguousarray(np.broadcast_to(np.asarray(new, np.float64), shp))
+            h.upload(v, out=dst[k])
+            if k in self._fp_host:
+                self._fp_host[k] = v
 
     def density_norm(self) -> float:
         out = ctypes.c_double()
@@ -486,7 +549,10 @@ class DeviceScenario:
                                            (1,), 1.05)
                     self._segment(f"finish_t{self._current(1)}",
                                   lambda: self.finish_tr(self.T))
-                self._segment("eos", self.eos_update)
+                if self.fluid_properties is None:
+                    self._segment("eos", self.eos_update)
+                else:            # host work: never captured
+                    self.eos_update()
                 self._density_norm_start()
                 yield KIND_WAIT, None
                 diff = self._reduction(1)[0]
