@@ -276,6 +276,8 @@ def main(argv=None):
 
     del fp
     torch.cuda.empty_cache()
+    ceiling = stream_ceiling()
+    log(f"streaming ceiling (c = a + b, FP32, 3 x 1 GiB): {ceiling:.0f} GB/s")
     traffic = None if args.no_traffic else measure_traffic(args, plan32)
     cfgs = None if args.no_configs else configs_section(args)
     sia = None if args.no_sia else sia_section(args, nx, ny, nz)
@@ -339,7 +341,11 @@ def main(argv=None):
                      "hbm_measured_gbs": tbytes / launch_s / 1e9 if tbytes else None,
                      "hbm_measured_frac": (tbytes / launch_s / 1e9 / HBM_PEAK_GBS
                                            if tbytes else None),
-                     "traffic_measurement": traffic},
+                     "traffic_measurement": traffic,
+                     # what a plain streaming kernel reaches on this device:
+                     # the practical ceiling beside the nominal peak
+                     "stream_ceiling_gbs": ceiling,
+                     "frac_of_stream_ceiling": achieved / ceiling},
         "cpu_baseline": cpu,
         "configs_faithful64": cfgs,
         "sia": sia,
@@ -348,6 +354,30 @@ def main(argv=None):
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def stream_ceiling(reps=10):
+    """HBM rate of a plain streaming kernel on this GPU: c = a + b over
+    three 1 GiB FP32 arrays (two reads, one write per element), timed with
+    HIP events: the practical bandwidth ceiling beside the nominal 8 TB/s."""
+    import torch
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.ones_like(a)
+    c = torch.empty_like(a)
+    torch.add(a, b, out=c)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        torch.add(a, b, out=c)
+    e1.record()
+    torch.cuda.synchronize()
+    sec = e0.elapsed_time(e1) * 1e-3 / reps
+    del a, b, c
+    torch.cuda.empty_cache()
+    return 3 * 4 * n / sec / 1e9
 
 
 def _pmc_per_dispatch(outdir, counter):
