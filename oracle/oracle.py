@@ -100,6 +100,8 @@ def _bind(L):
         L.tse_or_compute_t.restype = I
         L.tse_or_compute_t.argtypes = [G] + [_dp] * 17 + [I, D, D, D, I, D, D, D, D, R]
         L.tse_or_set_threads.argtypes = [I]
+        L.tse_or_set_progress.argtypes = [_PROGRESS, I]
+        L.tse_or_set_progress.restype = None
         L.tse_or_get_threads.restype = I
     return L
 
@@ -116,6 +118,19 @@ def _c(a) -> np.ndarray:
 
 def set_threads(n: int) -> None:
     lib().tse_or_set_threads(int(n))
+
+
+_PROGRESS = ctypes.CFUNCTYPE(None, ctypes.c_int)
+_progress_ref = None
+
+
+def set_progress(fn=None, every: int = 500) -> None:
+    """Call fn(num_it) every `every` iterations of a compute_P call (for
+    long runs whose callers must show that they are progressing); None
+    turns it off."""
+    global _progress_ref
+    _progress_ref = _PROGRESS(fn) if fn is not None else _PROGRESS()
+    lib().tse_or_set_progress(_progress_ref, int(every) if fn is not None else 0)
 
 
 def get_threads() -> int:

@@ -294,6 +294,17 @@ void tse_or_p_half_sweep(const tse_or_grid *g, double *P, const double *Pn,
             }
 }
 
+/* Optional progress report of long compute_P calls: called with the
+ * iteration count every `g_progress_every` iterations (no effect on any
+ * value). */
+static void (*g_progress)(int) = 0;
+static int g_progress_every = 0;
+
+void tse_or_set_progress(void (*cb)(int), int every) {
+    g_progress = cb;
+    g_progress_every = every;
+}
+
 /* compute_P (tse.py:647-742). */
 int tse_or_compute_p(const tse_or_grid *g, double *P, double *Pn,
                      const double *c_f, const double *nue,
@@ -313,6 +324,8 @@ int tse_or_compute_p(const tse_or_grid *g, double *P, double *Pn,
     while (!converged) {
         num_it += 1;
         num_sub_it += 1;
+        if (g_progress && g_progress_every > 0 && num_it % g_progress_every == 0)
+            g_progress(num_it);
         memcpy(Pold, P, N * sizeof(double));
         tse_or_p_half_sweep(g, P, Pn, c_f, nue, rho_f, phi, Kx, Ky, Kz, BP, WP,
                             c_r, dt_P, SOR, 0);

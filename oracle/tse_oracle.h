@@ -62,6 +62,10 @@ void tse_or_dispersivity(const tse_or_grid *g, const double *qx,
                          const double *B, int i, int j, int k, double out[6]);
 
 /* ---- solvers ----------------------------------------------------------- */
+/* Progress report of long compute_P calls: cb(num_it) every `every`
+ * iterations (NULL or every <= 0: off).  Changes no value. */
+void tse_or_set_progress(void (*cb)(int), int every);
+
 /* compute_P (tse.py:647-742). P is updated in place (it ends holding the
  * returned P), Pn receives the copy of P at entry. c_r is the frozen
  * module global. */
