@@ -146,7 +146,9 @@ def main(argv=None):
 
     if args.plan:
         fp.plans["mixed"] = tuple(args.plan)
+    t_tune = time.perf_counter()
     plan32 = fp.plan("mixed")
+    tune_s = time.perf_counter() - t_tune
     log(f"FP32 plan (ns, by, ry, zchunk) = {plan32}"
         + ("" if args.plan else " (autotuned)"))
 
@@ -172,9 +174,14 @@ def main(argv=None):
         fp.result_device()  # solution back into a split-layout device field
         return st
 
-    for _ in range(args.warmup):
+    cold = None
+    for w in range(args.warmup):
         om = args.omega if args.omega else tuner.omega
         st = step("mixed", om)
+        if w == 0:   # the first solve: the tuner's starting factor, untuned
+            cold = {"time_to_solution_s": st.seconds, "sweeps": st.sweeps,
+                    "omega": om, "converged": st.converged,
+                    "plan_autotune_s": tune_s if not args.plan else 0.0}
         if not args.omega:
             tuner.report(st.sweeps)
         log(f"warmup solve (omega {om:.5f}): {st.sweeps} sweeps, rel "
@@ -317,6 +324,9 @@ def main(argv=None):
                                    "tuned online over the warm-up solves",
                    "plan_ns_by_ry_zchunk_variant": list(plan32)},
         "time_to_solution_s": tts,
+        # the same solve before any tuning on this problem: the first warm-up
+        # solve at the tuner's starting SOR factor (2 - 2/max(nx, ny, nz))
+        "cold_start": cold,
         "solves_per_s": 1.0 / tts,
         "sweeps_per_solve": solve_sweeps // args.steps,
         "rel_residual": stats[-1].rel_residual,

@@ -97,3 +97,29 @@ def test_weak_view_references_follow_their_owner():
     del C, v, w, P
     gc.collect()
     assert _held(h) is None and _held(hp) is None
+
+
+def test_vectorised_eos_falls_back_when_cells_disagree(tmp_path, oracle_mod):
+    """A fluid_properties whose whole-field call differs from its per-cell
+    call (here: a density that uses the field mean, which per cell is the
+    cell itself) is caught by the sample check, and the driver runs the
+    per-cell loop, as the reference does (tse.py:181-196)."""
+    import numpy as np
+
+    from helpers import GOLDEN, oracle_backend
+    from transportse_amd.driver import Simulation
+    sim = Simulation(f"{GOLDEN}/configs/g1_salt.py", oracle_backend,
+                     verbose=False, out_dir=str(tmp_path), max_steps=1)
+    c = sim.config
+
+    def fp(C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+           D_fx, D_fy, D_fz, rho_r):
+        rho_f = 1000.0 + 700.0 * np.mean(C[0])
+        return (C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+                D_fx, D_fy, D_fz, rho_r)
+    c.fluid_properties = fp
+    sim._eos_vectorized = None
+    sim.update_fluid_properties()
+    assert sim._eos_vectorized is False
+    expect = 1000.0 + 700.0 * sim.st.C[..., 0]
+    assert np.array_equal(sim.st.rho_f, expect)

@@ -161,7 +161,9 @@ class Simulation:
             return self._fp_arrays()
         try:
             res = self._eos_vector()
-            ok = self._eos_vectorized or self._validate_eos(res)
+            # every call is checked against the per-cell contract: the first
+            # on a broad sample, later ones on fresh random cells
+            ok = self._validate_eos(res, first=not self._eos_vectorized)
         except Exception:
             ok = False
         if not ok:
@@ -176,13 +178,30 @@ class Simulation:
                 changed.append(a)
         return changed
 
-    def _validate_eos(self, res) -> bool:
-        """Check the whole-field call against the per-cell contract on a sample
-        of cells (bit-for-bit); any difference selects the per-cell loop."""
+    def _validate_eos(self, res, first=True) -> bool:
+        """Check the whole-field call against the per-cell contract
+        (tse.py:181-196: fluid_properties called cell by cell) bit for bit on
+        a sample of cells: the grid's corners and random cells at every call
+        (64 new ones), and at the first call also the cells holding the
+        minimum and maximum of every input (where value-dependent branches
+        of a callback act first) and 1024 random cells.  Any difference
+        selects the per-cell loop for this call and the rest of the run."""
         st = self.st
-        rng = numpy.random.default_rng(0)
+        shp = (st.nx, st.ny, st.nz)
+        if not hasattr(self, "_eos_rng"):
+            self._eos_rng = numpy.random.default_rng(0)
+        rng = self._eos_rng
         cells = {(0, 0, 0), (st.nx - 1, st.ny - 1, st.nz - 1)}
-        for _ in range(min(64, st.nx * st.ny * st.nz)):
+        for a in (self._fp_arrays() if first else ()):
+            v = numpy.asarray(a)
+            if v.ndim == 4:
+                v = v.reshape(shp + (-1,)).max(axis=-1)
+            if v.shape != shp or v.size == 0:
+                continue
+            for f in (numpy.argmin, numpy.argmax):
+                cells.add(tuple(int(x) for x in numpy.unravel_index(f(v), shp)))
+        n = min(1024 if first else 64, st.nx * st.ny * st.nz)
+        for _ in range(n):
             cells.add((int(rng.integers(st.nx)), int(rng.integers(st.ny)),
                        int(rng.integers(st.nz))))
         for (i, j, k) in cells:
