@@ -119,10 +119,6 @@ class Simulation:
         return [st.T[:, :, :, 0] if n == "T" and st.T.ndim == 4 else getattr(st, n)
                 for n in _FP_NAMES]
 
-    def _eos_cell(self, i, j, k):
-        args = [a[i, j, k] for a in self._fp_arrays()]
-        return self.config.fluid_properties(*args)
-
     def _eos_loop(self):
         st = self.st
         arrs = self._fp_arrays()
@@ -182,7 +178,7 @@ class Simulation:
         """Check the whole-field call against the per-cell contract
         (tse.py:181-196: fluid_properties called cell by cell) bit for bit on
         a sample of cells: the grid's corners and random cells at every call
-        (64 new ones), and at the first call also the cells holding the
+        (16 new ones), and at the first call also the cells holding the
         minimum and maximum of every input (where value-dependent branches
         of a callback act first) and 1024 random cells.  Any difference
         selects the per-cell loop for this call and the rest of the run."""
@@ -190,27 +186,31 @@ class Simulation:
         shp = (st.nx, st.ny, st.nz)
         if not hasattr(self, "_eos_rng"):
             self._eos_rng = numpy.random.default_rng(0)
-        rng = self._eos_rng
-        cells = {(0, 0, 0), (st.nx - 1, st.ny - 1, st.nz - 1)}
-        for a in (self._fp_arrays() if first else ()):
+        arrs = self._fp_arrays()
+        cells = [(0, 0, 0), (st.nx - 1, st.ny - 1, st.nz - 1)]
+        for a in (arrs if first else ()):
             v = numpy.asarray(a)
             if v.ndim == 4:
                 v = v.reshape(shp + (-1,)).max(axis=-1)
             if v.shape != shp or v.size == 0:
                 continue
             for f in (numpy.argmin, numpy.argmax):
-                cells.add(tuple(int(x) for x in numpy.unravel_index(f(v), shp)))
-        n = min(1024 if first else 64, st.nx * st.ny * st.nz)
-        for _ in range(n):
-            cells.add((int(rng.integers(st.nx)), int(rng.integers(st.ny)),
-                       int(rng.integers(st.nz))))
-        for (i, j, k) in cells:
-            ref = self._eos_cell(i, j, k)
-            for n, v, r in zip(_FP_NAMES, res, ref):
-                got = v[i, j, k]
-                if not numpy.array_equal(numpy.asarray(got, dtype=numpy.float64),
-                                         numpy.asarray(r, dtype=numpy.float64)):
-                    return False
+                cells.append(numpy.unravel_index(f(v), shp))
+        n = min(1024 if first else 16, st.nx * st.ny * st.nz)
+        cells = numpy.concatenate(
+            [numpy.array(cells, dtype=numpy.int64).reshape(-1, 3),
+             self._eos_rng.integers(0, shp, size=(n, 3))])
+        fp = self.config.fluid_properties
+        refs = [fp(*[a[i, j, k] for a in arrs]) for i, j, k in cells.tolist()]
+        I, J, K = cells.T
+        for n, v in enumerate(res):
+            got = numpy.ascontiguousarray(numpy.asarray(v)[I, J, K],
+                                          dtype=numpy.float64)
+            want = numpy.array([numpy.asarray(r[n], dtype=numpy.float64)
+                                for r in refs], dtype=numpy.float64)
+            if got.shape != want.shape or not numpy.array_equal(
+                    got.view(numpy.int64), want.view(numpy.int64)):
+                return False
         return True
 
     # ---- diagnostics (tse.py:331-396) ---------------------------------------------
