@@ -67,11 +67,15 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
+# variant 5 stores the three faces and the storage term as bfloat16 (the
+# diagonal is summed in the kernel): 8 + 4 + 4 + 4 bytes
+BYTES_FP32_SWEEP_BY_VARIANT = {5: 20}
 BYTES_FP64_SWEEP = 56
 # sweep kernel (name as rocprofv3 reports it, without template arguments)
 # per plan variant
 VARIANT_KERNEL = {0: "rb_zmarch_kernel", 1: "rb_zmarch_xl_kernel",
-                  2: "rb_zmarch_xr_kernel", 3: "rb_zmarch_x2_kernel"}
+                  2: "rb_zmarch_xr_kernel", 3: "rb_zmarch_x2_kernel",
+                  5: "rb_zmarch_x5_kernel"}
 
 
 def log(*a):
@@ -222,12 +226,14 @@ def main(argv=None):
     launch_s = ev0.elapsed_time(ev1) * 1e-3 / launches
     # algorithmic bytes of one launch: every operand once (28 B per cell); the
     # ns sweeps a launch fuses share that one pass
-    bytes_launch = BYTES_FP32_SWEEP * N
+    variant = plan32[4] if len(plan32) > 4 else 0
+    cell_bytes = BYTES_FP32_SWEEP_BY_VARIANT.get(variant, BYTES_FP32_SWEEP)
+    bytes_launch = cell_bytes * N
     achieved = bytes_launch / launch_s / 1e9
-    per_sweep = BYTES_FP32_SWEEP * N * ns / launch_s / 1e9
+    per_sweep = cell_bytes * N * ns / launch_s / 1e9
     log(f"sweep kernel: {launch_s * 1e6:.1f} us/launch ({ns} sweeps), "
         f"{achieved:.0f} GB/s one-pass ({achieved / HBM_PEAK_GBS:.3f} of peak), "
-        f"{per_sweep:.0f} GB/s counting every sweep at 28 B/cell")
+        f"{per_sweep:.0f} GB/s counting every sweep at {cell_bytes} B/cell")
 
     plain = None
     if not args.no_compare:
@@ -300,7 +306,7 @@ def main(argv=None):
         "value_is": "SOR cell-updates/s of the 512x512x256 mixed solve",
         "hbm_roofline_pct": 100.0 * achieved / HBM_PEAK_GBS,
         "hbm_roofline_pct_definition": "one pass of the sweep operands "
-                                       "(28 B x cells) per launch / launch time "
+                                       f"({cell_bytes} B x cells) per launch / launch time "
                                        "/ 8 TB/s",
         "serving_solves_per_s": srv0["solves_per_s"] if srv0 else None,
         "serving_p99_s": srv0["p99_s"] if srv0 else None,
@@ -314,7 +320,9 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 sweeps in f64 defect correction",
+        "dtype": ("f32 sweeps (inner operator stored as bf16) in f64 defect "
+                  "correction" if cell_bytes == 20 else
+                  "f32 sweeps in f64 defect correction"),
         "data": "synthetic",
         "config": {"workload": f"pressure solve {nx}x{ny}x{nz}, 5 m cells, "
                                "sigma_lnK 2.5, corr 16 cells, mixed precision "
@@ -337,11 +345,15 @@ def main(argv=None):
                      "traffic": tbytes,
                      "kernel": VARIANT_KERNEL[variant],
                      "algorithmic_bytes_per_launch": bytes_launch,
-                     "algorithmic_bytes_definition": "28 B per cell (3 face "
-                         "transmissibilities, 1/diag, rhs, unknown in and out, "
-                         "FP32), once per launch",
+                     "algorithmic_bytes_definition": (
+                         "20 B per cell (16-byte pair record of the 3 face "
+                         "transmissibilities and the storage term as bfloat16, "
+                         "rhs, unknown in and out FP32), once per launch"
+                         if cell_bytes == 20 else
+                         "28 B per cell (3 face transmissibilities, 1/diag, "
+                         "rhs, unknown in and out, FP32), once per launch"),
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6,
-                     # every fused sweep counted at 28 B per cell: the rate a
+                     # every fused sweep counted at cell_bytes per cell: the rate a
                      # one-sweep-per-launch kernel would need for the same
                      # sweep throughput (temporal blocking puts it above peak)
                      "effective_per_sweep_gbs": per_sweep,

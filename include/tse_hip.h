@@ -234,6 +234,14 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *              (2,32,2) (3,16,1) (3,24,2)
  *   variant 2: as variant 1, FP32 sweeps also (3,32,2)
  *   variant 3: (2,32,2) (3,32,2)
+ *   variant 5: (2,32,2) (3,32,2), FP32 pressure sweeps only, single systems
+ *              (not batches): variant 3's cell pairs on an inner operator
+ *              stored as bfloat16 (the three faces and the storage term of
+ *              each cell; the diagonal is summed in the kernel from the
+ *              rounded values), a graph Laplacian within a factor 1 +- 2^-9
+ *              of the FP64 one, so the defect correction keeps its stop
+ *              test and sweep count; 20 instead of 28 bytes per cell per
+ *              launch; even nx, at most 2^27 cells
  * Batches (tse_batch_begin) also take variant 4 with (2,32,2) (3,32,2):
  * variant 3 for members with an even nx, variant 2 for the others.
  * Transport systems: variant 0 or 2, (1,8,1) (1,16,1) (2,16,1), FP32
@@ -281,6 +289,12 @@ int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
                      tse_fast_result *out);
 /* Relative residual of the current iterate. */
 int tse_fast_p_residual(tse_ctx *ctx, double *rel);
+/* Copies the assembled FP64 operator (natural layout, N doubles each: the
+ * "+" faces Wx, Wy, Wz, the diagonal D and the rhs b) into device buffers;
+ * a NULL destination is skipped.  No counterpart in the reference: an
+ * inspection hook for tests and tools. */
+int tse_fast_p_operator(tse_ctx *ctx, double *Wx, double *Wy, double *Wz,
+                        double *D, double *b);
 /* Writes the current iterate into a split-layout field. */
 int tse_fast_p_result(tse_ctx *ctx, double *P);
 /* Timing hook: n_sweeps fused sweeps (FP64 on p, or fp32 != 0: FP32 on the

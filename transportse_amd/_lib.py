@@ -120,6 +120,7 @@ SIGNATURES = {
     "tse_fast_p_solve": (_I, [_V, ctypes.POINTER(FastParams),
                               ctypes.POINTER(FastResult)]),
     "tse_fast_p_residual": (_I, [_V, _DPH]),
+    "tse_fast_p_operator": (_I, [_V, _dp, _dp, _dp, _dp, _dp]),
     "tse_fast_p_result": (_I, [_V, _dp]),
     "tse_fast_p_sweeps": (_I, [_V, ctypes.POINTER(SweepPlan), ctypes.c_int32,
                                _D, ctypes.c_int32]),

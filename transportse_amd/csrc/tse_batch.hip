@@ -650,6 +650,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
     const tse_sweep_plan &pl = prm->plan;
     const bool mixed_pairs = fp32 && kind == SYS_P && pl.variant == 4 && pl.by == 32 &&
                              pl.ry == 2 && (pl.ns == 2 || pl.ns == 3);
+    if (pl.variant == 5)   // the bfloat16 operator has no batched sweep
+        return fail(TSE_ERR_ARG, "sweep variant 5 is not supported in batches");
     if (!(fast_plan_ok(pl, fp32, kind) || mixed_pairs) || prm->inner < pl.ns ||
         prm->inner % pl.ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");

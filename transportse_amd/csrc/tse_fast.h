@@ -824,9 +824,260 @@ rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
                                       zero_init, sP, sWy);
 }
 
+// Variant 5 (FP32 arithmetic, even nx): the cell-pair z-march of variant 3
+// on an inner operator stored in 8 bytes per cell.  Each pair (x, x+1) has a
+// 16-byte record of four 32-bit words: Wx, Wy, Wz and S = V R (the storage
+// term) of the pair, each as two bfloat16 values (element 0 in the low
+// half), read with one 16-byte buffer load; the rhs and the unknown stay
+// FP32.  The diagonal is summed in the kernel from the same rounded values,
+// D' = S' + sum of the cell's six rounded faces, and S' = +inf marks a
+// Dirichlet cell (1/D' = 0).  The inner operator
+//   A' = sum_f W'_f L_f + diag(S'),  W'_f = W_f (1 + d_f),  S' = S (1 + d_S),
+// |d| <= 2^-9, is a graph Laplacian with perturbed face weights plus a
+// perturbed positive diagonal, so (1 - 2^-9) A <= A' <= (1 + 2^-9) A: a
+// correction cycle that solved A' e = r exactly would leave at most
+// 2^-9 / (1 - 2^-9) of the error (energy norm), and a cycle of SOR sweeps
+// leaves the SOR contraction plus that.  The defect correction (FP64
+// residual with the FP64 operator, same stop test) therefore reaches the
+// same tolerance with the same sweep count while cycles reduce the residual
+// by less than ~2 decades (adaptive cycles aim at most that far).  A launch
+// moves 8 + 4 + 4 + 4 = 20 bytes per cell (28 for the FP32 operator).
+//
+// No validity mask: in the correction equation every cell whose 1/diagonal
+// is 0 (Dirichlet cells, lanes outside the grid, planes outside the chunk)
+// holds e = 0, and the update leaves it at (1 - omega) * 0 = 0; halo cells
+// near an interior tile edge are updated with stale neighbours, which cannot
+// reach the owned cells (a cell at distance d from an interior tile edge is
+// correct after phase h iff d >= h + 1, whatever the cells nearer the edge
+// hold, as long as they stay finite: the LDS rings start at zero).  So every
+// lane updates in every phase with no select.
+__device__ __forceinline__ uint16_t bf16_rne(double w) {  // w >= 0, finite
+    uint32_t u = __float_as_uint((float)w);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ double bf16_value(uint16_t h) {
+    return (double)__uint_as_float((uint32_t)h << 16);
+}
+constexpr uint16_t kBf16Inf = 0x7f80u;
+template <int E>
+__device__ __forceinline__ float bf_el(uint32_t v) {  // element E of a packed pair
+    return E ? __uint_as_float(v & 0xffff0000u) : __uint_as_float(v << 16);
+}
+typedef unsigned int tse_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ tse_u32x4 buf_ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
+    return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
+}
+
+// LDS ring of packed Wy pairs: row r of a slot sits in 64-word group r / 2,
+// half (r & 1) ^ ((r / 2) & 1), so rows r and r + 2 (the two halves of a
+// wavefront) fall into different 32-word halves: no bank conflicts.
+template <int NS, int BYT>
+using WyRing = uint32_t[2 * NS + 2][BYT + 1][64];
+__device__ __forceinline__ int wy_word(int r, int ip) {
+    return (((r & 1) ^ ((r >> 1) & 1)) << 5) + ip;
+}
+
+template <int NS, int BYT, int Q>
+__device__ __forceinline__ void
+rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pbytes,
+                   uint32_t rbytes, uint32_t ocol, uint32_t orcol, uint32_t oout,
+                   __amdgpu_buffer_rsrc_t rP, __amdgpu_buffer_rsrc_t rRec,
+                   __amdgpu_buffer_rsrc_t rB, __amdgpu_buffer_rsrc_t rOut, float omega,
+                   float (&sP)[2 * NS + 2][2 * BYT + 2][64], WyRing<NS, BYT> &sWy) {
+    constexpr int R = 2 * NS;
+    constexpr int L = R + 2;
+    const int klast = k1 + R - 1;
+    const int y = ly + 1;             // LDS row
+    const int cm1[2] = {(((0 ^ ((y - 1) >> 1)) & 1) << 5) + ip, (((1 ^ ((y - 1) >> 1)) & 1) << 5) + ip};
+    const int c0[2] = {(((0 ^ (y >> 1)) & 1) << 5) + ip, (((1 ^ (y >> 1)) & 1) << 5) + ip};
+    const int cp1[2] = {(((0 ^ ((y + 1) >> 1)) & 1) << 5) + ip, (((1 ^ ((y + 1) >> 1)) & 1) << 5) + ip};
+    const int gw0 = y >> 1, ww0 = wy_word(y, ip);            // own row in the Wy ring
+    const int gwm = (y - 1) >> 1, wwm = wy_word(y - 1, ip);  // row below
+    float p[2][L], rinv[2][L], rb[2][L];
+    uint32_t rwx[L], rwy[L], rwz[L];
+    tse_f2 np;
+    uint32_t nwz, nsd;
+#pragma unroll
+    for (int r = 0; r < L; ++r) {
+        rwx[r] = rwy[r] = rwz[r] = 0u;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) p[e][r] = rinv[e][r] = rb[e][r] = 0.f;
+    }
+    {
+        const bool in = kl0 <= kl1;
+        const uint32_t o = ocol + (in ? (uint32_t)kl0 * pbytes : 0x80000000u);
+        const uint32_t orr = orcol + (in ? (uint32_t)kl0 * rbytes : 0x80000000u);
+        np = buf_ld2(rP, o);
+        const tse_u32x4 w = buf_ld4(rRec, orr);
+        rwx[0] = w.x;
+        rwy[0] = w.y;
+        nwz = w.z;
+        nsd = w.w;
+        const tse_f2 b = buf_ld2(rB, o);
+        rb[0][0] = b.x;
+        rb[1][0] = b.y;
+    }
+    const bool row_start = ip == 0;   // lanes 0 and 32
+    for (int kb = kl0; kb <= klast; kb += L) {
+#pragma unroll
+        for (int u = 0; u < L; ++u) {
+            const int k = kb + u;
+            if (k > klast) break;
+            const int un = (u + 1) % L;
+            p[0][u] = np.x;
+            p[1][u] = np.y;
+            rwz[u] = nwz;
+            const uint32_t sd = nsd;   // storage term of plane k
+            sP[u][y][c0[0]] = np.x;
+            sP[u][y][c0[1]] = np.y;
+            sWy[u][gw0][ww0] = rwy[u];
+            {   // plane k+1 (reads 0 past kl1 and outside the grid)
+                const bool in = k + 1 <= kl1;
+                const uint32_t o = ocol + (in ? (uint32_t)(k + 1) * pbytes : 0x80000000u);
+                const uint32_t orr =
+                    orcol + (in ? (uint32_t)(k + 1) * rbytes : 0x80000000u);
+                np = buf_ld2(rP, o);
+                const tse_u32x4 w = buf_ld4(rRec, orr);
+                rwx[un] = w.x;
+                rwy[un] = w.y;
+                nwz = w.z;
+                nsd = w.w;
+                const tse_f2 b = buf_ld2(rB, o);
+                rb[0][un] = b.x;
+                rb[1][un] = b.y;
+            }
+            __syncthreads();
+            {   // 1/diagonal of plane k (first used by phase 0 at step k+1):
+                // S' + the six rounded faces; S' = +inf (Dirichlet) gives 0,
+                // and so does a zero plane (outside the chunk or the grid)
+                const uint32_t wyb = sWy[u][gwm][wwm];
+                const uint32_t wxw = (uint32_t)lane_shr1((int)rwx[u]);
+                const int ub = (u + L - 1) % L;
+                const float west0 = row_start ? 0.f : bf_el<1>(wxw);
+                const float d0 = ((bf_el<0>(sd) + bf_el<0>(rwx[u])) + (west0 + bf_el<0>(rwy[u]))) +
+                                 ((bf_el<0>(wyb) + bf_el<0>(rwz[u])) + bf_el<0>(rwz[ub]));
+                const float d1 = ((bf_el<1>(sd) + bf_el<1>(rwx[u])) +
+                                  (bf_el<0>(rwx[u]) + bf_el<1>(rwy[u]))) +
+                                 ((bf_el<1>(wyb) + bf_el<1>(rwz[u])) + bf_el<1>(rwz[ub]));
+                rinv[0][u] = d0 > 0.f ? __builtin_amdgcn_rcpf(d0) : 0.f;
+                rinv[1][u] = d1 > 0.f ? __builtin_amdgcn_rcpf(d1) : 0.f;
+            }
+            const int eact = (Q + u) & 1;   // compile-time after unrolling
+#pragma unroll
+            for (int h = 0; h < R; ++h) {
+                const int sm = (u - 1 - h + 2 * L) % L;     // plane m = k-1-h
+                const int sa = (u - h + 2 * L) % L;         // plane m + 1
+                const int sb = (u - 2 - h + 2 * L) % L;     // plane m - 1
+                const uint32_t wys = sWy[sm][gwm][wwm];
+                float pl, pr, wxl, wxo, wyo, wym, wzo, wzb;
+                if (eact == 0) {
+                    pl = from_left(p[1][sm]);
+                    pr = p[1][sm];
+                    const float w = bf_el<1>((uint32_t)lane_shr1((int)rwx[sm]));
+                    wxl = row_start ? 0.f : w;
+                    wxo = bf_el<0>(rwx[sm]);
+                    wyo = bf_el<0>(rwy[sm]);
+                    wym = bf_el<0>(wys);
+                    wzo = bf_el<0>(rwz[sm]);
+                    wzb = bf_el<0>(rwz[sb]);
+                } else {
+                    pl = p[0][sm];
+                    pr = from_right(p[0][sm]);
+                    wxl = bf_el<0>(rwx[sm]);
+                    wxo = bf_el<1>(rwx[sm]);
+                    wyo = bf_el<1>(rwy[sm]);
+                    wym = bf_el<1>(wys);
+                    wzo = bf_el<1>(rwz[sm]);
+                    wzb = bf_el<1>(rwz[sb]);
+                }
+                float acc = rb[eact][sm];
+                acc += wxo * pr;
+                acc += wxl * pl;
+                acc += wyo * sP[sm][y + 1][cp1[eact]];
+                acc += wym * sP[sm][y - 1][cm1[eact]];
+                acc += wzo * p[eact][sa];
+                acc += wzb * p[eact][sb];
+                const float pc = p[eact][sm];
+                const float pv = pc + omega * (acc * rinv[eact][sm] - pc);
+                p[eact][sm] = pv;
+                sP[sm][y][c0[eact]] = pv;
+            }
+            const int mo = k - R;
+            const uint32_t oo = mo >= k0 && mo < k1 ? (uint32_t)mo * pbytes : 0x80000000u;
+            const int so = (u - R + 2 * L) % L;
+            tse_f2 out;
+            out.x = p[0][so];
+            out.y = p[1][so];
+            buf_st2(out, rOut, oout + oo);
+        }
+    }
+}
+
+template <int NS, int BYT>
+__device__ __forceinline__ void
+rb_zmarch_tile_x5(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                  float *__restrict__ Pout, const uint32_t *__restrict__ rec,
+                  const float *__restrict__ rhs, float omega, int zero_init) {
+    constexpr int R = 2 * NS;
+    constexpr int BX = 64;
+    constexpr int BY = 2 * BYT;
+    constexpr int L = R + 2;
+    static_assert(BY - 2 * R >= 1, "tile too small");
+    __shared__ ZRing<float, NS, 2 * BYT> sP;
+    __shared__ WyRing<NS, BYT> sWy;
+
+    const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
+    const int tx = tile - zc * g.tiles_x;
+    const int zc2 = (int)fdiv((uint32_t)zc, g.div_ty);
+    const int ty = zc - zc2 * g.tiles_y;
+    const Span sx = tile_span(g.edge, tx, g.tiles_x, g.nx, BX, R);
+    const Span sy = tile_span(g.edge, ty, g.tiles_y, g.ny, BY, R);
+    const int ip = threadIdx.x & 31;
+    const int ly = ((threadIdx.y >> 1) << 2) + (threadIdx.y & 1) + ((threadIdx.x >> 5) << 1);
+    const int i = sx.x0 + 2 * ip;
+    const int j = sy.x0 + ly;
+    const int k0 = zc2 * g.zchunk;
+    const int k1 = min(k0 + g.zchunk, g.nz);
+    const int kl0 = max(k0 - R, 0);
+    const int kl1 = min(k1 + R, g.nz) - 1;
+    const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
+    const uint32_t pbytes = plane * 4u, rbytes = plane * 8u;
+    const uint32_t fbytes = pbytes * (uint32_t)g.nz;
+    const auto rP = buf_rsrc(Pin, zero_init ? 0u : fbytes);
+    const auto rRec = buf_rsrc(rec, rbytes * (uint32_t)g.nz);
+    const auto rB = buf_rsrc(rhs, fbytes);
+    const auto rOut = buf_rsrc(Pout, fbytes);
+    const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
+    const uint32_t cell = (uint32_t)(i + g.nx * j);
+    const uint32_t ocol = in_dom ? cell * 4u : kBufMax;
+    const uint32_t orcol = in_dom ? cell * 8u : kBufMax;
+    const bool own = in_dom && i >= sx.lo && i < sx.hi && j >= sy.lo && j < sy.hi;
+    const uint32_t oout = own ? ocol : kBufMax;
+    // every ring slot starts at zero: the phases of the first steps read
+    // planes below the chunk, which nothing writes (and the zero guard rows
+    // 0 and BY+1 stay zero)
+    {
+        const int t = threadIdx.x + 64 * threadIdx.y;
+        for (int q = t; q < L * (BY + 2) * 64; q += 64 * BYT) (&sP[0][0][0])[q] = 0.f;
+        for (int q = t; q < L * (BYT + 1) * 64; q += 64 * BYT) (&sWy[0][0][0])[q] = 0u;
+        __syncthreads();
+    }
+    const int q = __builtin_amdgcn_readfirstlane((j + kl0 - 1) & 1);
+    if (q == 0)
+        rb_zmarch_x5_march<NS, BYT, 0>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes, ocol,
+                                       orcol, oout, rP, rRec, rB, rOut, omega, sP, sWy);
+    else
+        rb_zmarch_x5_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes, ocol,
+                                       orcol, oout, rP, rRec, rB, rOut, omega, sP, sWy);
+}
+
 struct FastOp {  // natural-layout operator arrays
     double *Wx, *Wy, *Wz, *D, *invD, *b;
     float *Wx32, *Wy32, *Wz32, *invD32;
+    uint16_t *rec16;   // variant-5 pair records (bfloat16): Wx, Wy, Wz, V R of
+                       // pair q at 8q..8q+7, element 0 first; V R = +inf marks
+                       // a Dirichlet cell
 };
 
 
@@ -1169,7 +1420,8 @@ __device__ inline void block_sum2_to(double a, double c, double *out) {
 enum SysKind { SYS_P = 0, SYS_TR = 1 };
 
 // One linear system in the performance form, owned by the context.
-//   pressure:  buf64 = Wx Wy Wz D invD b | p0 p1      buf32 = Wx Wy Wz invD r | e0 e1
+//   pressure:  buf64 = Wx Wy Wz D invD b | p0 p1
+//              buf32 = Wx Wy Wz invD rec16(2 arrays) r | e0 e1
 //   transport: buf64 = c[6] D invD b | p0 p1          buf32 = c[6] invD r | e0 e1
 struct FastSys {
     int kind = SYS_P;
@@ -1184,7 +1436,7 @@ struct FastSys {
     bool ready = false;
 
     int n64() const { return kind == SYS_P ? 6 : 9; }
-    int n32() const { return kind == SYS_P ? 5 : 8; }
+    int n32() const { return kind == SYS_P ? 7 : 8; }
     double *p(int q) const { return buf64 + (n64() + q) * ld; }
     float *r32() const { return buf32 + (n32() - 1) * ld; }
     float *e(int q) const { return buf32 + (n32() + q) * ld; }
@@ -1200,6 +1452,7 @@ struct FastSys {
         op.Wy32 = buf32 + ld;
         op.Wz32 = buf32 + 2 * ld;
         op.invD32 = buf32 + 3 * ld;
+        op.rec16 = (uint16_t *)(buf32 + 4 * ld);
         return op;
     }
     TrOp trop() const {

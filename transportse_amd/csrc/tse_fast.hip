@@ -82,6 +82,18 @@ rb_zmarch_x2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                      zero_init);
 }
 
+// Variant 5: cell pairs on the bfloat16-face operator (rec: 12-byte pair
+// records of packed Wx, Wy, Wz; invD from the rounded faces).
+template <int NS, int BYT>
+__global__ void __launch_bounds__(64 * BYT)
+rb_zmarch_x5_kernel(ZGeo g, const float *__restrict__ Pin, float *__restrict__ Pout,
+                    const uint32_t *__restrict__ rec, const float *__restrict__ rhs,
+                    float omega, int zero_init) {
+    const int tile = xcd_tile(g);
+    if (tile >= g.ntiles) return;
+    rb_zmarch_tile_x5<NS, BYT>(g, tile, Pin, Pout, rec, rhs, omega, zero_init);
+}
+
 // ---- operator assembly (from split-layout fields) ---------------------------------------
 
 __device__ inline double f_dsp(const Geo &G, int n, int i, int j, int k) {
@@ -184,6 +196,14 @@ fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny,
         op.Wy32[t] = (float)W[2];
         op.Wz32[t] = (float)W[4];
         op.invD32[t] = (float)inv;
+        // variant-5 record: the "+" faces and the storage term rounded to
+        // bfloat16 (the sweep sums the diagonal from them), +inf for a cell
+        // that does not update
+        uint16_t *q = op.rec16 + 8 * (t >> 1) + (t & 1);
+        q[0] = bf16_rne(W[0]);
+        q[2] = bf16_rne(W[2]);
+        q[4] = bf16_rne(W[4]);
+        q[6] = active ? bf16_rne(V * R) : kBf16Inf;
     }
 }
 
@@ -537,6 +557,21 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
 }
 
+static int launch_zmarch5(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
+                          const float *Pin, float *Pout, const FastOp &op,
+                          const float *rhs, float omega, int zero_init) {
+    const dim3 grid(zg.per_xcd * 8);
+    if (pl.variant == 5 && pl.by == 32 && pl.ry == 2 && (pl.ns == 2 || pl.ns == 3)) {
+        const uint32_t *rec = (const uint32_t *)op.rec16;
+        auto k = pl.ns == 2 ? rb_zmarch_x5_kernel<2, 16> : rb_zmarch_x5_kernel<3, 16>;
+        hipLaunchKernelGGL(k, grid, dim3(64, 16), 0, st, zg, Pin, Pout, rec, rhs, omega,
+                           zero_init);
+        TSE_HIP_TRY(hipGetLastError());
+        return 0;
+    }
+    return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
+}
+
 template <typename T>
 static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
                             const T *Pin, T *Pout, const T *coef, long ld,
@@ -566,11 +601,13 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan
 }
 
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
+    if (pl.variant == 5) return N * 8L <= (long)kBufMax;   // 8-byte records per cell
     return pl.variant < 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
 }
 
 bool fast_plan_grid_ok(const tse_sweep_plan &pl, int nx) {
-    return pl.variant != 3 || nx % 2 == 0;   // variant 3 reads cell pairs
+    // variants 3 and 5 read cell pairs
+    return (pl.variant != 3 && pl.variant != 5) || nx % 2 == 0;
 }
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
@@ -587,7 +624,8 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
                                  {0, 3, 16, 1}, {1, 2, 24, 2}, {1, 2, 32, 2},
                                  {1, 3, 16, 1}, {1, 3, 24, 2}, {2, 2, 24, 2},
                                  {2, 2, 32, 2}, {2, 3, 16, 1}, {2, 3, 24, 2},
-                                 {2, 3, 32, 2}, {3, 2, 32, 2}, {3, 3, 32, 2}};
+                                 {2, 3, 32, 2}, {3, 2, 32, 2}, {3, 3, 32, 2},
+                                 {5, 2, 32, 2}, {5, 3, 32, 2}};
     auto hit = [&](const int c[4]) {
         return pl.variant == c[0] && pl.ns == c[1] && pl.by == c[2] && pl.ry == c[3];
     };
@@ -647,7 +685,11 @@ static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
     int e = 0;
     for (int q = 0; q < n / pl.ns; ++q) {
         int rc;
-        if (s->kind == SYS_P) {
+        if (s->kind == SYS_P && pl.variant == 5) {
+            FastOp op = s->pop();
+            rc = launch_zmarch5(ctx->stream, zg, pl, s->e(e), s->e(1 - e), op,
+                                s->r32(), sor, q == 0);
+        } else if (s->kind == SYS_P) {
             FastOp op = s->pop();
             rc = launch_zmarch<float>(ctx->stream, zg, pl,
                                       s->e(e), s->e(1 - e), op.Wx32, op.Wy32,
@@ -701,9 +743,10 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     if (!fast_plan_ok(*plan, fp32, s->kind) || n_sweeps < 0 || n_sweeps % plan->ns)
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
     if (!fast_plan_fits(*plan, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variants 2 and 3 need fields of at most 1 GiB");
+        return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
+                                 "(variant 5: 8 bytes per cell)");
     if (!fast_plan_grid_ok(*plan, ctx->geo.nx))
-        return fail(TSE_ERR_ARG, "sweep variant 3 needs an even nx");
+        return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
     const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk,
                               plan->variant >= 2);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
@@ -726,9 +769,10 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (prm->cycle_mode != TSE_CYCLE_FIXED && prm->cycle_mode != TSE_CYCLE_ADAPTIVE)
         return fail(TSE_ERR_ARG, "cycle_mode must be TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE");
     if (!fast_plan_fits(pl, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variants 2 and 3 need fields of at most 1 GiB");
+        return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
+                                 "(variant 5: 8 bytes per cell)");
     if (!fast_plan_grid_ok(pl, ctx->geo.nx))
-        return fail(TSE_ERR_ARG, "sweep variant 3 needs an even nx");
+        return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
     const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant >= 2);
     double rn, bn;
     int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
@@ -766,7 +810,12 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             long next = inner;
             if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
                 const double rate = std::log(rel / rel_prev) / run;     // < 0
-                const double need = std::log(prm->tol / rel) / rate;     // > 0
+                // a variant-5 cycle cannot reduce the error by much more
+                // than its inner operator's 2^-9 accuracy: aim at most two
+                // decades per cycle, so its sweeps are not spent past that
+                const double target =
+                    pl.variant == 5 ? std::max(prm->tol / rel, 1e-2) : prm->tol / rel;
+                const double need = std::log(target) / rate;             // > 0
                 next = (long)std::ceil(need * 1.02);
             }
             next = std::min<long>(std::max<long>(next, inner), 8L * inner);
@@ -810,6 +859,22 @@ int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt) {
 int tse_fast_p_result(tse_ctx *ctx, double *P) {
     int rc = check_ctx(ctx);
     return rc ? rc : sys_result(ctx, system_of(ctx, SYS_P), P);
+}
+
+int tse_fast_p_operator(tse_ctx *ctx, double *Wx, double *Wy, double *Wz, double *D,
+                        double *b) {
+    int rc = check_ctx(ctx);
+    if (rc) return rc;
+    FastSys *s = system_of(ctx, SYS_P);
+    if (!s->ready) return fail(TSE_ERR_STATE, "fast form not set up");
+    const FastOp op = s->pop();
+    const double *src[5] = {op.Wx, op.Wy, op.Wz, op.D, op.b};
+    double *dst[5] = {Wx, Wy, Wz, D, b};
+    for (int a = 0; a < 5; ++a)
+        if (dst[a])
+            TSE_HIP_TRY(hipMemcpyAsync(dst[a], src[a], sizeof(double) * s->N,
+                                       hipMemcpyDeviceToDevice, ctx->stream));
+    return 0;
 }
 
 int tse_fast_p_residual(tse_ctx *ctx, double *rel) {
@@ -946,8 +1011,10 @@ static tse_sweep_plan default_fast_plan(const tse_ctx *ctx, int kind, int fp32,
     const long N = (long)G.nx * G.ny * G.nz;
     tse_sweep_plan pl{explicit_step ? 1 : 2, 16, 1, 0, 2};
     if (kind == SYS_P && fp32) {
-        pl = tse_sweep_plan{3, 32, 2, 0, 3};
-        if (!fast_plan_grid_ok(pl, G.nx)) pl.variant = 2;
+        // the cell-pair sweep on the bfloat16 inner operator where nx is
+        // even and its records fit, else the unrolled-ring sweep
+        pl = tse_sweep_plan{3, 32, 2, 0, 5};
+        if (!fast_plan_grid_ok(pl, G.nx) || !fast_plan_fits(pl, N, fp32)) pl.variant = 2;
     }
     if (!fast_plan_fits(pl, N, fp32)) {  // fields over 1 GiB: no buffer loads
         pl = tse_sweep_plan{explicit_step ? 1 : 2, 16, 1, 0, 0};

@@ -183,7 +183,7 @@ def field_bytes(spec: ScenarioSpec) -> int:
     n = spec.cells
     S = spec.species
     own = 16 + 3 * S                      # split-layout FP64 fields
-    fast_p = 8 * 8 + 7 * 4                # bytes per cell
+    fast_p = 8 * 8 + 9 * 4                # bytes per cell
     fast_tr = 11 * 8 + 10 * 4
     rec = 13 * 8 + 1
     return int(n * (own * 8 + fast_p + fast_tr + rec) * 1.02) + (1 << 20)
