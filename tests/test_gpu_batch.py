@@ -203,11 +203,13 @@ def test_per_member_sor_and_rearm_many():
 EVEN_SHAPES = [(36, 23, 19), (64, 40, 33), (20, 50, 17), (90, 31, 12), (46, 45, 45)]
 
 
-@pytest.mark.parametrize("plan", [(3, 32, 2, 0, 3), (2, 32, 2, 7, 3)])
+@pytest.mark.parametrize("plan", [(3, 32, 2, 0, 3), (2, 32, 2, 7, 3), (3, 32, 2, 0, 5),
+                                  (2, 32, 2, 7, 5), (3, 32, 2, 9, 5)])
 def test_batch_pair_variant_matches_single_solves(plan):
-    """Sweep variant 3 (cell pairs, even nx) in the persistent batch kernel:
-    each member bit-identical to its single-system solve, with and without
-    graph replay."""
+    """Sweep variants 3 (cell pairs, even nx) and 5 (the same on the
+    bfloat16 inner operator) in the persistent batch kernel: each member
+    bit-identical to its single-system solve, with and without graph
+    replay."""
     import torch
     from transportse_amd.batch import Batch
     tol = 1e-10
@@ -232,15 +234,17 @@ def test_batch_pair_variant_matches_single_solves(plan):
             b.close()
             for s in systems:
                 s.assemble()
-        print("batch variant 3", plan, "ok", flush=True)
+        print("batch pair variant", plan, "ok", flush=True)
 
 
 @pytest.mark.parametrize("ns,zchunk", [(2, 0), (2, 7), (3, 0), (3, 9)])
-def test_batch_mixed_parity_variant(ns, zchunk):
-    """Batch variant 4: members with an even nx run the cell-pair sweep,
+@pytest.mark.parametrize("var", [4, 6])
+def test_batch_mixed_parity_variant(ns, zchunk, var):
+    """Batch variants 4 and 6: members with an even nx run the cell-pair
+    sweep (variant 4: on the FP32 operator, 6: on the bfloat16 operator),
     the others the unrolled-ring sweep, in one persistent launch; each is
     bit-identical to its single-system solve (variants 2 and 3 agree bit
-    for bit)."""
+    for bit; variant 6's even members against single variant-5 solves)."""
     import torch
     from transportse_amd.batch import Batch
     tol = 1e-10
@@ -248,13 +252,14 @@ def test_batch_mixed_parity_variant(ns, zchunk):
     with torch.cuda.stream(_stream()):
         systems = [_pressure(sh, 80 + n) for n, sh in enumerate(shapes)]
         alone = []
-        for s in systems:
-            st = s.solve("mixed", tol=tol, sor=1.9, inner=18, plan=(ns, 32, 2, zchunk, 2),
-                         max_sweeps=20000)
+        for s, sh in zip(systems, shapes):
+            single = 5 if var == 6 and sh[0] % 2 == 0 else 2
+            st = s.solve("mixed", tol=tol, sor=1.9, inner=18,
+                         plan=(ns, 32, 2, zchunk, single), max_sweeps=20000)
             assert st.converged
             alone.append((st, s.result()))
             s.assemble()
-        b = Batch(0, "mixed", (ns, 32, 2, zchunk, 4), inner=18, tol=tol, sor=1.9,
+        b = Batch(0, "mixed", (ns, 32, 2, zchunk, var), inner=18, tol=tol, sor=1.9,
                   max_sweeps=20000)
         slots = [b.add(s) for s in systems]
         b.run(graph=True)
@@ -264,4 +269,4 @@ def test_batch_mixed_parity_variant(ns, zchunk):
             assert ms.converged and ms.sweeps == st.sweeps
             assert np.array_equal(x, s.result())
         b.close()
-        print("batch variant 4", (ns, zchunk), "ok", flush=True)
+        print("batch variant", var, (ns, zchunk), "ok", flush=True)

@@ -205,9 +205,11 @@ def test_solves_agree_with_the_fp32_operator(shape, mode):
 
 
 def test_variant5_limits():
-    """Odd nx is refused; batches refuse variant 5."""
+    """Odd nx is refused by a single solve and by a variant-5 batch; FP64
+    sweeps have no variant 5."""
     from transportse_amd import _lib
     from transportse_amd.batch import Batch
+    from transportse_amd.pressure import FastPressure
     f = Fields(83, 41, 23, seed=27)
     hip, d = _setup(f, 3600.0)
     prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, 5), inner=18,
@@ -217,5 +219,11 @@ def test_variant5_limits():
     assert b"even nx" in hip.lib.tse_last_error()
     prm.form = 1   # FP64 sweeps have no variant 5
     assert hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm), ctypes.byref(res)) < 0
-    with pytest.raises(_lib.TseError, match="variant 5"):
-        Batch(0, "mixed", (3, 32, 2, 0, 5), inner=18, tol=1e-10, sor=1.9)
+    fp = FastPressure(*f.grid_args())
+    fp.load(f.P, 3600.0, **{n: getattr(f, n) for n in
+                            ("c_f", "nue", "rho_f", "phi", "Kx", "Ky", "Kz",
+                             "BP", "WP", "c_r")})
+    b = Batch(0, "mixed", (3, 32, 2, 0, 5), inner=18, tol=1e-10, sor=1.9)
+    with pytest.raises(_lib.TseError, match="even nx"):
+        b.add(fp)
+    b.close()

@@ -1016,16 +1016,15 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
 
 template <int NS, int BYT>
 __device__ __forceinline__ void
-rb_zmarch_tile_x5(const ZGeo &g, int tile, const float *__restrict__ Pin,
-                  float *__restrict__ Pout, const uint32_t *__restrict__ rec,
-                  const float *__restrict__ rhs, float omega, int zero_init) {
+rb_zmarch_tile_x5_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                     float *__restrict__ Pout, const uint32_t *__restrict__ rec,
+                     const float *__restrict__ rhs, float omega, int zero_init,
+                     ZRing<float, NS, 2 * BYT> &sP, WyRing<NS, BYT> &sWy) {
     constexpr int R = 2 * NS;
     constexpr int BX = 64;
     constexpr int BY = 2 * BYT;
     constexpr int L = R + 2;
     static_assert(BY - 2 * R >= 1, "tile too small");
-    __shared__ ZRing<float, NS, 2 * BYT> sP;
-    __shared__ WyRing<NS, BYT> sWy;
 
     const int zc = (int)fdiv((uint32_t)tile, g.div_tx);
     const int tx = tile - zc * g.tiles_x;
@@ -1070,6 +1069,17 @@ rb_zmarch_tile_x5(const ZGeo &g, int tile, const float *__restrict__ Pin,
     else
         rb_zmarch_x5_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes, ocol,
                                        orcol, oout, rP, rRec, rB, rOut, omega, sP, sWy);
+}
+
+template <int NS, int BYT>
+__device__ __forceinline__ void
+rb_zmarch_tile_x5(const ZGeo &g, int tile, const float *__restrict__ Pin,
+                  float *__restrict__ Pout, const uint32_t *__restrict__ rec,
+                  const float *__restrict__ rhs, float omega, int zero_init) {
+    __shared__ ZRing<float, NS, 2 * BYT> sP;
+    __shared__ WyRing<NS, BYT> sWy;
+    rb_zmarch_tile_x5_in<NS, BYT>(g, tile, Pin, Pout, rec, rhs, omega, zero_init, sP,
+                                  sWy);
 }
 
 struct FastOp {  // natural-layout operator arrays

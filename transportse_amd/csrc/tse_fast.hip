@@ -601,7 +601,8 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan
 }
 
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
-    if (pl.variant == 5) return N * 8L <= (long)kBufMax;   // 8-byte records per cell
+    // variant 5 (and 6, a batch's variant 5 for even nx): 8-byte records per cell
+    if (pl.variant == 5 || pl.variant == 6) return N * 8L <= (long)kBufMax;
     return pl.variant < 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
 }
 

@@ -121,8 +121,10 @@ class Server:
                  plan_p=None, plan_tr=(2, 16, 1, 64, 2), inner_p=51,
                  inner_tr=6, sor_p=1.95, sor_tr=1.0, device=0,
                  max_sweeps=400000, keep_results=False):
-        """plan_p None: the cell-pair sweep (variant 3) when every request of
-        a run has an even nx, else the unrolled-ring sweep (variant 2)."""
+        """plan_p None: the cell-pair sweep on the bfloat16 inner operator
+        (variant 5) when every request of a run has an even nx, else that
+        sweep for the even-nx requests and the unrolled-ring sweep for the
+        others (variant 6)."""
         import torch
         self.auto_plan_p = plan_p is None
         if plan_p is None:
@@ -145,7 +147,7 @@ class Server:
         reqs = sorted(requests, key=lambda r: r.arrival)
         if self.auto_plan_p:
             even = all(r.spec.nx % 2 == 0 for r in reqs)
-            plan = (3, 32, 2, 64, 3 if even else 2)
+            plan = (3, 32, 2, 64, 5 if even else 6)
             bp = self.batches[KIND_P]
             if bp.plan != plan and not bp.members:
                 form, tol, inner, sor, max_sweeps = self.p_args
