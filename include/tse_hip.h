@@ -238,7 +238,7 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *              (not batches): variant 3's cell pairs on an inner operator
  *              stored as bfloat16 (the three faces and the storage term of
  *              each cell; the diagonal is summed in the kernel from the
- *              rounded values), a graph Laplacian within a factor 1 +- 2^-9
+ *              rounded values), a graph Laplacian within a factor 1 +- 2^-8
  *              of the FP64 one, so the defect correction keeps its stop
  *              test and sweep count; 20 instead of 28 bytes per cell per
  *              launch; even nx, at most 2^27 cells

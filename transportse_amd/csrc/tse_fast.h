@@ -833,14 +833,15 @@ rb_zmarch_tile_x2(const ZGeo &g, int tile, const float *__restrict__ Pin,
 // D' = S' + sum of the cell's six rounded faces, and S' = +inf marks a
 // Dirichlet cell (1/D' = 0).  The inner operator
 //   A' = sum_f W'_f L_f + diag(S'),  W'_f = W_f (1 + d_f),  S' = S (1 + d_S),
-// |d| <= 2^-9, is a graph Laplacian with perturbed face weights plus a
-// perturbed positive diagonal, so (1 - 2^-9) A <= A' <= (1 + 2^-9) A: a
-// correction cycle that solved A' e = r exactly would leave at most
-// 2^-9 / (1 - 2^-9) of the error (energy norm), and a cycle of SOR sweeps
-// leaves the SOR contraction plus that.  The defect correction (FP64
-// residual with the FP64 operator, same stop test) therefore reaches the
-// same tolerance with the same sweep count while cycles reduce the residual
-// by less than ~2 decades (adaptive cycles aim at most that far).  A launch
+// |d| <= u = 2^-8 (bfloat16's unit roundoff), is a graph Laplacian with
+// perturbed face weights plus a perturbed positive diagonal, so
+// (1 - u) A <= A' <= (1 + u) A: a correction cycle that solved A' e = r
+// exactly would leave at most u / (1 - u) of the error (energy norm), and a
+// cycle of SOR sweeps leaves the SOR contraction plus that.  The defect
+// correction (FP64 residual with the FP64 operator, same stop test)
+// therefore reaches the same tolerance with the same sweep count while
+// cycles reduce the residual by less than ~2 decades (adaptive cycles aim at
+// most that far).  A launch
 // moves 8 + 4 + 4 + 4 = 20 bytes per cell (28 for the FP32 operator).
 //
 // No validity mask: in the correction equation every cell whose 1/diagonal

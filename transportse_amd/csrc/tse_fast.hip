@@ -812,7 +812,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
                 const double rate = std::log(rel / rel_prev) / run;     // < 0
                 // a variant-5 cycle cannot reduce the error by much more
-                // than its inner operator's 2^-9 accuracy: aim at most two
+                // than its inner operator's 2^-8 accuracy: aim at most two
                 // decades per cycle, so its sweeps are not spent past that
                 const double target =
                     pl.variant == 5 ? std::max(prm->tol / rel, 1e-2) : prm->tol / rel;
