@@ -214,7 +214,8 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
 /* The compute_P system (tse.py:657-697 at its fixed point) scaled by the cell
  * volume is symmetric: each cell stores its three "+" face transmissibilities,
  * 1/diagonal and the right-hand side (natural x-contiguous layout, owned by
- * the context).  Sweeps are fused red+black z-marches (NS sweeps per launch);
+ * the context; its rows are nx rounded up to even: an odd nx gets one
+ * inactive column).  Sweeps are fused red+black z-marches (NS sweeps per launch);
  * convergence is the relative residual ||b - A p|| / ||b|| in FP64.
  *   TSE_FORM_PLAIN64: FP64 sweeps on p.
  *   TSE_FORM_MIXED:   FP32 sweeps on the correction A e = r, r = b - A p in
@@ -225,25 +226,23 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  * (x neighbours through LDS), 1 (x neighbours exchanged across the lanes
  * of a wavefront; smaller LDS ring), 2 (as 1, with the register rings
  * rotated by unrolling and buffer loads; fields of at most 1 GiB, else
- * TSE_ERR_ARG) or 3 (FP32 only, as 2 with an x-pair of cells per thread,
- * so that every lane updates in every phase; nx must be even, else
- * TSE_ERR_ARG).  Supported (ns, by, ry):
+ * TSE_ERR_ARG), 3 (FP32 only, as 2 with an x-pair of cells per thread, so
+ * that every lane updates in every phase; pressure rows are even) or 5
+ * (below).  Supported (ns, by, ry):
  *   variant 0: (1,8,1) (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also
  *              (1,32,2) (2,32,2) (2,32,4) (3,16,1)
  *   variant 1: (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also (2,24,2)
  *              (2,32,2) (3,16,1) (3,24,2)
  *   variant 2: as variant 1, FP32 sweeps also (3,32,2)
  *   variant 3: (2,32,2) (3,32,2)
- *   variant 5: (2,32,2) (3,32,2), FP32 pressure sweeps only, single systems
- *              (not batches): variant 3's cell pairs on an inner operator
- *              stored as bfloat16 (the three faces and the storage term of
- *              each cell; the diagonal is summed in the kernel from the
- *              rounded values), a graph Laplacian within a factor 1 +- 2^-8
- *              of the FP64 one, so the defect correction keeps its stop
- *              test and sweep count; 20 instead of 28 bytes per cell per
- *              launch; even nx, at most 2^27 cells
- * Batches (tse_batch_begin) also take variant 4 with (2,32,2) (3,32,2):
- * variant 3 for members with an even nx, variant 2 for the others.
+ *   variant 5: (2,32,2) (3,32,2), FP32 pressure sweeps only (single
+ *              systems and batches): variant 3's cell pairs on an inner
+ *              operator stored as bfloat16 (the three faces and the
+ *              storage term of each cell; the diagonal is summed in the
+ *              kernel from the rounded values), a graph Laplacian within a
+ *              factor 1 +- 2^-8 of the FP64 one, so the defect correction
+ *              keeps its stop test and sweep count; 20 instead of 28 bytes
+ *              per cell per launch; at most 2^27 cells
  * Transport systems: variant 0 or 2, (1,8,1) (1,16,1) (2,16,1), FP32
  * (3,16,1). */
 typedef struct {
@@ -289,8 +288,9 @@ int tse_fast_p_solve(tse_ctx *ctx, const tse_fast_params *prm,
                      tse_fast_result *out);
 /* Relative residual of the current iterate. */
 int tse_fast_p_residual(tse_ctx *ctx, double *rel);
-/* Copies the assembled FP64 operator (natural layout, N doubles each: the
- * "+" faces Wx, Wy, Wz, the diagonal D and the rhs b) into device buffers;
+/* Copies the assembled FP64 operator (natural layout with rows of nx
+ * rounded up to even, nxf * ny * nz doubles each: the "+" faces Wx, Wy, Wz,
+ * the diagonal D and the rhs b) into device buffers;
  * a NULL destination is skipped.  No counterpart in the reference: an
  * inspection hook for tests and tools. */
 int tse_fast_p_operator(tse_ctx *ctx, double *Wx, double *Wy, double *Wz,

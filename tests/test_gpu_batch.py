@@ -206,10 +206,9 @@ EVEN_SHAPES = [(36, 23, 19), (64, 40, 33), (20, 50, 17), (90, 31, 12), (46, 45, 
 @pytest.mark.parametrize("plan", [(3, 32, 2, 0, 3), (2, 32, 2, 7, 3), (3, 32, 2, 0, 5),
                                   (2, 32, 2, 7, 5), (3, 32, 2, 9, 5)])
 def test_batch_pair_variant_matches_single_solves(plan):
-    """Sweep variants 3 (cell pairs, even nx) and 5 (the same on the
-    bfloat16 inner operator) in the persistent batch kernel: each member
-    bit-identical to its single-system solve, with and without graph
-    replay."""
+    """Sweep variants 3 (cell pairs) and 5 (the same on the bfloat16 inner
+    operator) in the persistent batch kernel: each member bit-identical to
+    its single-system solve, with and without graph replay."""
     import torch
     from transportse_amd.batch import Batch
     tol = 1e-10
@@ -238,29 +237,25 @@ def test_batch_pair_variant_matches_single_solves(plan):
 
 
 @pytest.mark.parametrize("ns,zchunk", [(2, 0), (2, 7), (3, 0), (3, 9)])
-@pytest.mark.parametrize("var", [4, 6])
-def test_batch_mixed_parity_variant(ns, zchunk, var):
-    """Batch variants 4 and 6: members with an even nx run the cell-pair
-    sweep (variant 4: on the FP32 operator, 6: on the bfloat16 operator),
-    the others the unrolled-ring sweep, in one persistent launch; each is
-    bit-identical to its single-system solve (variants 2 and 3 agree bit
-    for bit; variant 6's even members against single variant-5 solves)."""
+@pytest.mark.parametrize("var", [3, 5])
+def test_batch_odd_and_even_nx(ns, zchunk, var):
+    """Members with odd and even nx in one batch of a cell-pair plan (odd
+    rows get an inactive pad column): each member bit-identical to its
+    single-system solve with the same plan, with graph replay."""
     import torch
     from transportse_amd.batch import Batch
     tol = 1e-10
     shapes = [(36, 23, 19), (37, 23, 19), (64, 40, 33), (45, 45, 45), (90, 31, 12)]
+    plan = (ns, 32, 2, zchunk, var)
     with torch.cuda.stream(_stream()):
         systems = [_pressure(sh, 80 + n) for n, sh in enumerate(shapes)]
         alone = []
-        for s, sh in zip(systems, shapes):
-            single = 5 if var == 6 and sh[0] % 2 == 0 else 2
-            st = s.solve("mixed", tol=tol, sor=1.9, inner=18,
-                         plan=(ns, 32, 2, zchunk, single), max_sweeps=20000)
+        for s in systems:
+            st = s.solve("mixed", tol=tol, sor=1.9, inner=18, plan=plan, max_sweeps=20000)
             assert st.converged
             alone.append((st, s.result()))
             s.assemble()
-        b = Batch(0, "mixed", (ns, 32, 2, zchunk, var), inner=18, tol=tol, sor=1.9,
-                  max_sweeps=20000)
+        b = Batch(0, "mixed", plan, inner=18, tol=tol, sor=1.9, max_sweeps=20000)
         slots = [b.add(s) for s in systems]
         b.run(graph=True)
         for (st, x), s, slot in zip(alone, systems, slots):
@@ -269,4 +264,4 @@ def test_batch_mixed_parity_variant(ns, zchunk, var):
             assert ms.converged and ms.sweeps == st.sweeps
             assert np.array_equal(x, s.result())
         b.close()
-        print("batch variant", var, (ns, zchunk), "ok", flush=True)
+        print("batch odd and even nx", plan, "ok", flush=True)

@@ -184,8 +184,8 @@ def test_unrolled_ring_variant_equals_lane_exchange_variant(ns, by, ry, zchunk):
     assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
 
 
-# variant 3 (an x-pair of cells per thread): even nx; partial and single
-# tiles in x and y, several middle tiles, unit y / z extents
+# variant 3 (an x-pair of cells per thread): partial and single tiles in x
+# and y, several middle tiles, unit y / z extents
 PAIR_GRIDS = [(84, 41, 23), (40, 37, 21), (160, 90, 30), (64, 33, 2), (2, 20, 30),
               (52, 1, 60)]
 
@@ -214,27 +214,55 @@ def test_pair_variant_equals_unrolled_ring_variant(shape, ns, zchunk):
     assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
 
 
-def test_pair_variant_needs_even_nx():
-    """An odd nx is refused by the single-system solve and by a batch."""
+@pytest.mark.parametrize("shape", [(83, 41, 23), (33, 20, 17), (1, 20, 30), (9, 40, 12)])
+def test_pair_variant_on_odd_nx(shape):
+    """An odd nx: the natural layout gets one inactive column, so the cell-pair
+    sweep (variant 3) runs on it, bit-identical to the unrolled-ring sweep
+    (variant 2) on the same layout, single and batched; the solution matches
+    the oracle's red/black sweeps (FP64 sweeps of variant 2 at 1e-10 of the
+    change)."""
+    from oracle import Oracle
     from transportse_amd import _lib
     from transportse_amd.batch import Batch
     from transportse_amd.pressure import FastPressure
-    f = Fields(83, 41, 23, seed=27)
-    hip, d = _setup(f, 3600.0)
-    prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(ns=3, by=32, ry=2, zchunk=0,
-                                                      variant=3),
-                          inner=18, max_sweeps=12000, sor=1.9, tol=1e-10)
-    res = _lib.FastResult()
-    assert hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm), ctypes.byref(res)) < 0
-    assert b"even nx" in hip.lib.tse_last_error()
+    f = Fields(*shape, seed=27, dirichlet_p=shape[0] > 2)
+    dt, sor, n = 3600.0, 1.7, 4
+    o = Oracle(*f.grid_args(), frozen_source=types.SimpleNamespace(c_r=f.c_r))
+    Po = f.P.copy()
+    for _ in range(n):
+        for colour in (0, 1):
+            o.p_half_sweep(Po, f.P, f.c_f, f.nue, f.rho_f, f.phi, f.Kx, f.Ky,
+                           f.Kz, f.BP, f.WP, dt, sor, colour)
+    hip, d = _setup(f, dt)
+    plan = _lib.SweepPlan(2, 16, 1, 0, 2)
+    _lib.check(hip.lib.tse_fast_p_sweeps(hip.ctx, ctypes.byref(plan), n, sor, 0), "sweeps")
+    change = np.max(np.abs(Po - f.P))
+    assert np.max(np.abs(_result(hip, d) - Po)) <= 1e-10 * change + 1e-13 * np.max(np.abs(Po))
+    sol = []
+    for var in (2, 3):
+        hip, d = _setup(f, 3600.0)
+        prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, var), inner=18,
+                              max_sweeps=12000, sor=1.9, tol=1e-10)
+        res = _lib.FastResult()
+        _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
+                                            ctypes.byref(res)), "solve")
+        assert res.converged
+        sol.append((_result(hip, d), res.sweeps))
+    assert sol[0][1] == sol[1][1] and np.array_equal(sol[0][0], sol[1][0])
     fp = FastPressure(*f.grid_args())
-    fp.load(f.P, 3600.0, **{n: getattr(f, n) for n in
+    fp.load(f.P, 3600.0, **{nm: getattr(f, nm) for nm in
                             ("c_f", "nue", "rho_f", "phi", "Kx", "Ky", "Kz",
                              "BP", "WP", "c_r")})
-    b = Batch(0, "mixed", (3, 32, 2, 0, 3), inner=18, tol=1e-10, sor=1.9)
-    with pytest.raises(_lib.TseError, match="even nx"):
-        b.add(fp)
+    b = Batch(0, "mixed", (3, 32, 2, 0, 3), inner=18, tol=1e-10, sor=1.9,
+              max_sweeps=12000)
+    slot = b.add(fp)
+    b.run(graph=False)
+    ms = b.status(slot)
+    b.writeback(slot)
+    assert ms.converged and ms.sweeps == sol[1][1]
+    assert np.array_equal(fp.result(), sol[1][0])
     b.close()
+    print("odd nx", shape, "pair variant ok,", sol[1][1], "sweeps", flush=True)
 
 
 DEGENERATE = [(51, 51, 1), (1, 1, 145), (200, 1, 60), (1, 20, 30), (3, 2, 1), (64, 33, 2)]

@@ -2,8 +2,8 @@
 
 Members of a batched ensemble of the ensemble workload's kind (heat + one
 species, sigma_lnK in [0.5, 2], slope in [0, 10] deg, top salt Dirichlet in
-[0.2, 1], uneven grids with odd and even nx, the mixed-parity cell-pair
-sweep plan, hipGraph-replayed cycles), and requests of the serving
+[0.2, 1], uneven grids with odd and even nx, the cell-pair sweep on the
+bfloat16 inner operator, hipGraph-replayed cycles), and requests of the serving
 workload's kind (heat + three species, sigma_lnK in [0.5, 2.5]) served
 continuously in a mix of four, are compared with the oracle's SIA steps
 (tests/helpers.py oracle_sia_steps: the reference functions in tse.py
@@ -58,9 +58,9 @@ def _ensemble_specs():
 def test_ensemble_members_match_oracle():
     from transportse_amd.ensemble import Ensemble
     specs = _ensemble_specs()
-    # the plans the ensemble benchmark runs: cell-pair sweep for the even-nx
-    # member and the unrolled-ring sweep for the odd ones in one launch
-    ens = Ensemble(specs, graph=True, plan_p=(3, 32, 2, 0, 4),
+    # the plans the ensemble benchmark runs: the cell-pair sweep on the
+    # bfloat16 inner operator for every member (odd nx: one pad column)
+    ens = Ensemble(specs, graph=True, plan_p=(3, 32, 2, 0, 5),
                    plan_tr=(2, 16, 1, 0, 2), inner_p=51, inner_tr=6)
     st = ens.run(2)
     assert st.solves == 3 * 2 * 2 * 3

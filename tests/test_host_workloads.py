@@ -64,6 +64,11 @@ def test_variant2_plans_filtered_by_field_size():
     assert plan_fits(2, (1024, 1024, 256), True)          # exactly 1 GiB
     assert not plan_fits(2, (1024, 1024, 256), False)     # 2 GiB FP64
     assert plan_fits(1, (2048, 2048, 512), False)         # other variants: any size
-    assert plan_fits(3, (512, 512, 256), True)            # cell pairs: even nx
-    assert not plan_fits(3, (511, 512, 256), True)
+    assert plan_fits(3, (512, 512, 256), True)            # cell pairs
+    assert plan_fits(3, (511, 512, 256), True)            # odd nx: one pad column
     assert not plan_fits(3, (1024, 1024, 512), True)      # 2 GiB
+    assert plan_fits(2, (1023, 1024, 256), True)          # padded to exactly 1 GiB
+    assert not plan_fits(2, (1025, 1024, 256), True)      # padded past 1 GiB
+    assert plan_fits(5, (512, 512, 256), True)            # 8-byte records: 512 MiB
+    assert not plan_fits(5, (512, 512, 256), False)       # FP32 sweeps only
+    assert not plan_fits(5, (1024, 512, 512), True)       # 2 GiB of records

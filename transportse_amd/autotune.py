@@ -29,12 +29,9 @@ PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
 # variant 5: the cell-pair sweep on the inner operator stored as bfloat16
 # (faces and storage term; diagonal summed in the kernel)
 PLANS32 = PLANS32 + [(2, 32, 2, 5), (3, 32, 2, 5)]
-# the batched engine runs the same plans (plan_fits keeps variants 3 and 5
-# to batches whose members all have an even nx) and variants 4 (the cell-pair
-# sweep for members with an even nx, the unrolled-ring sweep for the others)
-# and 6 (variant 5 for members with an even nx, the unrolled-ring sweep on
-# the FP32 operator for the others)
-BATCH_PLANS32 = PLANS32 + [(2, 32, 2, 4), (3, 32, 2, 4), (2, 32, 2, 6), (3, 32, 2, 6)]
+# the batched engine runs the same plans (pressure rows are always even: an
+# odd nx gets one inactive column)
+BATCH_PLANS32 = PLANS32
 # transport: the cell's own coefficients stay in registers, one row per thread
 TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
               (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]
@@ -44,15 +41,15 @@ BUF_MAX = 1 << 30        # variants 2 and 3 read fields through buffer descripto
 
 
 def plan_fits(variant: int, shape, fp32: bool) -> bool:
-    """Variants 2 and 3 take fields of at most 1 GiB, variants 3 and 5 (cell
-    pairs) an even nx, variants 5 and 6 FP32 sweeps with 8 bytes of record
-    per cell within 1 GiB; the library rejects the others."""
+    """Pressure plans on a grid of this shape (the natural layout's rows are
+    nx rounded up to even): variants 2 and 3 take fields of at most 1 GiB,
+    variant 5 FP32 sweeps with 8 bytes of record per cell within 1 GiB; the
+    library rejects the others."""
     nx, ny, nz = shape
-    if variant in (3, 5) and nx % 2:
-        return False
-    if variant in (5, 6):
-        return fp32 and nx * ny * nz * 8 <= BUF_MAX
-    return variant < 2 or nx * ny * nz * (4 if fp32 else 8) <= BUF_MAX
+    n = (nx + nx % 2) * ny * nz
+    if variant == 5:
+        return fp32 and n * 8 <= BUF_MAX
+    return variant < 2 or n * (4 if fp32 else 8) <= BUF_MAX
 
 KINDS = {
     "p": ("tse_fast_p_sweeps", PLANS64, PLANS32),

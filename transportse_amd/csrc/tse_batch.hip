@@ -116,37 +116,11 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
             static_assert(sizeof(T) == 4 && RY == 2, "variant 3: FP32, cell pairs");
             rb_zmarch_tile_x2<NS, BYT, false>(m.g, w.y, pin, pout, wx, wy, wz, inv, rhs,
                                               omega, zero);
-        } else if constexpr (VAR == 4) {
-            // variant 4 (batches only): members with an even nx take the
-            // cell-pair sweep, the others the unrolled-ring sweep, on one set
-            // of LDS rings (the branch is uniform per block: one item)
-            static_assert(sizeof(T) == 4 && RY == 2, "variant 4: FP32, 2 rows");
-            __shared__ ZRing<float, NS, BYT * RY> sP, sWy;
-            if (m.nx % 2 == 0)
-                rb_zmarch_tile_x2_in<NS, BYT, false>(m.g, w.y, pin, pout, wx, wy, wz,
-                                                     inv, rhs, omega, zero, sP, sWy);
-            else
-                rb_zmarch_tile_xr_in<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz,
-                                                     inv, rhs, omega, zero, sP, sWy);
-        } else if constexpr (VAR == 5) {
-            // the cell-pair sweep on the bfloat16 inner operator (even nx)
-            static_assert(sizeof(T) == 4 && RY == 2, "variant 5: FP32, cell pairs");
+        } else {
+            // the cell-pair sweep on the bfloat16 inner operator
+            static_assert(sizeof(T) == 4 && RY == 2 && VAR == 5, "variant 5: FP32, cell pairs");
             rb_zmarch_tile_x5<NS, BYT>(m.g, w.y, pin, pout,
                                        (const uint32_t *)m.fop.rec16, rhs, omega, zero);
-        } else {
-            // variant 6 (batches only): variant 5 for members with an even
-            // nx, the unrolled-ring sweep on the FP32 operator for the others,
-            // on one LDS area (the bfloat16 Wy ring is the smaller)
-            static_assert(sizeof(T) == 4 && RY == 2 && VAR == 6, "variant 6: FP32");
-            __shared__ ZRing<float, NS, BYT * RY> sP, sWy;
-            if (m.nx % 2 == 0)
-                rb_zmarch_tile_x5_in<NS, BYT>(m.g, w.y, pin, pout,
-                                              (const uint32_t *)m.fop.rec16, rhs, omega,
-                                              zero, sP,
-                                              *reinterpret_cast<WyRing<NS, BYT> *>(&sWy));
-            else
-                rb_zmarch_tile_xr_in<T, NS, BYT, RY>(m.g, w.y, pin, pout, wx, wy, wz,
-                                                     inv, rhs, omega, zero, sP, sWy);
         }
         __syncthreads();  // LDS reuse by the next tile
     }
@@ -355,8 +329,8 @@ int grow_slots(tse_batch *b, int need) {
 }
 
 ZGeo member_zgeo(const tse_batch *b, const tse_ctx *c) {
-    return make_zgeo(c->geo, b->plan.ns, b->plan.by, b->plan.zchunk,
-                     b->plan.variant >= 2);
+    return make_zgeo(fast_tiling_geo(c->geo, b->kind), b->plan.ns, b->plan.by,
+                     b->plan.zchunk, b->plan.variant >= 2);
 }
 
 // Rebuild work-item lists, member table and the persistent grid size.
@@ -478,12 +452,8 @@ int launch_sweeps(tse_batch *b, int launch, unsigned base, bool size_only) {
         TSE_BP(2, 3, 16, 2)
         TSE_BP(3, 2, 16, 2)
         TSE_BP(3, 3, 16, 2)
-        TSE_BP(4, 2, 16, 2)
-        TSE_BP(4, 3, 16, 2)   // 28 bytes of scratch per lane; still 4.5 % faster than variant 2
         TSE_BP(5, 2, 16, 2)
         TSE_BP(5, 3, 16, 2)
-        TSE_BP(6, 2, 16, 2)
-        TSE_BP(6, 3, 16, 2)
         TSE_BT(0, 3, 16)
         TSE_BT(2, 3, 16)
     }
@@ -571,9 +541,9 @@ BMember member_entry(const tse_batch *b, tse_ctx *c, FastSys *s) {
     const Geo &G = c->geo;
     BMember m{};
     m.g = member_zgeo(b, c);
-    m.dnx = make_fastdiv(G.nx);
+    m.dnx = make_fastdiv(s->nxf);
     m.dny = make_fastdiv(G.ny);
-    m.nx = G.nx;
+    m.nx = s->nxf;   // row length of the natural layout
     m.ny = G.ny;
     m.nz = G.nz;
     m.N = s->N;
@@ -669,14 +639,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
     const int fp32 = prm->form == TSE_FORM_MIXED;
     if (prm->form != TSE_FORM_MIXED && prm->form != TSE_FORM_PLAIN64)
         return fail(TSE_ERR_ARG, "form must be PLAIN64 or MIXED");
-    // variant 4, batches only: variant 3 for members with an even nx, 2 for the others
     const tse_sweep_plan &pl = prm->plan;
-    const bool mixed_pairs = fp32 && kind == SYS_P && pl.variant == 4 && pl.by == 32 &&
-                             pl.ry == 2 && (pl.ns == 2 || pl.ns == 3);
-    // variant 6, batches only: variant 5 for members with an even nx, 2 for the others
-    const bool mixed_bf16 = fp32 && kind == SYS_P && pl.variant == 6 && pl.by == 32 &&
-                            pl.ry == 2 && (pl.ns == 2 || pl.ns == 3);
-    if (!(fast_plan_ok(pl, fp32, kind) || mixed_pairs || mixed_bf16) || prm->inner < pl.ns ||
+    if (!fast_plan_ok(pl, fp32, kind) || prm->inner < pl.ns ||
         prm->inner % pl.ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
     for (int s = 0; s < b->cap; ++s)
@@ -703,9 +667,9 @@ int tse_batch_add(tse_batch *b, tse_ctx *c) {
     FastSys *s = member_sys(b, c);
     if (!s || !s->ready) return fail(TSE_ERR_STATE, "member's fast system not set up");
     if (!fast_plan_fits(b->plan, s->N, b->form == TSE_FORM_MIXED))
-        return fail(TSE_ERR_ARG, "sweep variants 2-6 need fields of at most 1 GiB "
-                                 "(variants 5 and 6: 8 bytes per cell)");
-    if (!fast_plan_grid_ok(b->plan, c->geo.nx))
+        return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
+                                 "(variant 5: 8 bytes per cell)");
+    if (!fast_plan_grid_ok(b->plan, s->nxf))
         return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
     for (int q = 0; q < b->cap; ++q)
         if (b->hused[q] && b->ctx[q] == c)

@@ -1436,7 +1436,12 @@ enum SysKind { SYS_P = 0, SYS_TR = 1 };
 //   transport: buf64 = c[6] D invD b | p0 p1          buf32 = c[6] invD r | e0 e1
 struct FastSys {
     int kind = SYS_P;
-    long N = 0;
+    // row length of the natural layout: nx for transport, nx rounded up to
+    // even for pressure (an odd nx gets one inactive column: zero faces,
+    // 1/diagonal 0, storage term +inf, iterate 0), so that every pressure
+    // system can run the cell-pair sweeps
+    int nxf = 0;
+    long N = 0;                  // nxf * ny * nz
     long ld = 0;          // element distance between the carved arrays
     double *buf64 = nullptr;
     float *buf32 = nullptr;
@@ -1483,6 +1488,15 @@ struct FastSys {
 
 FastSys *system_of(tse_ctx *ctx, int kind);
 int ensure_sys(tse_ctx *ctx, FastSys *s);
+// Row length of kind's natural layout on grid G (see FastSys::nxf).
+inline int fast_nx(const Geo &G, int kind) { return kind == SYS_P ? (G.nx + 1) & ~1 : G.nx; }
+// G with the natural layout's row length, for the sweep and residual tilings
+// (which read nx, ny, nz only; never pass it to a split-layout kernel).
+inline Geo fast_tiling_geo(const Geo &G, int kind) {
+    Geo F = G;
+    F.nx = fast_nx(G, kind);
+    return F;
+}
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind);
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32);
 bool fast_plan_grid_ok(const tse_sweep_plan &pl, int nx);

@@ -142,21 +142,31 @@ __device__ inline double face_G(const Geo &G, const double *rho, int i, int j,
            (G.dz[k + 1] + G.dz[k]) * G.cosb;
 }
 
+// nxf: row length of the natural layout (nx or nx + 1); the cells of a pad
+// column (i = nx) are written inactive.
 __global__ void __launch_bounds__(kBlock)
-fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny,
+fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny, int nxf,
                      const double *__restrict__ Pn, const double *__restrict__ c_f,
                      const double *__restrict__ nue, const double *__restrict__ rho,
                      const double *__restrict__ phi, const double *__restrict__ Kx,
                      const double *__restrict__ Ky, const double *__restrict__ Kz,
                      const double *__restrict__ BP, const double *__restrict__ WP,
                      const double *__restrict__ c_r, double dt, FastOp op) {
-    const long N = (long)G.nx * G.ny * G.nz;
+    const long N = (long)nxf * G.ny * G.nz;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
         const int row = (int)fdiv((uint32_t)t, dnx);
-        const int i = (int)t - row * G.nx;
+        const int i = (int)t - row * nxf;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * G.ny;
+        if (i >= G.nx) {   // pad column
+            op.Wx[t] = op.Wy[t] = op.Wz[t] = op.D[t] = op.invD[t] = op.b[t] = 0.0;
+            op.Wx32[t] = op.Wy32[t] = op.Wz32[t] = op.invD32[t] = 0.f;
+            uint16_t *q = op.rec16 + 8 * (t >> 1) + (t & 1);
+            q[0] = q[2] = q[4] = 0;
+            q[6] = kBf16Inf;
+            continue;
+        }
         const int c = slot(G, i, j, k);
         double W[6] = {0, 0, 0, 0, 0, 0};  // x+, x-, y+, y-, z+, z-
         double Gx = 0, Gxm = 0, Gz = 0, Gzm = 0;
@@ -209,31 +219,33 @@ fast_assemble_kernel(Geo G, FastDiv dnx, FastDiv dny,
 
 // ---- layout conversion split <-> natural ------------------------------------------------------
 
+// Natural layout with row length nxf (nx or nx + 1): a pad column reads 0
+// and is not written back.
 __global__ void __launch_bounds__(kBlock)
-split_to_nat_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ src,
-                    double *__restrict__ dst) {
-    const long N = (long)G.nx * G.ny * G.nz;
+split_to_nat_kernel(Geo G, FastDiv dnx, FastDiv dny, int nxf,
+                    const double *__restrict__ src, double *__restrict__ dst) {
+    const long N = (long)nxf * G.ny * G.nz;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
         const int row = (int)fdiv((uint32_t)t, dnx);
-        const int i = (int)t - row * G.nx;
+        const int i = (int)t - row * nxf;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * G.ny;
-        dst[t] = src[slot(G, i, j, k)];
+        dst[t] = i < G.nx ? src[slot(G, i, j, k)] : 0.0;
     }
 }
 
 __global__ void __launch_bounds__(kBlock)
-nat_to_split_kernel(Geo G, FastDiv dnx, FastDiv dny, const double *__restrict__ src,
-                    double *__restrict__ dst) {
-    const long N = (long)G.nx * G.ny * G.nz;
+nat_to_split_kernel(Geo G, FastDiv dnx, FastDiv dny, int nxf,
+                    const double *__restrict__ src, double *__restrict__ dst) {
+    const long N = (long)nxf * G.ny * G.nz;
     for (long t = blockIdx.x * (long)blockDim.x + threadIdx.x; t < N;
          t += (long)gridDim.x * blockDim.x) {
         const int row = (int)fdiv((uint32_t)t, dnx);
-        const int i = (int)t - row * G.nx;
+        const int i = (int)t - row * nxf;
         const int k = (int)fdiv((uint32_t)row, dny);
         const int j = row - k * G.ny;
-        dst[slot(G, i, j, k)] = src[t];
+        if (i < G.nx) dst[slot(G, i, j, k)] = src[t];
     }
 }
 
@@ -433,14 +445,15 @@ void fast_free(tse_ctx *ctx) {
 
 int ensure_sys(tse_ctx *ctx, FastSys *s) {
     const Geo &G = ctx->geo;
-    const long N = (long)G.nx * G.ny * G.nz;
+    const int nxf = fast_nx(G, s->kind);
+    const long N = (long)nxf * G.ny * G.nz;
     if (!s->partials) {
         if (hipMalloc((void **)&s->partials, sizeof(double) * 2 * 4096) != hipSuccess ||
             hipMalloc((void **)&s->red, sizeof(double) * 2) != hipSuccess ||
             hipHostMalloc((void **)&s->red_host, sizeof(double) * 2) != hipSuccess)
             return fail(TSE_ERR_NOMEM, "fast-form scratch allocation failed");
     }
-    if (s->N == N && s->buf64 && s->buf32) return 0;
+    if (s->N == N && s->nxf == nxf && s->buf64 && s->buf32) return 0;
     if (s->buf64 || s->buf32) {
         TSE_HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (s->buf64) TSE_HIP_TRY(hipFree(s->buf64));
@@ -451,6 +464,7 @@ int ensure_sys(tse_ctx *ctx, FastSys *s) {
     s->buf64 = nullptr;
     s->buf32 = nullptr;
     s->N = 0;
+    s->nxf = 0;
     s->ld = 0;
     s->ready = false;
     // the arrays are 1088 elements apart beyond a multiple of 64 (FP32:
@@ -474,6 +488,7 @@ int ensure_sys(tse_ctx *ctx, FastSys *s) {
     s->buf32 = (float *)b32;
     s->ld = ld;
     s->N = N;
+    s->nxf = nxf;
     return 0;
 }
 
@@ -486,7 +501,7 @@ static int grid_n(long n) {
 template <int MODE>
 static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
                     double *pout, double *rn, double *bn) {
-    const Geo &G = ctx->geo;
+    const Geo G = fast_tiling_geo(ctx->geo, s->kind);
     const RzGeo rg = make_rzgeo(G);
     const int nb = std::max(1, std::min(rg.ntiles, 4096));
     const FastOp pop = s->kind == SYS_P ? s->pop() : FastOp{};
@@ -601,8 +616,7 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan
 }
 
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
-    // variant 5 (and 6, a batch's variant 5 for even nx): 8-byte records per cell
-    if (pl.variant == 5 || pl.variant == 6) return N * 8L <= (long)kBufMax;
+    if (pl.variant == 5) return N * 8L <= (long)kBufMax;   // 8-byte records per cell
     return pl.variant < 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
 }
 
@@ -712,7 +726,7 @@ static int load_iterate(tse_ctx *ctx, FastSys *s, const double *X_split) {
     const Geo &G = ctx->geo;
     s->cur = 0;
     hipLaunchKernelGGL(split_to_nat_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       ctx->stream, G, make_fastdiv(s->nxf), make_fastdiv(G.ny), s->nxf,
                        X_split, s->p(0));
     TSE_HIP_TRY(hipGetLastError());
     s->ready = true;
@@ -723,7 +737,7 @@ static int sys_result(tse_ctx *ctx, FastSys *s, double *X_split) {
     if (!s->ready || !X_split) return fail(TSE_ERR_STATE, "fast form not set up");
     const Geo &G = ctx->geo;
     hipLaunchKernelGGL(nat_to_split_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny),
+                       ctx->stream, G, make_fastdiv(s->nxf), make_fastdiv(G.ny), s->nxf,
                        s->p(s->cur), X_split);
     TSE_HIP_TRY(hipGetLastError());
     return 0;
@@ -746,10 +760,10 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     if (!fast_plan_fits(*plan, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
                                  "(variant 5: 8 bytes per cell)");
-    if (!fast_plan_grid_ok(*plan, ctx->geo.nx))
+    if (!fast_plan_grid_ok(*plan, s->nxf))
         return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
-    const ZGeo zg = make_zgeo(ctx->geo, plan->ns, plan->by, plan->zchunk,
-                              plan->variant >= 2);
+    const ZGeo zg = make_zgeo(fast_tiling_geo(ctx->geo, s->kind), plan->ns, plan->by,
+                              plan->zchunk, plan->variant >= 2);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
     int e;  // FP32 timing path: correction sweeps on the last residual
     return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
@@ -772,9 +786,10 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (!fast_plan_fits(pl, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
                                  "(variant 5: 8 bytes per cell)");
-    if (!fast_plan_grid_ok(pl, ctx->geo.nx))
+    if (!fast_plan_grid_ok(pl, s->nxf))
         return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
-    const ZGeo zg = make_zgeo(ctx->geo, pl.ns, pl.by, pl.zchunk, pl.variant >= 2);
+    const ZGeo zg = make_zgeo(fast_tiling_geo(ctx->geo, s->kind), pl.ns, pl.by, pl.zchunk,
+                              pl.variant >= 2);
     double rn, bn;
     int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
                   : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
@@ -850,7 +865,7 @@ int tse_fast_p_setup(tse_ctx *ctx, const tse_p_fields *f, double dt) {
     const Geo &G = ctx->geo;
     const double *Pn = f->Pn ? f->Pn : f->P;
     hipLaunchKernelGGL(fast_assemble_kernel, dim3(grid_n(s->N)), dim3(kBlock), 0,
-                       ctx->stream, G, make_fastdiv(G.nx), make_fastdiv(G.ny), Pn,
+                       ctx->stream, G, make_fastdiv(s->nxf), make_fastdiv(G.ny), s->nxf, Pn,
                        f->c_f, f->nue, f->rho_f, f->phi, f->Kx, f->Ky, f->Kz,
                        f->BP, f->WP, f->c_r, dt, s->pop());
     TSE_HIP_TRY(hipGetLastError());
@@ -1009,13 +1024,13 @@ namespace tse {
 static tse_sweep_plan default_fast_plan(const tse_ctx *ctx, int kind, int fp32,
                                         bool explicit_step) {
     const Geo &G = ctx->geo;
-    const long N = (long)G.nx * G.ny * G.nz;
+    const long N = (long)fast_nx(G, kind) * G.ny * G.nz;
     tse_sweep_plan pl{explicit_step ? 1 : 2, 16, 1, 0, 2};
     if (kind == SYS_P && fp32) {
-        // the cell-pair sweep on the bfloat16 inner operator where nx is
-        // even and its records fit, else the unrolled-ring sweep
+        // the cell-pair sweep on the bfloat16 inner operator (pressure rows
+        // are even) where its records fit, else the unrolled-ring sweep
         pl = tse_sweep_plan{3, 32, 2, 0, 5};
-        if (!fast_plan_grid_ok(pl, G.nx) || !fast_plan_fits(pl, N, fp32)) pl.variant = 2;
+        if (!fast_plan_fits(pl, N, fp32)) pl.variant = 2;
     }
     if (!fast_plan_fits(pl, N, fp32)) {  // fields over 1 GiB: no buffer loads
         pl = tse_sweep_plan{explicit_step ? 1 : 2, 16, 1, 0, 0};

@@ -122,14 +122,10 @@ class Server:
                  inner_tr=6, sor_p=1.95, sor_tr=1.0, device=0,
                  max_sweeps=400000, keep_results=False):
         """plan_p None: the cell-pair sweep on the bfloat16 inner operator
-        (variant 5) when every request of a run has an even nx, else that
-        sweep for the even-nx requests and the unrolled-ring sweep for the
-        others (variant 6)."""
+        (variant 5)."""
         import torch
-        self.auto_plan_p = plan_p is None
         if plan_p is None:
-            plan_p = (3, 32, 2, 64, 2)
-        self.p_args = (form, tol, inner_p, sor_p, max_sweeps)
+            plan_p = (3, 32, 2, 64, 5)
         self.budget = float(budget_bytes)
         self.device = device
         self.stream = torch.cuda.Stream(device)   # default priority
@@ -145,13 +141,6 @@ class Server:
         time_scale); returns latency / throughput statistics."""
         import torch
         reqs = sorted(requests, key=lambda r: r.arrival)
-        if self.auto_plan_p:
-            even = all(r.spec.nx % 2 == 0 for r in reqs)
-            plan = (3, 32, 2, 64, 5 if even else 6)
-            bp = self.batches[KIND_P]
-            if bp.plan != plan and not bp.members:
-                form, tol, inner, sor, max_sweeps = self.p_args
-                bp.configure(form, plan, inner, tol, sor, max_sweeps)
         waiting: deque = deque()
         live = {}            # rid -> [request, scenario, generator, kind, slots, need]
         shapes = {}          # grid shape -> live requests of that shape
