@@ -110,6 +110,8 @@ class Batch:
                             r.initial_rel, r.rel_residual)
 
     def close(self):
+        """Hand every member's iterate back to its context, then free the
+        batch."""
         if self.h:
             for s in list(self.members):
                 self.remove(s)
@@ -117,7 +119,12 @@ class Batch:
             self.h = None
 
     def __del__(self):
+        # only the batch's own buffers: a finalizer may run after a member's
+        # context was destroyed (garbage cycles are finalized in any order),
+        # so no write-back into the members here
         try:
-            self.close()
+            if self.h:
+                self.lib.tse_batch_destroy(self.h)
+                self.h = None
         except Exception:
             pass
