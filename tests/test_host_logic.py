@@ -123,3 +123,56 @@ def test_vectorised_eos_falls_back_when_cells_disagree(tmp_path, oracle_mod):
     assert sim._eos_vectorized is False
     expect = 1000.0 + 700.0 * sim.st.C[..., 0]
     assert np.array_equal(sim.st.rho_f, expect)
+
+
+def test_slabbed_eos_equals_the_per_cell_loop(tmp_path, oracle_mod, monkeypatch):
+    """The whole-field EOS call split into x-slabs on threads (forced here on
+    a small grid) gives the per-cell loop's fields bit for bit, species-last
+    C included, and a callback that differs per cell still falls back."""
+    import numpy as np
+
+    from helpers import GOLDEN, oracle_backend
+    from transportse_amd import driver
+    from transportse_amd.driver import Simulation
+    monkeypatch.setattr(driver, "_EOS_MIN_CELLS", 1)
+    sim = Simulation(f"{GOLDEN}/configs/g1_salt.py", oracle_backend,
+                     verbose=False, out_dir=str(tmp_path), max_steps=1)
+    assert len(sim._eos_slabs()) == min(driver._EOS_THREADS, sim.st.nx) > 1
+    c = sim.config
+    rng = np.random.default_rng(3)
+    st = sim.st
+    st.C[...] = rng.random(st.C.shape)
+    st.T[...] = 10.0 + 5.0 * rng.random(st.T.shape)
+
+    def fp(C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+           D_fx, D_fy, D_fz, rho_r):
+        rho_f = 1000.0 + 700.0 * C[0] - 0.2 * (T - 4.0) ** 2
+        nue = 1.0e-3 * np.exp(-0.02 * T) * (1.0 + 0.5 * C[0])
+        C = C * 0.999
+        return (C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+                D_fx, D_fy, D_fz, rho_r)
+    c.fluid_properties = fp
+    names = ("C", "T", "rho_f", "nue", "l_f")
+    before = {n: getattr(st, n).copy() for n in names}
+    sim._eos_loop()
+    loop = {n: getattr(st, n).copy() for n in names}
+    for n in names:
+        getattr(st, n)[...] = before[n]
+    sim._eos_vectorized = None
+    changed = sim.update_fluid_properties()
+    assert sim._eos_vectorized is True
+    assert any(a is st.rho_f for a in changed) and any(a is st.C for a in changed)
+    assert not any(a is st.T for a in changed)
+    for n in names:
+        assert np.array_equal(getattr(st, n), loop[n]), n
+
+    def fp_mean(C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky,
+                Kz, D_fx, D_fy, D_fz, rho_r):
+        rho_f = 1000.0 + 700.0 * np.mean(C[0])
+        return (C, T, P, nue, rho_f, c_f, cp_f, l_f, cp_r, l_r, phi, Kx, Ky, Kz,
+                D_fx, D_fy, D_fz, rho_r)
+    c.fluid_properties = fp_mean
+    sim._eos_vectorized = None
+    sim.update_fluid_properties()
+    assert sim._eos_vectorized is False
+    assert np.array_equal(st.rho_f, 1000.0 + 700.0 * st.C[..., 0])

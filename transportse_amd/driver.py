@@ -31,7 +31,7 @@ import types
 import numpy
 
 from .config_loader import load_config
-from .device import pcopy, pcopyto
+from .device import pcopy
 from .snapshot import PickleWriter
 
 # config attributes the reference binds at start-up (tse.py:81-151)
@@ -42,6 +42,20 @@ _STATE_ARRAYS = ("phi", "Kx", "Ky", "Kz", "c_r", "cp_r", "l_r", "rho_r",
 
 _FP_NAMES = ("C", "T", "P", "nue", "rho_f", "c_f", "cp_f", "l_f", "cp_r",
              "l_r", "phi", "Kx", "Ky", "Kz", "D_fx", "D_fy", "D_fz", "rho_r")
+
+
+# threads of the whole-field EOS call (numpy releases the GIL in its loops)
+_EOS_THREADS = 8
+_EOS_MIN_CELLS = 1 << 19     # cells per slab at least
+_EOS_POOL = None
+
+
+def _eos_pool():
+    global _EOS_POOL
+    if _EOS_POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _EOS_POOL = ThreadPoolExecutor(max_workers=_EOS_THREADS)
+    return _EOS_POOL
 
 
 class ConvergenceError(SystemExit):
@@ -130,25 +144,55 @@ class Simulation:
                     for a, v in zip(arrs, out):
                         a[i, j, k] = v
 
+    def _eos_slabs(self):
+        """x-slabs of the whole-field EOS call: up to _EOS_THREADS, each of
+        at least _EOS_MIN_CELLS cells (one slab for small grids)."""
+        st = self.st
+        cells = st.nx * st.ny * st.nz
+        k = max(1, min(_EOS_THREADS, st.nx, cells // _EOS_MIN_CELLS))
+        step = -(-st.nx // k)
+        return [(x0, min(x0 + step, st.nx)) for x0 in range(0, st.nx, step)]
+
     def _eos_vector(self):
-        """Call fluid_properties once on whole fields (species axis first)."""
+        """Call fluid_properties on whole fields (species axis first), one
+        call per x-slab on the EOS threads.  The reference calls it cell by
+        cell inside prange loops (tse.py:181-196), so the cells of a call
+        are independent of each other and of the other slabs' calls; the
+        sample check holds the assembled result to that contract.  Returns
+        per output a list of (x0, x1, slab values)."""
         arrs = self._fp_arrays()
-        args = []
-        for a in arrs:
-            v = (numpy.moveaxis(a, -1, 0) if a.ndim == 4 else a).view()
-            # read-only: an argument returned unchanged is known to be
-            # unchanged (an in-place write raises and selects the loop)
-            v.flags.writeable = False
-            args.append(v)
-        out = self.config.fluid_properties(*args)
-        self._eos_same = [v is arg for v, arg in zip(out, args)]
-        res = []
-        for a, v in zip(arrs, out):
-            v = numpy.asarray(v, dtype=numpy.float64)
-            if a.ndim == 4 and v.ndim == 4:
-                v = numpy.moveaxis(v, 0, -1)  # back to species-last
-            res.append(numpy.broadcast_to(v, a.shape))
-        return res
+        fp = self.config.fluid_properties
+        slabs = self._eos_slabs()
+
+        def call(x0, x1):
+            args = []
+            for a in arrs:
+                v = (numpy.moveaxis(a, -1, 0)[:, x0:x1] if a.ndim == 4
+                     else a[x0:x1]).view()
+                # read-only: an argument returned unchanged is known to be
+                # unchanged (an in-place write raises and selects the loop)
+                v.flags.writeable = False
+                args.append(v)
+            out = fp(*args)
+            same = [v is arg for v, arg in zip(out, args)]
+            vals = []
+            for a, v in zip(arrs, out):
+                v = numpy.asarray(v, dtype=numpy.float64)
+                if a.ndim == 4 and v.ndim == 4:
+                    v = numpy.moveaxis(v, 0, -1)  # back to species-last
+                vals.append(numpy.broadcast_to(v, a[x0:x1].shape))
+            return same, vals
+
+        if len(slabs) == 1:
+            parts = [call(*slabs[0])]
+        else:
+            futs = [_eos_pool().submit(call, x0, x1) for x0, x1 in slabs]
+            parts = [f.result() for f in futs]
+        if any(len(p[1]) != len(arrs) for p in parts):
+            raise ValueError("fluid_properties returned the wrong number of values")
+        self._eos_same = [all(p[0][n] for p in parts) for n in range(len(arrs))]
+        return [[(x0, x1, p[1][n]) for (x0, x1), p in zip(slabs, parts)]
+                for n in range(len(arrs))]
 
     def update_fluid_properties(self):
         """Returns the state arrays whose values may have changed."""
@@ -168,10 +212,17 @@ class Simulation:
             return self._fp_arrays()
         self._eos_vectorized = True
         changed = []
-        for a, v, same in zip(self._fp_arrays(), res, self._eos_same):
+        jobs = []
+        for a, parts, same in zip(self._fp_arrays(), res, self._eos_same):
             if not same:       # the callback returned its argument unchanged
-                pcopyto(a, v)
+                jobs += [(a[x0:x1], v) for x0, x1, v in parts]
                 changed.append(a)
+        if len(jobs) > 1:
+            for f in [_eos_pool().submit(numpy.copyto, d, v) for d, v in jobs]:
+                f.result()
+        else:
+            for d, v in jobs:
+                numpy.copyto(d, v)
         return changed
 
     def _validate_eos(self, res, first=True) -> bool:
@@ -203,9 +254,11 @@ class Simulation:
         fp = self.config.fluid_properties
         refs = [fp(*[a[i, j, k] for a in arrs]) for i, j, k in cells.tolist()]
         I, J, K = cells.T
-        for n, v in enumerate(res):
-            got = numpy.ascontiguousarray(numpy.asarray(v)[I, J, K],
-                                          dtype=numpy.float64)
+        for n, parts in enumerate(res):
+            got = numpy.empty((len(I),) + parts[0][2].shape[3:], dtype=numpy.float64)
+            for x0, x1, v in parts:
+                sel = (I >= x0) & (I < x1)
+                got[sel] = v[I[sel] - x0, J[sel], K[sel]]
             want = numpy.array([numpy.asarray(r[n], dtype=numpy.float64)
                                 for r in refs], dtype=numpy.float64)
             if got.shape != want.shape or not numpy.array_equal(
