@@ -416,10 +416,15 @@ __device__ inline void pair_sweep(const Geo &G, int colour, double *__restrict__
     for (int r0 = 2 * (blockIdx.x * blockDim.x + threadIdx.x); r0 < G.S;
          r0 += 2 * gridDim.x * blockDim.x) {
         const long ri = rbase + r0;  // even: the pair is 16-byte aligned
+        // the record is read once per half-sweep: non-temporal loads leave
+        // the L2 to the other colour's values, which the blocks of the
+        // neighbouring rows and planes read again (256x256x128 compute_P
+        // half-sweep 99.3 -> 90.5 us)
         dpair R[NREC];
 #pragma unroll
         for (int f = 0; f < NREC; ++f)
-            R[f] = *reinterpret_cast<const dpair *>(rec + f * G.Fr + ri);
+            R[f] = __builtin_nontemporal_load(
+                reinterpret_cast<const dpair *>(rec + f * G.Fr + ri));
         const uint16_t mm = *reinterpret_cast<const uint16_t *>(mask + ri);
         // r0 = S-1 (S odd): the second slot is the pad slot, mask 0
         const int r1 = r0 + 1 < G.S ? r0 + 1 : r0;
