@@ -68,14 +68,15 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 # variant 5 stores the three faces and the storage term as bfloat16 (the
-# diagonal is summed in the kernel): 8 + 4 + 4 + 4 bytes
-BYTES_FP32_SWEEP_BY_VARIANT = {5: 20}
+# diagonal is summed in the kernel): 8 + 4 + 4 + 4 bytes; variant 7 the rhs
+# as well: 8 + 2 + 4 + 4 bytes
+BYTES_FP32_SWEEP_BY_VARIANT = {5: 20, 7: 18}
 BYTES_FP64_SWEEP = 56
 # sweep kernel (name as rocprofv3 reports it, without template arguments)
 # per plan variant
 VARIANT_KERNEL = {0: "rb_zmarch_kernel", 1: "rb_zmarch_xl_kernel",
                   2: "rb_zmarch_xr_kernel", 3: "rb_zmarch_x2_kernel",
-                  5: "rb_zmarch_x5_kernel"}
+                  5: "rb_zmarch_x5_kernel", 7: "rb_zmarch_x5_kernel"}
 
 
 def log(*a):

@@ -227,7 +227,7 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  * of a wavefront; smaller LDS ring), 2 (as 1, with the register rings
  * rotated by unrolling and buffer loads; fields of at most 1 GiB, else
  * TSE_ERR_ARG), 3 (FP32 only, as 2 with an x-pair of cells per thread, so
- * that every lane updates in every phase; pressure rows are even) or 5
+ * that every lane updates in every phase; pressure rows are even), 5 or 7
  * (below).  Supported (ns, by, ry):
  *   variant 0: (1,8,1) (1,16,1) (2,16,1) (1,24,2); FP32 sweeps also
  *              (1,32,2) (2,32,2) (2,32,4) (3,16,1)
@@ -243,6 +243,11 @@ double tse_host_l2_norm_seq(const double *New, const double *Old, int64_t n,
  *              factor 1 +- 2^-8 of the FP64 one, so the defect correction
  *              keeps its stop test and sweep count; 20 instead of 28 bytes
  *              per cell per launch; at most 2^27 cells
+ *   variant 7: (2,32,2) (3,32,2), single FP32 pressure systems only (a
+ *              batch rejects it): variant 5 with the correction's rhs
+ *              stored as bfloat16 too, rounded from its FP32 value with a
+ *              dither fixed by the cell index; 18 bytes per cell per
+ *              launch, but cycles that reduce the residual somewhat less
  * Transport systems: variant 0 or 2, (1,8,1) (1,16,1) (2,16,1), FP32
  * (3,16,1). */
 typedef struct {

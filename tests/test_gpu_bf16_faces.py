@@ -10,7 +10,11 @@ diagonal is summed in the kernel from the rounded values; no validity mask).
    FP32-rounded FP64 residual.  The same cycle of variant 3 equals the sweeps
    on the FP32-rounded operator (control), and the two models differ by far
    more than the tolerance, so the check sees which operator the kernel
-   used.  Tolerance: 2e-5 of max|e|.
+   used.  Tolerance: 2e-5 of max|e|.  Variant 7 (the rhs stored as
+   bfloat16 too, rounded stochastically from its FP32 value with a dither
+   fixed by the cell index) equals the sweeps on the same operator with the
+   rhs rounded that way, and differs from variant 5's model by far more than
+   the tolerance.
 2. Solves to relative residual 1e-10 reach the oracle's converged compute_P
    fixed point within the error that residual bounds (1e-6 of the change),
    and agree with variant 3's solves (sweep counts within 10 %), with fixed
@@ -73,6 +77,28 @@ def _bf16(w):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def _lowbias32(x):
+    """The lowbias32 integer mix (C. Wellons) on uint32 arrays."""
+    x = x.astype(np.uint32)
+    x ^= x >> np.uint32(16)
+    x *= np.uint32(0x7FEB352D)
+    x ^= x >> np.uint32(15)
+    x *= np.uint32(0x846CA68B)
+    x ^= x >> np.uint32(16)
+    return x
+
+
+def _bf16_dithered(r):
+    """Variant 7's rhs: the FP32 value plus a dither of hash(t) mod 2^16 in
+    its 16 low bits, truncated to bfloat16; t is the natural-layout index
+    (x fastest) of the cell.  r is indexed [i, j, k]."""
+    nx, ny, nz = r.shape
+    t = np.arange(nx * ny * nz, dtype=np.uint64).reshape(nz, ny, nx).transpose(2, 1, 0)
+    d = _lowbias32(t.astype(np.uint32)) & np.uint32(0xFFFF)
+    u = r.astype(np.float32).view(np.uint32) + d
+    return (u >> np.uint32(16) << np.uint32(16)).view(np.float32).astype(np.float64)
+
+
 def _faces(Wx, Wy, Wz):
     """The six faces of every cell: x+, x-, y+, y-, z+, z-."""
     z = np.zeros_like
@@ -115,7 +141,7 @@ def test_one_cycle_equals_sor_on_the_rounded_operator(shape, ns, zchunk):
     f = Fields(*shape, seed=31, dirichlet_p=shape[0] > 2)
     sor, n = 1.7, 4 * ns
     fields = {}
-    for var in (3, 5):
+    for var in (3, 5, 7):
         hip, d = _setup(f, 3600.0)
         Wx, Wy, Wz, D, b = _operator(hip, shape)
         prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(ns, 32, 2, zchunk, var),
@@ -140,19 +166,26 @@ def test_one_cycle_equals_sor_on_the_rounded_operator(shape, ns, zchunk):
     inv5 = np.where(active & (D5 > 0), 1.0 / D5, 0.0)
     e32 = _model_sweeps(W6_32, inv32, r32, n, sor, shape)
     e16 = _model_sweeps(W6_16, inv5, r32, n, sor, shape)
+    # variant 7: the rhs rounded to bfloat16 from its FP32 value (dithered)
+    e7 = _model_sweeps(W6_16, inv5, _bf16_dithered(r32), n, sor, shape)
     scale = np.max(np.abs(e32))
     d3 = np.max(np.abs(fields[3] - e32)) / scale
     d5 = np.max(np.abs(fields[5] - e16)) / scale
+    d7 = np.max(np.abs(fields[7] - e7)) / scale
     gap = np.max(np.abs(e16 - e32)) / scale
+    gap7 = np.max(np.abs(e7 - e16)) / scale
     print("one cycle", shape, (ns, zchunk), "variant 3 vs FP32 model", d3,
-          "variant 5 vs bf16 model", d5, "models apart", gap, flush=True)
+          "variant 5 vs bf16 model", d5, "variant 7 vs bf16-rhs model", d7,
+          "models apart", gap, gap7, flush=True)
     assert scale > 0
-    assert d3 <= 2e-5 and d5 <= 2e-5
+    assert d3 <= 2e-5 and d5 <= 2e-5 and d7 <= 2e-5
     if shape[0] * shape[1] * shape[2] > 1000:
         assert gap > 10 * max(d5, 1e-7)
+        assert gap7 > 10 * max(d7, 1e-7)
 
 
-def test_solve_reaches_oracle_fixed_point():
+@pytest.mark.parametrize("variant", [5, 7])
+def test_solve_reaches_oracle_fixed_point(variant):
     from oracle import Oracle
     from transportse_amd import _lib
     f = Fields(40, 30, 24, seed=22)
@@ -164,7 +197,7 @@ def test_solve_reaches_oracle_fixed_point():
                     inc_it=1.0, max_dt=dt, P_conv=1e-15)
     assert r[4]
     hip, d = _setup(f, dt)
-    prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, 5), inner=18,
+    prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, variant), inner=18,
                           max_sweeps=200000, sor=1.9, tol=1e-10)
     res = _lib.FastResult()
     _lib.check(hip.lib.tse_fast_p_solve(hip.ctx, ctypes.byref(prm),
@@ -173,7 +206,7 @@ def test_solve_reaches_oracle_fixed_point():
     Ph = _result(hip, d)
     change = np.max(np.abs(Po - f.P))
     err = np.max(np.abs(Ph - Po))
-    print("variant 5 solve: sweeps", res.sweeps, "rel", res.rel_residual,
+    print(f"variant {variant} solve: sweeps", res.sweeps, "rel", res.rel_residual,
           "err/change", err / change, flush=True)
     assert err <= 1e-6 * change
 
@@ -181,13 +214,13 @@ def test_solve_reaches_oracle_fixed_point():
 @pytest.mark.parametrize("shape", GRIDS)
 @pytest.mark.parametrize("mode", [0, 1])
 def test_solves_agree_with_the_fp32_operator(shape, mode):
-    """Fixed and adaptive cycles: variant 5 converges to the same tolerance
-    with about the same sweep count as variant 3, and the solutions agree
-    within what the residual bounds."""
+    """Fixed and adaptive cycles: variants 5 and 7 converge to the same
+    tolerance with about the same sweep count as variant 3, and the solutions
+    agree within what the residual bounds."""
     from transportse_amd import _lib
     f = Fields(*shape, seed=32, dirichlet_p=shape[0] > 2)
     out = {}
-    for var in (3, 5):
+    for var in (3, 5, 7):
         hip, d = _setup(f, 3600.0)
         prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, var), inner=18,
                               max_sweeps=60000, cycle_mode=mode, sor=1.9, tol=1e-10)
@@ -196,19 +229,21 @@ def test_solves_agree_with_the_fp32_operator(shape, mode):
                                             ctypes.byref(res)), "solve")
         assert res.converged and res.rel_residual <= 1e-10
         out[var] = (_result(hip, d), res.sweeps, res.cycles)
-    (a, na, ca), (b, nb, cb) = out[3], out[5]
+    a, na, ca = out[3]
     change = np.max(np.abs(a - f.P))
-    print("variant 5 vs 3", shape, "mode", mode, "sweeps", na, nb, "cycles", ca, cb,
-          "diff/change", np.max(np.abs(a - b)) / change, flush=True)
-    assert nb <= 1.1 * na + 36
-    assert np.max(np.abs(a - b)) <= 1e-6 * change
+    for var in (5, 7):
+        b, nb, cb = out[var]
+        print(f"variant {var} vs 3", shape, "mode", mode, "sweeps", na, nb, "cycles",
+              ca, cb, "diff/change", np.max(np.abs(a - b)) / change, flush=True)
+        assert nb <= 1.1 * na + 36
+        assert np.max(np.abs(a - b)) <= 1e-6 * change
 
 
 @pytest.mark.parametrize("shape", [(83, 41, 23), (33, 20, 17), (1, 20, 30)])
 def test_odd_nx_and_limits(shape):
-    """An odd nx runs variant 5 on the padded layout (one inactive column)
-    and reaches the oracle's converged fixed point like variant 3; FP64
-    sweeps have no variant 5."""
+    """An odd nx runs variants 5 and 7 on the padded layout (one inactive
+    column) and reaches the oracle's converged fixed point like variant 3;
+    FP64 sweeps have no variant 5."""
     from oracle import Oracle
     from transportse_amd import _lib
     f = Fields(*shape, seed=27, dirichlet_p=shape[0] > 2)
@@ -220,7 +255,7 @@ def test_odd_nx_and_limits(shape):
                     inc_it=1.0, max_dt=dt, P_conv=1e-15)
     assert r[4]
     change = np.max(np.abs(Po - f.P))
-    for var in (3, 5):
+    for var in (3, 5, 7):
         hip, d = _setup(f, dt)
         prm = _lib.FastParams(form=2, plan=_lib.SweepPlan(3, 32, 2, 0, var), inner=18,
                               max_sweeps=400000, sor=1.9, tol=1e-10)

@@ -72,3 +72,15 @@ def test_variant2_plans_filtered_by_field_size():
     assert plan_fits(5, (512, 512, 256), True)            # 8-byte records: 512 MiB
     assert not plan_fits(5, (512, 512, 256), False)       # FP32 sweeps only
     assert not plan_fits(5, (1024, 512, 512), True)       # 2 GiB of records
+    assert plan_fits(7, (512, 512, 256), True)            # variant 5 + bfloat16 rhs
+    assert not plan_fits(7, (512, 512, 256), False)
+    assert not plan_fits(7, (1024, 512, 512), True)
+
+
+def test_variant_7_is_opt_in():
+    """Variant 7 (bfloat16 rhs) is not among the tuners' candidates: the
+    tuners rank plans by time per sweep, and its cycles reduce the residual
+    less; the batched engine cannot run it (the library rejects it)."""
+    from transportse_amd.autotune import BATCH_PLANS32, OPT_IN_PLANS32, PLANS32
+    assert all(p[3] != 7 for p in PLANS32 + BATCH_PLANS32)
+    assert (3, 32, 2, 7) in OPT_IN_PLANS32 and (3, 32, 2, 5) in PLANS32

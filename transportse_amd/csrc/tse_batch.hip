@@ -119,7 +119,7 @@ batch_rb_zmarch_kernel(const BMember *__restrict__ mem,
         } else {
             // the cell-pair sweep on the bfloat16 inner operator
             static_assert(sizeof(T) == 4 && RY == 2 && VAR == 5, "variant 5: FP32, cell pairs");
-            rb_zmarch_tile_x5<NS, BYT>(m.g, w.y, pin, pout,
+            rb_zmarch_tile_x5<NS, BYT, false>(m.g, w.y, pin, pout,
                                        (const uint32_t *)m.fop.rec16, rhs, omega, zero);
         }
         __syncthreads();  // LDS reuse by the next tile
@@ -643,6 +643,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
     if (!fast_plan_ok(pl, fp32, kind) || prm->inner < pl.ns ||
         prm->inner % pl.ns || prm->max_sweeps < 0)
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
+    if (pl.variant == 7)   // the batch residual writes an FP32 rhs
+        return fail(TSE_ERR_ARG, "sweep variant 7 runs single systems only");
     for (int s = 0; s < b->cap; ++s)
         if (b->hused[s]) return fail(TSE_ERR_STATE, "batch still has members");
     b->kind = kind;

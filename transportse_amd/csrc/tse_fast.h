@@ -879,10 +879,14 @@ __device__ __forceinline__ int wy_word(int r, int ip) {
     return (((r & 1) ^ ((r >> 1) & 1)) << 5) + ip;
 }
 
-template <int NS, int BYT, int Q>
+// RB16 (variant 7): the rhs is read as one packed bfloat16 pair per thread
+// (4 bytes at ohcol + k * hbytes) and kept packed in its register ring;
+// otherwise as an FP32 pair at the unknown's offsets.
+template <int NS, int BYT, int Q, bool RB16>
 __device__ __forceinline__ void
 rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pbytes,
-                   uint32_t rbytes, uint32_t ocol, uint32_t orcol, uint32_t oout,
+                   uint32_t rbytes, uint32_t hbytes, uint32_t ocol, uint32_t orcol,
+                   uint32_t ohcol, uint32_t oout,
                    __amdgpu_buffer_rsrc_t rP, __amdgpu_buffer_rsrc_t rRec,
                    __amdgpu_buffer_rsrc_t rB, __amdgpu_buffer_rsrc_t rOut, float omega,
                    float (&sP)[2 * NS + 2][2 * BYT + 2][64], WyRing<NS, BYT> &sWy) {
@@ -895,15 +899,17 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
     const int cp1[2] = {(((0 ^ ((y + 1) >> 1)) & 1) << 5) + ip, (((1 ^ ((y + 1) >> 1)) & 1) << 5) + ip};
     const int gw0 = y >> 1, ww0 = wy_word(y, ip);            // own row in the Wy ring
     const int gwm = (y - 1) >> 1, wwm = wy_word(y - 1, ip);  // row below
-    float p[2][L], rinv[2][L], rb[2][L];
-    uint32_t rwx[L], rwy[L], rwz[L];
+    float p[2][L], rinv[2][L], rb[2][RB16 ? 1 : L];
+    uint32_t rwx[L], rwy[L], rwz[L], rbh[RB16 ? L : 1];
     tse_f2 np;
     uint32_t nwz, nsd;
 #pragma unroll
     for (int r = 0; r < L; ++r) {
         rwx[r] = rwy[r] = rwz[r] = 0u;
 #pragma unroll
-        for (int e = 0; e < 2; ++e) p[e][r] = rinv[e][r] = rb[e][r] = 0.f;
+        for (int e = 0; e < 2; ++e) p[e][r] = rinv[e][r] = 0.f;
+        if constexpr (RB16) rbh[r] = 0u;
+        else rb[0][r] = rb[1][r] = 0.f;
     }
     {
         const bool in = kl0 <= kl1;
@@ -915,9 +921,14 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
         rwy[0] = w.y;
         nwz = w.z;
         nsd = w.w;
-        const tse_f2 b = buf_ld2(rB, o);
-        rb[0][0] = b.x;
-        rb[1][0] = b.y;
+        if constexpr (RB16) {
+            const uint32_t oh = ohcol + (in ? (uint32_t)kl0 * hbytes : 0x80000000u);
+            rbh[0] = __builtin_amdgcn_raw_buffer_load_b32(rB, (int)oh, 0, 0);
+        } else {
+            const tse_f2 b = buf_ld2(rB, o);
+            rb[0][0] = b.x;
+            rb[1][0] = b.y;
+        }
     }
     const bool row_start = ip == 0;   // lanes 0 and 32
     for (int kb = kl0; kb <= klast; kb += L) {
@@ -944,9 +955,15 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                 rwy[un] = w.y;
                 nwz = w.z;
                 nsd = w.w;
-                const tse_f2 b = buf_ld2(rB, o);
-                rb[0][un] = b.x;
-                rb[1][un] = b.y;
+                if constexpr (RB16) {
+                    const uint32_t oh =
+                        ohcol + (in ? (uint32_t)(k + 1) * hbytes : 0x80000000u);
+                    rbh[un] = __builtin_amdgcn_raw_buffer_load_b32(rB, (int)oh, 0, 0);
+                } else {
+                    const tse_f2 b = buf_ld2(rB, o);
+                    rb[0][un] = b.x;
+                    rb[1][un] = b.y;
+                }
             }
             __syncthreads();
             {   // 1/diagonal of plane k (first used by phase 0 at step k+1):
@@ -971,8 +988,9 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                 const int sa = (u - h + 2 * L) % L;         // plane m + 1
                 const int sb = (u - 2 - h + 2 * L) % L;     // plane m - 1
                 const uint32_t wys = sWy[sm][gwm][wwm];
-                float pl, pr, wxl, wxo, wyo, wym, wzo, wzb;
+                float pl, pr, wxl, wxo, wyo, wym, wzo, wzb, bc;
                 if (eact == 0) {
+                    bc = RB16 ? bf_el<0>(rbh[RB16 ? sm : 0]) : rb[0][RB16 ? 0 : sm];
                     pl = from_left(p[1][sm]);
                     pr = p[1][sm];
                     const float w = bf_el<1>((uint32_t)lane_shr1((int)rwx[sm]));
@@ -983,6 +1001,7 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                     wzo = bf_el<0>(rwz[sm]);
                     wzb = bf_el<0>(rwz[sb]);
                 } else {
+                    bc = RB16 ? bf_el<1>(rbh[RB16 ? sm : 0]) : rb[1][RB16 ? 0 : sm];
                     pl = p[0][sm];
                     pr = from_right(p[0][sm]);
                     wxl = bf_el<0>(rwx[sm]);
@@ -992,7 +1011,7 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                     wzo = bf_el<1>(rwz[sm]);
                     wzb = bf_el<1>(rwz[sb]);
                 }
-                float acc = rb[eact][sm];
+                float acc = bc;
                 acc += wxo * pr;
                 acc += wxl * pl;
                 acc += wyo * sP[sm][y + 1][cp1[eact]];
@@ -1015,11 +1034,11 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
     }
 }
 
-template <int NS, int BYT>
+template <int NS, int BYT, bool RB16>
 __device__ __forceinline__ void
 rb_zmarch_tile_x5_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
                      float *__restrict__ Pout, const uint32_t *__restrict__ rec,
-                     const float *__restrict__ rhs, float omega, int zero_init,
+                     const void *__restrict__ rhs, float omega, int zero_init,
                      ZRing<float, NS, 2 * BYT> &sP, WyRing<NS, BYT> &sWy) {
     constexpr int R = 2 * NS;
     constexpr int BX = 64;
@@ -1042,16 +1061,17 @@ rb_zmarch_tile_x5_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
     const int kl0 = max(k0 - R, 0);
     const int kl1 = min(k1 + R, g.nz) - 1;
     const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
-    const uint32_t pbytes = plane * 4u, rbytes = plane * 8u;
+    const uint32_t pbytes = plane * 4u, rbytes = plane * 8u, hbytes = plane * 2u;
     const uint32_t fbytes = pbytes * (uint32_t)g.nz;
     const auto rP = buf_rsrc(Pin, zero_init ? 0u : fbytes);
     const auto rRec = buf_rsrc(rec, rbytes * (uint32_t)g.nz);
-    const auto rB = buf_rsrc(rhs, fbytes);
+    const auto rB = buf_rsrc(rhs, RB16 ? hbytes * (uint32_t)g.nz : fbytes);
     const auto rOut = buf_rsrc(Pout, fbytes);
     const bool in_dom = (unsigned)i < (unsigned)g.nx && (unsigned)j < (unsigned)g.ny;
     const uint32_t cell = (uint32_t)(i + g.nx * j);
     const uint32_t ocol = in_dom ? cell * 4u : kBufMax;
     const uint32_t orcol = in_dom ? cell * 8u : kBufMax;
+    const uint32_t ohcol = in_dom ? cell * 2u : kBufMax;
     const bool own = in_dom && i >= sx.lo && i < sx.hi && j >= sy.lo && j < sy.hi;
     const uint32_t oout = own ? ocol : kBufMax;
     // every ring slot starts at zero: the phases of the first steps read
@@ -1065,22 +1085,24 @@ rb_zmarch_tile_x5_in(const ZGeo &g, int tile, const float *__restrict__ Pin,
     }
     const int q = __builtin_amdgcn_readfirstlane((j + kl0 - 1) & 1);
     if (q == 0)
-        rb_zmarch_x5_march<NS, BYT, 0>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes, ocol,
-                                       orcol, oout, rP, rRec, rB, rOut, omega, sP, sWy);
+        rb_zmarch_x5_march<NS, BYT, 0, RB16>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes,
+                                             hbytes, ocol, orcol, ohcol, oout, rP, rRec,
+                                             rB, rOut, omega, sP, sWy);
     else
-        rb_zmarch_x5_march<NS, BYT, 1>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes, ocol,
-                                       orcol, oout, rP, rRec, rB, rOut, omega, sP, sWy);
+        rb_zmarch_x5_march<NS, BYT, 1, RB16>(ly, ip, k0, k1, kl0, kl1, pbytes, rbytes,
+                                             hbytes, ocol, orcol, ohcol, oout, rP, rRec,
+                                             rB, rOut, omega, sP, sWy);
 }
 
-template <int NS, int BYT>
+template <int NS, int BYT, bool RB16>
 __device__ __forceinline__ void
 rb_zmarch_tile_x5(const ZGeo &g, int tile, const float *__restrict__ Pin,
                   float *__restrict__ Pout, const uint32_t *__restrict__ rec,
-                  const float *__restrict__ rhs, float omega, int zero_init) {
+                  const void *__restrict__ rhs, float omega, int zero_init) {
     __shared__ ZRing<float, NS, 2 * BYT> sP;
     __shared__ WyRing<NS, BYT> sWy;
-    rb_zmarch_tile_x5_in<NS, BYT>(g, tile, Pin, Pout, rec, rhs, omega, zero_init, sP,
-                                  sWy);
+    rb_zmarch_tile_x5_in<NS, BYT, RB16>(g, tile, Pin, Pout, rec, rhs, omega, zero_init,
+                                        sP, sWy);
 }
 
 struct FastOp {  // natural-layout operator arrays

@@ -82,16 +82,17 @@ rb_zmarch_x2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
                                      zero_init);
 }
 
-// Variant 5: cell pairs on the bfloat16-face operator (rec: 12-byte pair
-// records of packed Wx, Wy, Wz; invD from the rounded faces).
-template <int NS, int BYT>
+// Variant 5: cell pairs on the bfloat16 inner operator (rec: 16-byte pair
+// records of packed Wx, Wy, Wz and V R; the diagonal summed from the rounded
+// values); RB16 (variant 7): the rhs as packed bfloat16 pairs as well.
+template <int NS, int BYT, bool RB16>
 __global__ void __launch_bounds__(64 * BYT)
 rb_zmarch_x5_kernel(ZGeo g, const float *__restrict__ Pin, float *__restrict__ Pout,
-                    const uint32_t *__restrict__ rec, const float *__restrict__ rhs,
+                    const uint32_t *__restrict__ rec, const void *__restrict__ rhs,
                     float omega, int zero_init) {
     const int tile = xcd_tile(g);
     if (tile >= g.ntiles) return;
-    rb_zmarch_tile_x5<NS, BYT>(g, tile, Pin, Pout, rec, rhs, omega, zero_init);
+    rb_zmarch_tile_x5<NS, BYT, RB16>(g, tile, Pin, Pout, rec, rhs, omega, zero_init);
 }
 
 // ---- operator assembly (from split-layout fields) ---------------------------------------
@@ -326,11 +327,27 @@ struct RzGeo {
     int tiles_x, tiles_y, zchunk, ntiles;
 };
 
-template <int KIND, int MODE>
+// R16: the correction's rhs is written as bfloat16 for sweep variant 7, as
+// FP32 otherwise.  The bfloat16 value is the FP32 value rounded
+// stochastically with a dither fixed by the cell index: the 16 discarded
+// bits plus hash(t) mod 2^16 carry into the kept ones.  Nearest rounding of
+// a smooth residual leaves a smooth (low-frequency) error, the kind SOR
+// removes slowest in the next cycle; the dithered error is white (512x512x256
+// at omega 1.99672: 6318 sweeps to 1e-10 with nearest rounding, 5865 with
+// the FP32 rhs).  hash: the lowbias32 integer mix (C. Wellons).
+__device__ __forceinline__ uint32_t rhs_dither(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x & 0xffffu;
+}
+template <int KIND, int MODE, bool R16>
 __global__ void __launch_bounds__(kBlock)
 residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ pin,
                    const float *__restrict__ e, double *__restrict__ pout, FastOp pop,
-                   TrOp trop, float *__restrict__ r32, double *__restrict__ partials) {
+                   TrOp trop, void *__restrict__ rout, double *__restrict__ partials) {
     const long plane = (long)nx * ny;
     const long N = plane * nz;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
@@ -395,7 +412,14 @@ residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ 
             r = inv != 0.0 ? r : 0.0;          // Dirichlet cells: r = 0
             sr += r * r;
             sb += inv != 0.0 ? bb * bb : 0.0;
-            if (MODE != 0) r32[t] = (float)r;
+            if (MODE != 0) {
+                if constexpr (R16) {
+                    const uint32_t u = __float_as_uint((float)r) + rhs_dither((uint32_t)t);
+                    ((uint16_t *)rout)[t] = (uint16_t)(u >> 16);
+                } else {
+                    ((float *)rout)[t] = (float)r;
+                }
+            }
             pm = pc;
             pc = kt ? pp : 0.0;
         }
@@ -498,22 +522,28 @@ static int grid_n(long n) {
 }
 
 // Launches a residual pass and returns ||r||, ||b|| (synchronises).
+// r16: write the rhs as bfloat16 (pressure systems, sweep variant 7) into
+// the FP32 rhs array's memory.
 template <int MODE>
 static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
-                    double *pout, double *rn, double *bn) {
+                    double *pout, double *rn, double *bn, bool r16 = false) {
     const Geo G = fast_tiling_geo(ctx->geo, s->kind);
     const RzGeo rg = make_rzgeo(G);
     const int nb = std::max(1, std::min(rg.ntiles, 4096));
     const FastOp pop = s->kind == SYS_P ? s->pop() : FastOp{};
     const TrOp trop = s->kind == SYS_TR ? s->trop() : TrOp{};
-    if (s->kind == SYS_P)
-        hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE>), dim3(nb), dim3(kBlock), 0,
-                           ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
-                           s->r32(), s->partials);
+    if (s->kind == SYS_P && r16)
+        hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE, true>), dim3(nb), dim3(kBlock),
+                           0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
+                           (void *)s->r32(), s->partials);
+    else if (s->kind == SYS_P)
+        hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE, false>), dim3(nb), dim3(kBlock),
+                           0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
+                           (void *)s->r32(), s->partials);
     else
-        hipLaunchKernelGGL((residual_zm_kernel<SYS_TR, MODE>), dim3(nb), dim3(kBlock), 0,
-                           ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
-                           s->r32(), s->partials);
+        hipLaunchKernelGGL((residual_zm_kernel<SYS_TR, MODE, false>), dim3(nb), dim3(kBlock),
+                           0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
+                           (void *)s->r32(), s->partials);
     TSE_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
                        s->partials, nb, s->red);
@@ -572,13 +602,19 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
 }
 
+// Variants 5 and 7; rhs: FP32 (5) or bfloat16 (7) values.
 static int launch_zmarch5(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
                           const float *Pin, float *Pout, const FastOp &op,
-                          const float *rhs, float omega, int zero_init) {
+                          const void *rhs, float omega, int zero_init) {
     const dim3 grid(zg.per_xcd * 8);
-    if (pl.variant == 5 && pl.by == 32 && pl.ry == 2 && (pl.ns == 2 || pl.ns == 3)) {
+    if ((pl.variant == 5 || pl.variant == 7) && pl.by == 32 && pl.ry == 2 &&
+        (pl.ns == 2 || pl.ns == 3)) {
         const uint32_t *rec = (const uint32_t *)op.rec16;
-        auto k = pl.ns == 2 ? rb_zmarch_x5_kernel<2, 16> : rb_zmarch_x5_kernel<3, 16>;
+        auto k = pl.variant == 7
+                     ? (pl.ns == 2 ? rb_zmarch_x5_kernel<2, 16, true>
+                                   : rb_zmarch_x5_kernel<3, 16, true>)
+                     : (pl.ns == 2 ? rb_zmarch_x5_kernel<2, 16, false>
+                                   : rb_zmarch_x5_kernel<3, 16, false>);
         hipLaunchKernelGGL(k, grid, dim3(64, 16), 0, st, zg, Pin, Pout, rec, rhs, omega,
                            zero_init);
         TSE_HIP_TRY(hipGetLastError());
@@ -616,13 +652,14 @@ static int launch_tr_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan
 }
 
 bool fast_plan_fits(const tse_sweep_plan &pl, long N, int fp32) {
-    if (pl.variant == 5) return N * 8L <= (long)kBufMax;   // 8-byte records per cell
+    if (pl.variant == 5 || pl.variant == 7)
+        return N * 8L <= (long)kBufMax;   // 8-byte records per cell
     return pl.variant < 2 || N * (fp32 ? 4L : 8L) <= (long)kBufMax;
 }
 
 bool fast_plan_grid_ok(const tse_sweep_plan &pl, int nx) {
-    // variants 3 and 5 read cell pairs
-    return (pl.variant != 3 && pl.variant != 5) || nx % 2 == 0;
+    // variants 3, 5 and 7 read cell pairs
+    return (pl.variant != 3 && pl.variant != 5 && pl.variant != 7) || nx % 2 == 0;
 }
 
 bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
@@ -640,7 +677,8 @@ bool fast_plan_ok(const tse_sweep_plan &pl, int fp32, int kind) {
                                  {1, 3, 16, 1}, {1, 3, 24, 2}, {2, 2, 24, 2},
                                  {2, 2, 32, 2}, {2, 3, 16, 1}, {2, 3, 24, 2},
                                  {2, 3, 32, 2}, {3, 2, 32, 2}, {3, 3, 32, 2},
-                                 {5, 2, 32, 2}, {5, 3, 32, 2}};
+                                 {5, 2, 32, 2}, {5, 3, 32, 2}, {7, 2, 32, 2},
+                                 {7, 3, 32, 2}};
     auto hit = [&](const int c[4]) {
         return pl.variant == c[0] && pl.ns == c[1] && pl.by == c[2] && pl.ry == c[3];
     };
@@ -700,7 +738,7 @@ static int correction_sweeps(tse_ctx *ctx, FastSys *s, const ZGeo &zg,
     int e = 0;
     for (int q = 0; q < n / pl.ns; ++q) {
         int rc;
-        if (s->kind == SYS_P && pl.variant == 5) {
+        if (s->kind == SYS_P && (pl.variant == 5 || pl.variant == 7)) {
             FastOp op = s->pop();
             rc = launch_zmarch5(ctx->stream, zg, pl, s->e(e), s->e(1 - e), op,
                                 s->r32(), sor, q == 0);
@@ -758,10 +796,10 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     if (!fast_plan_ok(*plan, fp32, s->kind) || n_sweeps < 0 || n_sweeps % plan->ns)
         return fail(TSE_ERR_ARG, "unsupported plan or n_sweeps not a multiple of ns");
     if (!fast_plan_fits(*plan, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
-                                 "(variant 5: 8 bytes per cell)");
+        return fail(TSE_ERR_ARG, "sweep variants 2, 3, 5 and 7 need fields of at most 1 GiB "
+                                 "(variants 5 and 7: 8 bytes per cell)");
     if (!fast_plan_grid_ok(*plan, s->nxf))
-        return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
+        return fail(TSE_ERR_ARG, "sweep variants 3, 5 and 7 need an even nx");
     const ZGeo zg = make_zgeo(fast_tiling_geo(ctx->geo, s->kind), plan->ns, plan->by,
                               plan->zchunk, plan->variant >= 2);
     if (!fp32) return plain_sweeps(ctx, s, zg, *plan, n_sweeps, sor);
@@ -784,14 +822,15 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     if (prm->cycle_mode != TSE_CYCLE_FIXED && prm->cycle_mode != TSE_CYCLE_ADAPTIVE)
         return fail(TSE_ERR_ARG, "cycle_mode must be TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE");
     if (!fast_plan_fits(pl, s->N, fp32))
-        return fail(TSE_ERR_ARG, "sweep variants 2, 3 and 5 need fields of at most 1 GiB "
-                                 "(variant 5: 8 bytes per cell)");
+        return fail(TSE_ERR_ARG, "sweep variants 2, 3, 5 and 7 need fields of at most 1 GiB "
+                                 "(variants 5 and 7: 8 bytes per cell)");
     if (!fast_plan_grid_ok(pl, s->nxf))
-        return fail(TSE_ERR_ARG, "sweep variants 3 and 5 need an even nx");
+        return fail(TSE_ERR_ARG, "sweep variants 3, 5 and 7 need an even nx");
     const ZGeo zg = make_zgeo(fast_tiling_geo(ctx->geo, s->kind), pl.ns, pl.by, pl.zchunk,
                               pl.variant >= 2);
     double rn, bn;
-    int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn)
+    const bool r16 = pl.variant == 7;   // the correction's rhs in bfloat16
+    int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn, r16)
                   : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
     if (rc) return rc;
     const double bs = bn > 0 ? bn : 1.0;
@@ -807,7 +846,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             rc = correction_sweeps(ctx, s, zg, pl, run, (float)prm->sor, &e);
             if (rc) return rc;
             rc = residual<1>(ctx, s, s->p(s->cur), s->e(e), s->p(1 - s->cur),
-                             &rn, &bn);
+                             &rn, &bn, r16);
             if (rc) return rc;
             s->cur = 1 - s->cur;
         } else {
@@ -826,11 +865,13 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             long next = inner;
             if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
                 const double rate = std::log(rel / rel_prev) / run;     // < 0
-                // a variant-5 cycle cannot reduce the error by much more
-                // than its inner operator's 2^-8 accuracy: aim at most two
-                // decades per cycle, so its sweeps are not spent past that
-                const double target =
-                    pl.variant == 5 ? std::max(prm->tol / rel, 1e-2) : prm->tol / rel;
+                // a variant-5 or -7 cycle cannot reduce the error by much
+                // more than its inner operator's (and variant 7's rhs's)
+                // 2^-8 accuracy: aim at most two decades per cycle, so its
+                // sweeps are not spent past that
+                const double target = pl.variant == 5 || pl.variant == 7
+                                          ? std::max(prm->tol / rel, 1e-2)
+                                          : prm->tol / rel;
                 const double need = std::log(target) / rate;             // > 0
                 next = (long)std::ceil(need * 1.02);
             }
