@@ -781,13 +781,16 @@ int tse_batch_rearm_many(tse_batch *b, const int32_t *slots, int32_t n) {
 }
 
 // Hands the member's current iterate back to its context (the fast-form
-// result / residual calls then see it) without removing it.
+// result / residual calls then see it) without removing it.  Host only: every
+// batch call that launches work (add, re-arm, cycle, run) ends by reading the
+// state back after its stream has drained, so hst is current and the
+// batch's kernels are done; work the caller has queued since (a member's next
+// set-up on the same stream) is not waited for.
 int tse_batch_writeback(tse_batch *b, int32_t slot) {
     int rc = check_batch(b);
     if (rc) return rc;
     if (slot < 0 || slot >= b->cap || !b->hused[slot])
         return fail(TSE_ERR_ARG, "no member in that slot");
-    TSE_B_TRY(hipStreamSynchronize(b->stream));
     member_sys(b, b->ctx[slot])->cur = b->hst[slot].cur;
     return 0;
 }
