@@ -188,9 +188,10 @@ def main(argv=None):
                     "omega": om, "converged": st.converged,
                     "plan_autotune_s": tune_s if not args.plan else 0.0}
         if not args.omega:
-            tuner.report(st.sweeps)
+            tuner.report(st.sweeps, st.contraction)
         log(f"warmup solve (omega {om:.5f}): {st.sweeps} sweeps, rel "
-            f"{st.rel_residual:.2e}, {st.seconds:.2f}s")
+            f"{st.rel_residual:.2e}, {st.seconds:.2f}s, contraction per sweep "
+            f"{st.contraction:.6f}")
     if not args.omega:      # the best factor the warm-up solves found
         tuner.u = tuner.best_u if tuner.best_u is not None else tuner.u
     omega = args.omega if args.omega else tuner.omega

@@ -77,10 +77,10 @@ def test_variant2_plans_filtered_by_field_size():
     assert not plan_fits(7, (1024, 512, 512), True)
 
 
-def test_variant_7_is_opt_in():
-    """Variant 7 (bfloat16 rhs) is not among the tuners' candidates: the
-    tuners rank plans by time per sweep, and its cycles reduce the residual
-    less; the batched engine cannot run it (the library rejects it)."""
-    from transportse_amd.autotune import BATCH_PLANS32, OPT_IN_PLANS32, PLANS32
-    assert all(p[3] != 7 for p in PLANS32 + BATCH_PLANS32)
-    assert (3, 32, 2, 7) in OPT_IN_PLANS32 and (3, 32, 2, 5) in PLANS32
+def test_variant_7_single_systems_only():
+    """Variant 7 (FP16 rhs) is a candidate of the single-system tuner; the
+    batched engine writes an FP32 rhs, so its tuner leaves it out (the
+    library rejects it in a batch)."""
+    from transportse_amd.autotune import BATCH_PLANS32, PLANS32
+    assert all(p[3] != 7 for p in BATCH_PLANS32)
+    assert (3, 32, 2, 7) in PLANS32 and (3, 32, 2, 5) in BATCH_PLANS32

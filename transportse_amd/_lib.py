@@ -63,7 +63,8 @@ class FastResult(ctypes.Structure):
     _fields_ = [("sweeps", ctypes.c_int32), ("cycles", ctypes.c_int32),
                 ("converged", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("initial_rel", ctypes.c_double),
-                ("rel_residual", ctypes.c_double)]
+                ("rel_residual", ctypes.c_double),
+                ("contraction", ctypes.c_double)]
 
 
 class EosParams(ctypes.Structure):

@@ -31,14 +31,12 @@ PLANS32 = PLANS64 + [(1, 32, 2, 0), (2, 32, 2, 0), (2, 32, 4, 0), (3, 16, 1, 0),
 PLANS32 = PLANS32 + [(2, 32, 2, 5), (3, 32, 2, 5)]
 # the batched engine runs the same plans (pressure rows are always even: an
 # odd nx gets one inactive column)
-BATCH_PLANS32 = PLANS32
-# variant 7 (single systems, opt-in through an explicit plan): variant 5
-# with the correction's rhs stored as bfloat16 as well (18 instead of 20
-# bytes per cell).  Its launches are ~4.5 % shorter, but its cycles reduce
-# the residual less (512x512x256: 6 % less per cycle at the tuned SOR
-# factor, up to 25 % less at the tuner's starting factor), so the tuner,
-# which ranks plans by time per sweep, does not offer it.
-OPT_IN_PLANS32 = [(2, 32, 2, 7), (3, 32, 2, 7)]
+BATCH_PLANS32 = list(PLANS32)
+# variant 7 (single systems): variant 5 with the correction's rhs stored as
+# FP16 as well (18 instead of 20 bytes per cell), scaled by a power of two
+# and rounded with a dither; the batched engine writes an FP32 rhs and does
+# not run it
+PLANS32 = PLANS32 + [(2, 32, 2, 7), (3, 32, 2, 7)]
 # transport: the cell's own coefficients stay in registers, one row per thread
 TR_PLANS64 = [(1, 8, 1, 0), (1, 16, 1, 0), (2, 16, 1, 0),
               (1, 8, 1, 2), (1, 16, 1, 2), (2, 16, 1, 2)]

@@ -881,6 +881,7 @@ int tse_batch_status(tse_batch *b, int32_t slot, tse_fast_result *out) {
     out->pad = x.active;
     out->initial_rel = x.rel0;
     out->rel_residual = x.rel;
+    out->contraction = 0.0;
     return 0;
 }
 

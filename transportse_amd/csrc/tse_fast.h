@@ -865,6 +865,10 @@ template <int E>
 __device__ __forceinline__ float bf_el(uint32_t v) {  // element E of a packed pair
     return E ? __uint_as_float(v & 0xffff0000u) : __uint_as_float(v << 16);
 }
+template <int E>
+__device__ __forceinline__ float h_el(uint32_t v) {  // element E of a packed FP16 pair
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(E ? v >> 16 : v & 0xffffu));
+}
 typedef unsigned int tse_u32x4 __attribute__((ext_vector_type(4)));
 __device__ __forceinline__ tse_u32x4 buf_ld4(__amdgpu_buffer_rsrc_t r, uint32_t off) {
     return __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0);
@@ -879,9 +883,10 @@ __device__ __forceinline__ int wy_word(int r, int ip) {
     return (((r & 1) ^ ((r >> 1) & 1)) << 5) + ip;
 }
 
-// RB16 (variant 7): the rhs is read as one packed bfloat16 pair per thread
-// (4 bytes at ohcol + k * hbytes) and kept packed in its register ring;
-// otherwise as an FP32 pair at the unknown's offsets.
+// RB16 (variant 7): the rhs is read as one packed FP16 pair per thread (4
+// bytes at ohcol + k * hbytes, scaled by a power of two chosen by the
+// residual pass) and kept packed in its register ring; otherwise as an FP32
+// pair at the unknown's offsets.
 template <int NS, int BYT, int Q, bool RB16>
 __device__ __forceinline__ void
 rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pbytes,
@@ -990,7 +995,7 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                 const uint32_t wys = sWy[sm][gwm][wwm];
                 float pl, pr, wxl, wxo, wyo, wym, wzo, wzb, bc;
                 if (eact == 0) {
-                    bc = RB16 ? bf_el<0>(rbh[RB16 ? sm : 0]) : rb[0][RB16 ? 0 : sm];
+                    bc = RB16 ? h_el<0>(rbh[RB16 ? sm : 0]) : rb[0][RB16 ? 0 : sm];
                     pl = from_left(p[1][sm]);
                     pr = p[1][sm];
                     const float w = bf_el<1>((uint32_t)lane_shr1((int)rwx[sm]));
@@ -1001,7 +1006,7 @@ rb_zmarch_x5_march(int ly, int ip, int k0, int k1, int kl0, int kl1, uint32_t pb
                     wzo = bf_el<0>(rwz[sm]);
                     wzb = bf_el<0>(rwz[sb]);
                 } else {
-                    bc = RB16 ? bf_el<1>(rbh[RB16 ? sm : 0]) : rb[1][RB16 ? 0 : sm];
+                    bc = RB16 ? h_el<1>(rbh[RB16 ? sm : 0]) : rb[1][RB16 ? 0 : sm];
                     pl = p[0][sm];
                     pr = from_right(p[0][sm]);
                     wxl = bf_el<0>(rwx[sm]);

@@ -327,32 +327,36 @@ struct RzGeo {
     int tiles_x, tiles_y, zchunk, ntiles;
 };
 
-// R16: the correction's rhs is written as bfloat16 for sweep variant 7, as
-// FP32 otherwise.  The bfloat16 value is the FP32 value rounded
-// stochastically with a dither fixed by the cell index: the 16 discarded
-// bits plus hash(t) mod 2^16 carry into the kept ones.  Nearest rounding of
-// a smooth residual leaves a smooth (low-frequency) error, the kind SOR
-// removes slowest in the next cycle; the dithered error is white (512x512x256
-// at omega 1.99672: 6318 sweeps to 1e-10 with nearest rounding, 5865 with
-// the FP32 rhs).  hash: the lowbias32 integer mix (C. Wellons).
-__device__ __forceinline__ uint32_t rhs_dither(uint32_t x) {
+// R16: the correction's rhs is written as FP16 for sweep variant 7, as FP32
+// otherwise.  The FP16 value is r * rscale (rscale a power of two, so the
+// product is exact) rounded to FP32 and then to FP16 stochastically, with a
+// dither fixed by the cell index: the 13 discarded mantissa bits plus
+// hash(t) mod 2^13 carry into the kept ones, and the result is clamped to
+// +-65504.  Nearest rounding of a smooth residual leaves a smooth
+// (low-frequency) error, the kind SOR removes slowest in the next cycle; the
+// dithered error is white.  In MODE 1 the correction read back is e * escale
+// (escale = 1 / the rscale of the rhs the sweeps solved with).  hash: the
+// lowbias32 integer mix (C. Wellons).
+__device__ __forceinline__ uint32_t rhs_hash(uint32_t x) {
     x ^= x >> 16;
     x *= 0x7feb352du;
     x ^= x >> 15;
     x *= 0x846ca68bu;
     x ^= x >> 16;
-    return x & 0xffffu;
+    return x;
 }
 template <int KIND, int MODE, bool R16>
 __global__ void __launch_bounds__(kBlock)
 residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ pin,
                    const float *__restrict__ e, double *__restrict__ pout, FastOp pop,
-                   TrOp trop, void *__restrict__ rout, double *__restrict__ partials) {
+                   TrOp trop, void *__restrict__ rout, double *__restrict__ partials,
+                   double escale, double rscale) {
     const long plane = (long)nx * ny;
     const long N = plane * nz;
     const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
     double sr = 0.0, sb = 0.0;
     auto X = [&](long o) -> double {
+        if constexpr (MODE == 1 && R16) return pin[o] + (double)e[o] * escale;
         return MODE == 1 ? pin[o] + (double)e[o] : pin[o];
     };
     // x / y neighbours of the edge cells are read at clamped indices and
@@ -414,8 +418,11 @@ residual_zm_kernel(int nx, int ny, int nz, RzGeo rg, const double *__restrict__ 
             sb += inv != 0.0 ? bb * bb : 0.0;
             if (MODE != 0) {
                 if constexpr (R16) {
-                    const uint32_t u = __float_as_uint((float)r) + rhs_dither((uint32_t)t);
-                    ((uint16_t *)rout)[t] = (uint16_t)(u >> 16);
+                    const uint32_t u = __float_as_uint((float)(r * rscale)) +
+                                       (rhs_hash((uint32_t)t) & 0x1fffu);
+                    const float x = fminf(fmaxf(__uint_as_float(u & ~0x1fffu), -65504.f),
+                                          65504.f);
+                    ((_Float16 *)rout)[t] = (_Float16)x;
                 } else {
                     ((float *)rout)[t] = (float)r;
                 }
@@ -522,11 +529,13 @@ static int grid_n(long n) {
 }
 
 // Launches a residual pass and returns ||r||, ||b|| (synchronises).
-// r16: write the rhs as bfloat16 (pressure systems, sweep variant 7) into
-// the FP32 rhs array's memory.
+// r16: write the rhs as FP16 scaled by rscale (pressure systems, sweep
+// variant 7) into the FP32 rhs array's memory, and read the correction as
+// e * escale.
 template <int MODE>
 static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
-                    double *pout, double *rn, double *bn, bool r16 = false) {
+                    double *pout, double *rn, double *bn, bool r16 = false,
+                    double escale = 1.0, double rscale = 1.0) {
     const Geo G = fast_tiling_geo(ctx->geo, s->kind);
     const RzGeo rg = make_rzgeo(G);
     const int nb = std::max(1, std::min(rg.ntiles, 4096));
@@ -535,15 +544,15 @@ static int residual(tse_ctx *ctx, FastSys *s, const double *pin, const float *e,
     if (s->kind == SYS_P && r16)
         hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE, true>), dim3(nb), dim3(kBlock),
                            0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
-                           (void *)s->r32(), s->partials);
+                           (void *)s->r32(), s->partials, escale, rscale);
     else if (s->kind == SYS_P)
         hipLaunchKernelGGL((residual_zm_kernel<SYS_P, MODE, false>), dim3(nb), dim3(kBlock),
                            0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
-                           (void *)s->r32(), s->partials);
+                           (void *)s->r32(), s->partials, escale, rscale);
     else
         hipLaunchKernelGGL((residual_zm_kernel<SYS_TR, MODE, false>), dim3(nb), dim3(kBlock),
                            0, ctx->stream, G.nx, G.ny, G.nz, rg, pin, e, pout, pop, trop,
-                           (void *)s->r32(), s->partials);
+                           (void *)s->r32(), s->partials, escale, rscale);
     TSE_HIP_TRY(hipGetLastError());
     hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kBlock), 0, ctx->stream,
                        s->partials, nb, s->red);
@@ -829,12 +838,31 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     const ZGeo zg = make_zgeo(fast_tiling_geo(ctx->geo, s->kind), pl.ns, pl.by, pl.zchunk,
                               pl.variant >= 2);
     double rn, bn;
-    const bool r16 = pl.variant == 7;   // the correction's rhs in bfloat16
-    int rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn, r16)
-                  : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
+    // variant 7: the correction's rhs in FP16, scaled by a power of two near
+    // 1 / RMS of the last residual whose norm is known (for the first one a
+    // norm-only pass), so no entry can exceed the FP16 range while the
+    // residual shrinks (|r_i| <= sqrt(N) RMS < 65504 for N < 2^32)
+    const bool r16 = pl.variant == 7;
+    auto rhs_scale = [&](double rnorm) {
+        if (!(rnorm > 0) || !std::isfinite(rnorm)) return 1.0;
+        int ex;
+        (void)std::frexp(std::sqrt((double)s->N) / rnorm, &ex);
+        return std::ldexp(1.0, std::min(std::max(ex - 1, -120), 120));
+    };
+    double rscale = 1.0;
+    int rc = 0;
+    if (fp32 && r16) {
+        rc = residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
+        if (rc) return rc;
+        rscale = rhs_scale(rn);
+    }
+    rc = fp32 ? residual<2>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn, r16, 1.0,
+                            rscale)
+              : residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
     if (rc) return rc;
     const double bs = bn > 0 ? bn : 1.0;
     out->initial_rel = rn / bs;
+    out->contraction = 0.0;
     double rel = rn / bs;
     int sweeps = 0, cycles = 0;
     while (rel > prm->tol && sweeps < prm->max_sweeps) {
@@ -845,8 +873,10 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             int e;
             rc = correction_sweeps(ctx, s, zg, pl, run, (float)prm->sor, &e);
             if (rc) return rc;
+            const double escale = 1.0 / rscale;   // exact: a power of two
+            rscale = r16 ? rhs_scale(rn) : 1.0;
             rc = residual<1>(ctx, s, s->p(s->cur), s->e(e), s->p(1 - s->cur),
-                             &rn, &bn, r16);
+                             &rn, &bn, r16, escale, rscale);
             if (rc) return rc;
             s->cur = 1 - s->cur;
         } else {
@@ -860,15 +890,19 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
         const double rel_prev = rel;
         rel = rn / bs;
         if (!(rel == rel)) break;  // NaN
+        // per-sweep contraction of this cycle (the first one starts with the
+        // fast-decaying components, so only later cycles are reported)
+        if (cycles > 1 && rel < rel_prev && rel > 0)
+            out->contraction = std::exp(std::log(rel / rel_prev) / run);
         if (prm->cycle_mode == TSE_CYCLE_ADAPTIVE) {
             // sweeps still needed at the last cycle's rate, clamped
             long next = inner;
             if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
                 const double rate = std::log(rel / rel_prev) / run;     // < 0
                 // a variant-5 or -7 cycle cannot reduce the error by much
-                // more than its inner operator's (and variant 7's rhs's)
-                // 2^-8 accuracy: aim at most two decades per cycle, so its
-                // sweeps are not spent past that
+                // more than its inner operator's 2^-8 accuracy: aim at most
+                // two decades per cycle, so its sweeps are not spent past
+                // that
                 const double target = pl.variant == 5 || pl.variant == 7
                                           ? std::max(prm->tol / rel, 1e-2)
                                           : prm->tol / rel;
@@ -1068,9 +1102,10 @@ static tse_sweep_plan default_fast_plan(const tse_ctx *ctx, int kind, int fp32,
     const long N = (long)fast_nx(G, kind) * G.ny * G.nz;
     tse_sweep_plan pl{explicit_step ? 1 : 2, 16, 1, 0, 2};
     if (kind == SYS_P && fp32) {
-        // the cell-pair sweep on the bfloat16 inner operator (pressure rows
-        // are even) where its records fit, else the unrolled-ring sweep
-        pl = tse_sweep_plan{3, 32, 2, 0, 5};
+        // the cell-pair sweep on the bfloat16 inner operator with the FP16
+        // rhs (pressure rows are even) where its records fit, else the
+        // unrolled-ring sweep
+        pl = tse_sweep_plan{3, 32, 2, 0, 7};
         if (!fast_plan_fits(pl, N, fp32)) pl.variant = 2;
     }
     if (!fast_plan_fits(pl, N, fp32)) {  // fields over 1 GiB: no buffer loads

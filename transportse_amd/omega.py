@@ -12,6 +12,28 @@ after every solve moves u by a step towards the side that reduced the sweep
 count, reversing and halving the step when a move did not.  Its state
 depends only on the scenario's own solves, so a scenario gets the same
 factors, hence the same arithmetic, whatever it is batched with.
+
+A solve may also report the residual contraction per sweep it observed
+(``lam``, over its last correction cycle after the first).  The pressure
+operator is symmetric positive definite with the 7-point red/black
+ordering, i.e. consistently ordered, so SOR's eigenvalues obey Young's
+relation (lam + omega - 1)^2 = lam omega^2 mu^2, mu = the Jacobi
+iteration's spectral radius.  Above the optimum every eigenvalue has
+modulus omega - 1; below it the dominant one is real and larger.  A solve
+whose contraction is clearly slower than omega - 1 therefore ran below the
+optimum, and mu = (lam + omega - 1) / (omega sqrt(lam)) gives the optimum
+omega_opt = 2 / (1 + sqrt(1 - mu^2)) (Young, Iterative Solution of Large
+Linear Systems, 1971, ch. 6; Hageman and Young, Applied Iterative Methods,
+1981, ch. 9).  The tuner moves there, then refines from just above it with
+small hill-climbing steps, where the sweep count rises gently; a solve
+whose contraction is about omega - 1 sends the next move downwards.
+Scenarios report sweep counts only (batched solves observe no contraction,
+and the estimate is continuous in it), so a scenario's factors stay the same
+whether it runs alone, batched or in a server; single solves (the pressure
+benchmark's warm-up) report both.  The
+irregular sweep counts of adaptive cycles (a few percent between
+neighbouring factors) mislead a hill climb alone; the estimate does not
+depend on them.
 """
 from __future__ import annotations
 
@@ -19,6 +41,12 @@ import math
 
 
 class OmegaTuner:
+    # a contraction per sweep slower than (omega - 1)^BELOW marks a factor
+    # below the optimum (the measured one above it is within ~1 % of
+    # omega - 1 in rate)
+    BELOW = 0.9
+    MAX_JUMPS = 3
+
     def __init__(self, n_max: int, step: float = 0.5, u_min: float = 2.0,
                  u_max: float = 10.0, min_step: float = 1.0 / 32):
         self.u = min(max(math.log2(max(n_max, 4) / 2.0), u_min), u_max)
@@ -27,17 +55,50 @@ class OmegaTuner:
         self.u_min, self.u_max, self.min_step = u_min, u_max, min_step
         self.best_u = None
         self.best_s = None
+        self.jumps = 0
 
     @property
     def omega(self) -> float:
         return 2.0 - 2.0 ** (-self.u)
 
-    def report(self, sweeps: int) -> None:
-        """Sweeps the last solve at ``omega`` took; picks the next omega."""
+    def model_u(self, lam: float):
+        """u of the optimal factor from the contraction per sweep `lam`
+        observed at the current factor, if it shows that factor to be below
+        the optimum; else None."""
+        om = self.omega
+        if not (0.0 < lam < 1.0) or not (1.0 < om < 2.0):
+            return None
+        if math.log(lam) <= self.BELOW * math.log(om - 1.0):
+            return None           # about omega - 1: at or above the optimum
+        mu = (lam + om - 1.0) / (om * math.sqrt(lam))
+        if not (0.0 < mu < 1.0):
+            return None
+        om_opt = 2.0 / (1.0 + math.sqrt(1.0 - mu * mu))
+        return -math.log2(2.0 - om_opt)
+
+    def report(self, sweeps: int, contraction: float = 0.0) -> None:
+        """Sweeps the last solve at ``omega`` took (and, if known, the
+        residual contraction per sweep it observed); picks the next omega."""
+        est = self.model_u(contraction) if contraction else None
+        # a contraction of about omega - 1: at or above the optimum, so a
+        # move that continues goes down
+        above = 0.0 < contraction < 1.0 and est is None
+        if est is not None and est - self.u >= self.min_step and \
+                self.jumps < self.MAX_JUMPS:
+            if self.best_s is None or sweeps < self.best_s:
+                self.best_u, self.best_s = self.u, sweeps
+            # below the optimum: go to the estimate, then refine upwards
+            self.jumps += 1
+            self.u = min(max(est, self.u_min), self.u_max)
+            self.dir = 1.0
+            self.step = min(self.step, 1.0 / 16)
+            return
         if self.best_s is None or sweeps < self.best_s:
             self.best_u, self.best_s = self.u, sweeps
             if self.step < self.min_step:
                 return
+            if above:
+                self.dir = -1.0
             nxt = self.u + self.dir * self.step
         else:
             self.dir = -self.dir

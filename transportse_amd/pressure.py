@@ -35,6 +35,7 @@ class SolveStats:
     rel_residual: float
     seconds: float
     plan: tuple
+    contraction: float = 0.0   # per sweep, last cycle after the first (0: none)
 
 
 class FastSystem:
@@ -92,7 +93,8 @@ class FastSystem:
                                      ctypes.byref(res)), f"tse_fast_{self.KIND}_solve")
         sec = time.perf_counter() - t0
         return SolveStats(form, res.sweeps, res.cycles, bool(res.converged),
-                          res.initial_rel, res.rel_residual, sec, tuple(pl))
+                          res.initial_rel, res.rel_residual, sec, tuple(pl),
+                          res.contraction)
 
     def result_device(self):
         """Write the current iterate into a split-layout device field (kept
