@@ -14,11 +14,14 @@ Reported (one JSON line on stdout):
   roofline     dominant kernel (FP32 fused red/black sweep, NS sweeps per
                launch), timed with HIP events on the library's stream:
                achieved = algorithmic bytes per launch (one pass of the
-               operands, 28 B per cell) / mean launch time, frac = achieved /
+               operands: 18 B per cell for the tuned plan's sweep with the
+               bfloat16 operator and FP16 rhs, 20 B with the FP32 rhs, 28 B
+               with the FP32 operator) / mean launch time, frac = achieved /
                8 TB/s; traffic = HBM bytes per launch measured in this run by
                two rocprofv3 --pmc child runs (FETCH_SIZE, WRITE_SIZE, each
                scaled by its reading for a 1 GiB copy); the per-sweep figure
-               (every fused sweep at 28 B per cell) is effective_per_sweep_*
+               (every fused sweep at those bytes per cell) is
+               effective_per_sweep_*
   cpu_baseline the oracle's OpenMP red/black sweeps on the host cores, and
                the time to solution of one compute_P call (oracle, faithful
                and mixed GPU forms) on the 64x32x32 problem
@@ -71,6 +74,19 @@ BYTES_FP32_SWEEP = 28   # 3 faces + 1/diag + rhs + unknown read + write, FP32
 # diagonal is summed in the kernel): 8 + 4 + 4 + 4 bytes; variant 7 the rhs
 # as well: 8 + 2 + 4 + 4 bytes
 BYTES_FP32_SWEEP_BY_VARIANT = {5: 20, 7: 18}
+BYTES_DEFINITION = {
+    5: "20 B per cell (16-byte pair record of the 3 face transmissibilities and "
+       "the storage term as bfloat16, rhs, unknown in and out FP32), once per "
+       "launch",
+    7: "18 B per cell (16-byte pair record of the 3 face transmissibilities and "
+       "the storage term as bfloat16, rhs FP16, unknown in and out FP32), once "
+       "per launch"}
+BYTES_DEFINITION_FP32 = ("28 B per cell (3 face transmissibilities, 1/diag, rhs, "
+                         "unknown in and out, FP32), once per launch")
+DTYPE_BY_VARIANT = {
+    5: "f32 sweeps (inner operator stored as bf16) in f64 defect correction",
+    7: "f32 sweeps (inner operator stored as bf16, rhs as fp16) in f64 defect "
+       "correction"}
 BYTES_FP64_SWEEP = 56
 # sweep kernel (name as rocprofv3 reports it, without template arguments)
 # per plan variant
@@ -226,8 +242,8 @@ def main(argv=None):
     ev1.record(stream)
     torch.cuda.synchronize()
     launch_s = ev0.elapsed_time(ev1) * 1e-3 / launches
-    # algorithmic bytes of one launch: every operand once (28 B per cell); the
-    # ns sweeps a launch fuses share that one pass
+    # algorithmic bytes of one launch: every operand once (the plan's bytes
+    # per cell); the ns sweeps a launch fuses share that one pass
     variant = plan32[4] if len(plan32) > 4 else 0
     cell_bytes = BYTES_FP32_SWEEP_BY_VARIANT.get(variant, BYTES_FP32_SWEEP)
     bytes_launch = cell_bytes * N
@@ -322,9 +338,7 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("f32 sweeps (inner operator stored as bf16) in f64 defect "
-                  "correction" if cell_bytes == 20 else
-                  "f32 sweeps in f64 defect correction"),
+        "dtype": DTYPE_BY_VARIANT.get(variant, "f32 sweeps in f64 defect correction"),
         "data": "synthetic",
         "config": {"workload": f"pressure solve {nx}x{ny}x{nz}, 5 m cells, "
                                "sigma_lnK 2.5, corr 16 cells, mixed precision "
@@ -347,13 +361,8 @@ def main(argv=None):
                      "traffic": tbytes,
                      "kernel": VARIANT_KERNEL[variant],
                      "algorithmic_bytes_per_launch": bytes_launch,
-                     "algorithmic_bytes_definition": (
-                         "20 B per cell (16-byte pair record of the 3 face "
-                         "transmissibilities and the storage term as bfloat16, "
-                         "rhs, unknown in and out FP32), once per launch"
-                         if cell_bytes == 20 else
-                         "28 B per cell (3 face transmissibilities, 1/diag, "
-                         "rhs, unknown in and out, FP32), once per launch"),
+                     "algorithmic_bytes_definition": BYTES_DEFINITION.get(
+                         variant, BYTES_DEFINITION_FP32),
                      "sweeps_per_launch": ns, "launch_us": launch_s * 1e6,
                      # every fused sweep counted at cell_bytes per cell: the rate a
                      # one-sweep-per-launch kernel would need for the same

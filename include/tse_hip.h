@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TSE_ABI_VERSION 3
+#define TSE_ABI_VERSION 4
 
 enum tse_status {
     TSE_OK = 0,

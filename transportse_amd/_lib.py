@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtse_hip.so")
 
-ABI_VERSION = 3
+ABI_VERSION = 4
 
 FORM_FAITHFUL64 = 0
 FORM_PLAIN64 = 1
