@@ -281,8 +281,7 @@ typedef struct {
     double initial_rel; /* relative residual of the initial iterate         */
     double rel_residual;
     double contraction; /* residual contraction per sweep over the last
-                         * cycle after the first (single solves; 0 if none,
-                         * and in a batch's status) */
+                         * cycle after the first (0 if none)               */
 } tse_fast_result;
 
 /* Assembles the operator from compute_P's inputs at time step dt (Pn = the

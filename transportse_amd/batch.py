@@ -27,6 +27,7 @@ class MemberStatus:
     active: bool
     initial_rel: float
     rel_residual: float
+    contraction: float = 0.0   # per sweep, last cycle after the first (0: none)
 
 
 class Batch:
@@ -107,7 +108,7 @@ class Batch:
         _lib.check(self.lib.tse_batch_status(self.h, slot, ctypes.byref(r)),
                    "tse_batch_status")
         return MemberStatus(r.sweeps, r.cycles, bool(r.converged), bool(r.pad),
-                            r.initial_rel, r.rel_residual)
+                            r.initial_rel, r.rel_residual, r.contraction)
 
     def close(self):
         """Hand every member's iterate back to its context, then free the

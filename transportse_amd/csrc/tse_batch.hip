@@ -48,6 +48,7 @@ struct BStat {      // per-slot state, device-resident, mirrored to the host
     double bnorm;   // ||b|| (1 if b = 0)
     double rel;     // last relative residual
     double rel0;    // initial relative residual
+    double rel_prev;  // relative residual before the last cycle (0: none)
 };
 
 __global__ void batch_reset_kernel(unsigned *queue) {
@@ -232,9 +233,11 @@ batch_reduce_kernel(const BMember *__restrict__ mem, BStat *__restrict__ st,
             x.bnorm = bn > 0 ? bn : 1.0;
             x.rel = rn / x.bnorm;
             x.rel0 = x.rel;
+            x.rel_prev = 0.0;
             x.sweeps = 0;
             x.cycles = 0;
         } else {
+            x.rel_prev = x.rel;
             x.rel = rn / x.bnorm;
             x.sweeps += inner;
             x.cycles += 1;
@@ -881,7 +884,11 @@ int tse_batch_status(tse_batch *b, int32_t slot, tse_fast_result *out) {
     out->pad = x.active;
     out->initial_rel = x.rel0;
     out->rel_residual = x.rel;
-    out->contraction = 0.0;
+    // per sweep over the last cycle after the first (every cycle of a batch
+    // runs `inner` sweeps), as a single solve reports it
+    out->contraction = x.cycles > 1 && x.rel_prev > 0 && x.rel > 0 && x.rel < x.rel_prev
+                           ? std::exp(std::log(x.rel / x.rel_prev) / b->inner)
+                           : 0.0;
     return 0;
 }
 

@@ -203,7 +203,7 @@ class Ensemble:
                             continue
                         b.writeback(slots[kind][i])
                         if kind == KIND_P and self.tune_omega:
-                            self.scn[i].omega_p.report(st.sweeps)
+                            self.scn[i].omega_p.report(st.sweeps, st.contraction)
                         solves += 1
                     for i, it in resume_all(gens, ids).items():
                         if it is None:
@@ -271,7 +271,7 @@ class Ensemble:
                         continue
                     b.writeback(slots[kind][i])
                     if kind == KIND_P and self.tune_omega:
-                        self.scn[i].omega_p.report(st.sweeps)
+                        self.scn[i].omega_p.report(st.sweeps, st.contraction)
                     solves += 1
                 for i, it in resume_all(gens, done).items():
                     if it is None:

@@ -24,16 +24,19 @@ whose contraction is clearly slower than omega - 1 therefore ran below the
 optimum, and mu = (lam + omega - 1) / (omega sqrt(lam)) gives the optimum
 omega_opt = 2 / (1 + sqrt(1 - mu^2)) (Young, Iterative Solution of Large
 Linear Systems, 1971, ch. 6; Hageman and Young, Applied Iterative Methods,
-1981, ch. 9).  The tuner moves there, then refines from just above it with
-small hill-climbing steps, where the sweep count rises gently; a solve
-whose contraction is about omega - 1 sends the next move downwards.
-Scenarios report sweep counts only (batched solves observe no contraction,
-and the estimate is continuous in it), so a scenario's factors stay the same
-whether it runs alone, batched or in a server; single solves (the pressure
-benchmark's warm-up) report both.  The
-irregular sweep counts of adaptive cycles (a few percent between
-neighbouring factors) mislead a hill climb alone; the estimate does not
-depend on them.
+1981, ch. 9).  The tuner moves there (rounded up to 1/128 in u), then refines
+from just above it with small hill-climbing steps, where the sweep count
+rises gently.  A solve whose contraction is about omega - 1 ran at or above
+the optimum: until the first move to an estimate, the tuner then doubles
+2 - omega (u - 1) per solve, since above the optimum the sweep count falls
+monotonically towards it, and the first solve below it yields the
+estimate.  The irregular sweep counts of adaptive cycles (a few percent
+between neighbouring factors) mislead a hill climb alone; the estimate
+does not depend on them.  Batched solves (fixed cycles) and single ones
+report the same contraction, computed from the member's own residuals, so
+a scenario's factors are the same alone, batched or in a server wherever
+its arithmetic is; the rounding of the estimate keeps them the same where
+different kernels give residuals a few ulps apart.
 """
 from __future__ import annotations
 
@@ -45,7 +48,9 @@ class OmegaTuner:
     # below the optimum (the measured one above it is within ~1 % of
     # omega - 1 in rate)
     BELOW = 0.9
-    MAX_JUMPS = 3
+    MAX_JUMPS = 2             # later estimates only steer the hill climb
+    GRID = 128                # estimates rounded up to 1/GRID in u (the
+                              # gentle side of the optimum)
 
     def __init__(self, n_max: int, step: float = 0.5, u_min: float = 2.0,
                  u_max: float = 10.0, min_step: float = 1.0 / 32):
@@ -81,17 +86,25 @@ class OmegaTuner:
         residual contraction per sweep it observed); picks the next omega."""
         est = self.model_u(contraction) if contraction else None
         # a contraction of about omega - 1: at or above the optimum, so a
-        # move that continues goes down
+        # move that continues goes down; an estimate: below it, so up
         above = 0.0 < contraction < 1.0 and est is None
-        if est is not None and est - self.u >= self.min_step and \
-                self.jumps < self.MAX_JUMPS:
+        target = None if est is None else \
+            min(max(math.ceil(est * self.GRID) / self.GRID, self.u_min), self.u_max)
+        if target is not None and target > self.u and self.jumps < self.MAX_JUMPS:
             if self.best_s is None or sweeps < self.best_s:
                 self.best_u, self.best_s = self.u, sweeps
-            # below the optimum: go to the estimate, then refine upwards
+            # below the optimum: go to the estimate, then refine
             self.jumps += 1
-            self.u = min(max(est, self.u_min), self.u_max)
+            self.u = target
             self.dir = 1.0
             self.step = min(self.step, 1.0 / 16)
+            return
+        if above and self.jumps == 0 and self.u - 1.0 >= self.u_min:
+            # descending from above the optimum: every step down pays
+            if self.best_s is None or sweeps < self.best_s:
+                self.best_u, self.best_s = self.u, sweeps
+            self.u -= 1.0
+            self.dir = -1.0
             return
         if self.best_s is None or sweeps < self.best_s:
             self.best_u, self.best_s = self.u, sweeps
@@ -99,6 +112,8 @@ class OmegaTuner:
                 return
             if above:
                 self.dir = -1.0
+            elif est is not None:
+                self.dir = 1.0
             nxt = self.u + self.dir * self.step
         else:
             self.dir = -self.dir

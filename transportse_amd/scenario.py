@@ -589,9 +589,7 @@ class DeviceScenario:
                 r = sysm.solve(form, tol=tol, sor=om, inner=inner_p,
                                plan=plan_p, max_sweeps=max_sweeps, adaptive=adaptive)
                 if sor_p is None:
-                    # sweeps only, as the batched drivers report: the
-                    # scenario's factors then do not depend on its driver
-                    self.omega_p.report(r.sweeps)
+                    self.omega_p.report(r.sweeps, r.contraction)
             else:
                 r = sysm.solve(form, tol=tol, sor=sor_tr, inner=inner_tr,
                                plan=plan_tr, max_sweeps=max_sweeps)

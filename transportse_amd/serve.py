@@ -210,7 +210,7 @@ class Server:
                         continue
                     b.writeback(slots[kind])
                     if kind == KIND_P:
-                        scn.omega_p.report(st.sweeps)
+                        scn.omega_p.report(st.sweeps, st.contraction)
                     r.solves += 1
                 nxt_items = resume_all({rid: live[rid][2] for rid in fin}, fin)
                 for rid in fin:
