@@ -163,3 +163,17 @@ def test_no_contraction_keeps_the_hill_climb():
         assert a.omega == b.omega
         a.report(f(a.omega))
         b.report(f(b.omega), 0.0)
+
+
+def test_contraction_near_one_below_the_optimum_moves_up():
+    """A contraction slower than omega - 1 but too close to 1 to resolve mu
+    (mu rounds to >= 1) still means "below the optimum": the tuner moves up,
+    not down."""
+    t = OmegaTuner(64)
+    u0 = t.u
+    lam = 1.0 - 1e-17          # rounds to 1 - 2^-53 or to 1 itself
+    lam = min(lam, math.nextafter(1.0, 0.0))
+    assert not t.is_above(lam)
+    assert t.model_u(lam) == t.u_max
+    t.report(1000, lam)
+    assert t.u > u0

@@ -66,18 +66,25 @@ class OmegaTuner:
     def omega(self) -> float:
         return 2.0 - 2.0 ** (-self.u)
 
+    def is_above(self, lam: float) -> bool:
+        """Whether the contraction per sweep `lam` observed at the current
+        factor is about omega - 1, i.e. the factor is at or above the
+        optimum."""
+        om = self.omega
+        return 0.0 < lam < 1.0 and 1.0 < om < 2.0 and \
+            math.log(lam) <= self.BELOW * math.log(om - 1.0)
+
     def model_u(self, lam: float):
         """u of the optimal factor from the contraction per sweep `lam`
         observed at the current factor, if it shows that factor to be below
-        the optimum; else None."""
+        the optimum (u_max if lam is too close to 1 to resolve mu); else
+        None."""
         om = self.omega
-        if not (0.0 < lam < 1.0) or not (1.0 < om < 2.0):
+        if not (0.0 < lam < 1.0) or not (1.0 < om < 2.0) or self.is_above(lam):
             return None
-        if math.log(lam) <= self.BELOW * math.log(om - 1.0):
-            return None           # about omega - 1: at or above the optimum
         mu = (lam + om - 1.0) / (om * math.sqrt(lam))
-        if not (0.0 < mu < 1.0):
-            return None
+        if mu >= 1.0:
+            return self.u_max
         om_opt = 2.0 / (1.0 + math.sqrt(1.0 - mu * mu))
         return -math.log2(2.0 - om_opt)
 
@@ -87,7 +94,7 @@ class OmegaTuner:
         est = self.model_u(contraction) if contraction else None
         # a contraction of about omega - 1: at or above the optimum, so a
         # move that continues goes down; an estimate: below it, so up
-        above = 0.0 < contraction < 1.0 and est is None
+        above = self.is_above(contraction)
         target = None if est is None else \
             min(max(math.ceil(est * self.GRID) / self.GRID, self.u_min), self.u_max)
         if target is not None and target > self.u and self.jumps < self.MAX_JUMPS:
