@@ -72,7 +72,7 @@ def test_variant2_plans_filtered_by_field_size():
     assert plan_fits(5, (512, 512, 256), True)            # 8-byte records: 512 MiB
     assert not plan_fits(5, (512, 512, 256), False)       # FP32 sweeps only
     assert not plan_fits(5, (1024, 512, 512), True)       # 2 GiB of records
-    assert plan_fits(7, (512, 512, 256), True)            # variant 5 + bfloat16 rhs
+    assert plan_fits(7, (512, 512, 256), True)            # variant 5 + FP16 rhs
     assert not plan_fits(7, (512, 512, 256), False)
     assert not plan_fits(7, (1024, 512, 512), True)
 

@@ -84,7 +84,7 @@ rb_zmarch_x2_kernel(ZGeo g, const T *__restrict__ Pin, T *__restrict__ Pout,
 
 // Variant 5: cell pairs on the bfloat16 inner operator (rec: 16-byte pair
 // records of packed Wx, Wy, Wz and V R; the diagonal summed from the rounded
-// values); RB16 (variant 7): the rhs as packed bfloat16 pairs as well.
+// values); RB16 (variant 7): the rhs as packed FP16 pairs (scaled).
 template <int NS, int BYT, bool RB16>
 __global__ void __launch_bounds__(64 * BYT)
 rb_zmarch_x5_kernel(ZGeo g, const float *__restrict__ Pin, float *__restrict__ Pout,
@@ -611,7 +611,7 @@ static int launch_zmarch(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &p
     return fail(TSE_ERR_ARG, "unsupported sweep plan (ns, by, ry, variant)");
 }
 
-// Variants 5 and 7; rhs: FP32 (5) or bfloat16 (7) values.
+// Variants 5 and 7; rhs: FP32 (5) or scaled FP16 (7) values.
 static int launch_zmarch5(hipStream_t st, const ZGeo &zg, const tse_sweep_plan &pl,
                           const float *Pin, float *Pout, const FastOp &op,
                           const void *rhs, float omega, int zero_init) {
