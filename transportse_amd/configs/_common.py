@@ -95,16 +95,29 @@ def build(ns, nx, ny, nz, dx=10.0, dy=10.0, dz=None, beta=0.0, sigma=1.0,
     def chemical_reactions(C, P, T, rho_f, phi, Kx, Ky, Kz, dt_chem, i, j, k):
         return C, P, T, rho_f, phi, Kx, Ky, Kz
 
+    # pore volume per cell, phi * V (phi is constant in these configs: no
+    # chemistry, and the EOS leaves it alone); recomputed if ns['phi'] is
+    # rebound.  The mass sums are then one dot product each (256x256x128:
+    # 2 ms against 50 ms for the product-then-sum form)
+    pore = {}
+
+    def pore_volume():
+        if pore.get('phi') is not ns['phi']:
+            pore['phi'] = ns['phi']
+            pore['w'] = numpy.ascontiguousarray(ns['phi'] * vol).ravel()
+        return pore['w']
+
     def mass_balance(C, qx, qy, qz, WP, WC, WT, D_fx, D_fy, D_fz, dt, mass):
         """[salt mass in place, salt advected out through the x = 0 / x = lx
         Dirichlet faces (cumulative), fluid mass in place] in kg."""
         mass = numpy.array(mass, dtype=float)
-        mass[0] = float(numpy.sum(C[:, :, :, 0] * ns['phi'] * vol))
+        w = pore_volume()
+        mass[0] = float(numpy.dot(numpy.ravel(C[:, :, :, 0]), w))
         area = ns['dy'][:, None] * dzs[None, :]
         out = (numpy.sum(numpy.maximum(qx[-2], 0.0) * C[-2, :, :, 0] * area)
                + numpy.sum(numpy.maximum(-qx[0], 0.0) * C[1, :, :, 0] * area))
         mass[1] += dt * float(out)
-        mass[2] = float(numpy.sum(ns['rho_f'] * ns['phi'] * vol))
+        mass[2] = float(numpy.dot(numpy.ravel(ns['rho_f']), w))
         return mass
 
     ns['fluid_properties'] = fluid_properties
