@@ -190,8 +190,11 @@ def main(argv=None):
     explicit_omega = args.omega is not None
 
     def step(form, omega):
+        # with the factor from the tuner (no --omega), each solve may also
+        # move it inside the solve (adaptive SOR, Young's estimate)
         fp.assemble()
-        st = fp.solve(form, tol=args.tol, sor=omega, adaptive=True)
+        st = fp.solve(form, tol=args.tol, sor=omega, adaptive=True,
+                      adapt_sor=not explicit_omega)
         fp.result_device()  # solution back into a split-layout device field
         return st
 
@@ -201,11 +204,11 @@ def main(argv=None):
         st = step("mixed", om)
         if w == 0:   # the first solve: the tuner's starting factor, untuned
             cold = {"time_to_solution_s": st.seconds, "sweeps": st.sweeps,
-                    "omega": om, "converged": st.converged,
+                    "omega": om, "omega_final": st.sor, "converged": st.converged,
                     "plan_autotune_s": tune_s if not args.plan else 0.0}
         if not args.omega:
-            tuner.report(st.sweeps, st.contraction)
-        log(f"warmup solve (omega {om:.5f}): {st.sweeps} sweeps, rel "
+            tuner.report(st.sweeps, st.contraction, st.sor)
+        log(f"warmup solve (omega {om:.5f} -> {st.sor:.5f}): {st.sweeps} sweeps, rel "
             f"{st.rel_residual:.2e}, {st.seconds:.2f}s, contraction per sweep "
             f"{st.contraction:.6f}")
     if not args.omega:      # the best factor the warm-up solves found

@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define TSE_ABI_VERSION 4
+#define TSE_ABI_VERSION 5
 
 enum tse_status {
     TSE_OK = 0,
@@ -260,8 +260,14 @@ typedef struct {
     int32_t inner;      /* sweeps per residual check / correction cycle     */
     int32_t max_sweeps; /* stop after this many sweeps                      */
     int32_t cycle_mode; /* TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE (solve only) */
-    int32_t pad;
-    double sor;         /* over-relaxation factor                           */
+    int32_t adapt_sor;  /* 1 (single solves only; a batch rejects it): after
+                         * each cycle from the second on whose contraction
+                         * shows sor below the optimum, continue at the
+                         * estimate of tse_sor_model (rounded up to 1/128 in
+                         * log2(2 - sor); at most twice per solve); 0: sor
+                         * throughout                                        */
+    double sor;         /* over-relaxation factor (the starting one if
+                         * adapt_sor)                                        */
     double tol;         /* relative residual target                         */
 } tse_fast_params;
 /* Cycle lengths of tse_fast_*_solve.  FIXED: every cycle is `inner` sweeps.
@@ -281,8 +287,24 @@ typedef struct {
     double initial_rel; /* relative residual of the initial iterate         */
     double rel_residual;
     double contraction; /* residual contraction per sweep over the last
-                         * cycle after the first (0 if none)               */
+                         * cycle after the first, if that cycle of n sweeps
+                         * at factor sor spans n |log(sor - 1)| >= 1/4
+                         * (shorter ones are dominated by their restart
+                         * transient); else 0                               */
+    double sor;         /* over-relaxation factor of the last cycle (single
+                         * solves; the batch's or the member's in a status) */
 } tse_fast_result;
+
+/* Young's relation for red/black SOR on a consistently ordered matrix (the
+ * 7-point pressure operator in red/black order): from the residual
+ * contraction per sweep `contraction` observed at factor `omega`, returns 1
+ * and the optimal factor in *omega_opt when the contraction is clearly
+ * slower than omega - 1 (omega below the optimum; the estimate is capped at
+ * 2 - 2^-10 when it cannot be resolved), 0 when it is about omega - 1 (at or
+ * above the optimum), TSE_ERR_ARG for inputs outside (0, 1) and (1, 2).
+ * Host only.  No counterpart in the reference: the adaptive-SOR estimate of
+ * the performance forms (Young 1971, Hageman and Young 1981). */
+int tse_sor_model(double omega, double contraction, double *omega_opt);
 
 /* Assembles the operator from compute_P's inputs at time step dt (Pn = the
  * storage-term reference pressure; NULL uses P) and loads P as the iterate. */

@@ -177,3 +177,34 @@ def test_contraction_near_one_below_the_optimum_moves_up():
     assert t.model_u(lam) == t.u_max
     t.report(1000, lam)
     assert t.u > u0
+
+
+def test_library_estimate_matches_the_tuner_model():
+    """tse_sor_model (the estimate the library's adaptive-SOR solves use)
+    agrees with OmegaTuner.model_u on a grid of factors and contractions,
+    both in deciding "below the optimum" and in the estimate (the tuner's
+    estimate is itself pinned on an operator's SOR spectrum above)."""
+    import ctypes
+    import numpy as np
+    from transportse_amd import _lib
+    lib = _lib.load()
+    out = ctypes.c_double()
+    for om in (1.5, 1.8, 1.95, 1.99, 1.99609375, 1.999):
+        t = OmegaTuner(8)
+        t.u = -math.log2(2 - om)
+        for lam in np.concatenate([np.linspace(0.2, 0.999, 40), 1 - np.logspace(-9, -2, 30)]):
+            rc = lib.tse_sor_model(om, float(lam), ctypes.byref(out))
+            u = t.model_u(float(lam))
+            assert rc in (0, 1)
+            assert (rc == 1) == (u is not None), (om, lam)
+            if rc == 1:
+                assert math.isclose(out.value, 2 - 2 ** -u, rel_tol=0, abs_tol=1e-12), \
+                    (om, lam, out.value, 2 - 2 ** -u)
+    assert lib.tse_sor_model(2.0, 0.5, ctypes.byref(out)) < 0
+    assert lib.tse_sor_model(1.9, 1.0, ctypes.byref(out)) < 0
+
+
+def test_report_adopts_the_factor_a_solve_ended_at():
+    t = OmegaTuner(512)
+    t.report(7000, 0.9966, omega=1.99667)
+    assert abs(t.omega - 1.99667) < 2e-4 and t.step <= 1 / 16 and t.jumps >= 1

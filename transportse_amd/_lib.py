@@ -11,7 +11,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtse_hip.so")
 
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 FORM_FAITHFUL64 = 0
 FORM_PLAIN64 = 1
@@ -55,7 +55,7 @@ CYCLE_FIXED, CYCLE_ADAPTIVE = 0, 1
 class FastParams(ctypes.Structure):
     _fields_ = [("form", ctypes.c_int32), ("plan", SweepPlan),
                 ("inner", ctypes.c_int32), ("max_sweeps", ctypes.c_int32),
-                ("cycle_mode", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("cycle_mode", ctypes.c_int32), ("adapt_sor", ctypes.c_int32),
                 ("sor", ctypes.c_double), ("tol", ctypes.c_double)]
 
 
@@ -64,7 +64,7 @@ class FastResult(ctypes.Structure):
                 ("converged", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("initial_rel", ctypes.c_double),
                 ("rel_residual", ctypes.c_double),
-                ("contraction", ctypes.c_double)]
+                ("contraction", ctypes.c_double), ("sor", ctypes.c_double)]
 
 
 class EosParams(ctypes.Structure):
@@ -90,6 +90,7 @@ _I, _D, _L, _V = ctypes.c_int, ctypes.c_double, ctypes.c_int64, ctypes.c_void_p
 _DPH = ctypes.POINTER(ctypes.c_double)
 SIGNATURES = {
     "tse_abi_version": (_I, []),
+    "tse_sor_model": (_I, [_D, _D, _DPH]),
     "tse_last_error": (ctypes.c_char_p, []),
     "tse_ctx_create": (_I, [ctypes.POINTER(_V), _I, _V]),
     "tse_ctx_destroy": (_I, [_V]),

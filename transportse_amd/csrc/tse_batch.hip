@@ -648,6 +648,8 @@ int tse_batch_begin(tse_batch *b, int32_t kind, const tse_fast_params *prm) {
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
     if (pl.variant == 7)   // the batch residual writes an FP32 rhs
         return fail(TSE_ERR_ARG, "sweep variant 7 runs single systems only");
+    if (prm->adapt_sor)    // members keep their own factor (tse_batch_set_sor)
+        return fail(TSE_ERR_ARG, "adapt_sor is for single solves");
     for (int s = 0; s < b->cap; ++s)
         if (b->hused[s]) return fail(TSE_ERR_STATE, "batch still has members");
     b->kind = kind;
@@ -885,10 +887,16 @@ int tse_batch_status(tse_batch *b, int32_t slot, tse_fast_result *out) {
     out->initial_rel = x.rel0;
     out->rel_residual = x.rel;
     // per sweep over the last cycle after the first (every cycle of a batch
-    // runs `inner` sweeps), as a single solve reports it
-    out->contraction = x.cycles > 1 && x.rel_prev > 0 && x.rel > 0 && x.rel < x.rel_prev
-                           ? std::exp(std::log(x.rel / x.rel_prev) / b->inner)
-                           : 0.0;
+    // runs `inner` sweeps), as a single solve reports it: only from a cycle
+    // that spans inner |log(sor - 1)| >= 1/4
+    const double sor = b->hmem[slot].sor;
+    const bool resolved = sor > 1.0 && sor < 2.0 &&
+                          b->inner * std::fabs(std::log(sor - 1.0)) >= 0.25;
+    out->contraction =
+        resolved && x.cycles > 1 && x.rel_prev > 0 && x.rel > 0 && x.rel < x.rel_prev
+            ? std::exp(std::log(x.rel / x.rel_prev) / b->inner)
+            : 0.0;
+    out->sor = sor;
     return 0;
 }
 

@@ -816,6 +816,27 @@ static int sys_sweeps(tse_ctx *ctx, FastSys *s, const tse_sweep_plan *plan,
     return correction_sweeps(ctx, s, zg, *plan, n_sweeps, (float)sor, &e);
 }
 
+// Young's relation (see tse_sor_model): 1 and the optimum when the
+// contraction per sweep lam at omega shows omega below it, 2 when it does
+// but the optimum is not resolved (mu >= 1 or past the cap: *omega_opt =
+// the cap), else 0.
+static int sor_model(double omega, double lam, double *omega_opt) {
+    if (std::log(lam) <= 0.9 * std::log(omega - 1.0)) return 0;   // about omega - 1
+    const double mu = (lam + omega - 1.0) / (omega * std::sqrt(lam));
+    const double cap = 2.0 - 0x1p-10;
+    const double opt = mu >= 1.0 ? cap : 2.0 / (1.0 + std::sqrt(1.0 - mu * mu));
+    *omega_opt = std::min(cap, opt);
+    return opt >= cap ? 2 : 1;
+}
+
+// Whether a cycle of n sweeps at omega resolves a contraction near
+// omega - 1: the cycle must span n |log(omega - 1)| >= 1/4 (a reduction by
+// e^-1/4 at that rate); shorter cycles (omega near 2) are dominated by the
+// transient after their restart from e = 0.
+static bool resolves_contraction(int n, double omega) {
+    return omega > 1.0 && omega < 2.0 && n * std::fabs(std::log(omega - 1.0)) >= 0.25;
+}
+
 static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
                      tse_fast_result *out) {
     if (!s->ready || !prm || !out) return fail(TSE_ERR_STATE, "fast form not set up");
@@ -830,6 +851,8 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
         return fail(TSE_ERR_ARG, "unsupported plan / inner sweep count");
     if (prm->cycle_mode != TSE_CYCLE_FIXED && prm->cycle_mode != TSE_CYCLE_ADAPTIVE)
         return fail(TSE_ERR_ARG, "cycle_mode must be TSE_CYCLE_FIXED or TSE_CYCLE_ADAPTIVE");
+    if (prm->adapt_sor != 0 && prm->adapt_sor != 1)
+        return fail(TSE_ERR_ARG, "adapt_sor must be 0 or 1");
     if (!fast_plan_fits(pl, s->N, fp32))
         return fail(TSE_ERR_ARG, "sweep variants 2, 3, 5 and 7 need fields of at most 1 GiB "
                                  "(variants 5 and 7: 8 bytes per cell)");
@@ -864,14 +887,15 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     out->initial_rel = rn / bs;
     out->contraction = 0.0;
     double rel = rn / bs;
-    int sweeps = 0, cycles = 0;
+    int sweeps = 0, cycles = 0, switches = 0;
+    double sor = prm->sor;
     while (rel > prm->tol && sweeps < prm->max_sweeps) {
         // the last cycle ends at max_sweeps (rounded up to the plan's ns)
         const int left = (prm->max_sweeps - sweeps + pl.ns - 1) / pl.ns * pl.ns;
         const int run = std::min(chunk, left);
         if (fp32) {
             int e;
-            rc = correction_sweeps(ctx, s, zg, pl, run, (float)prm->sor, &e);
+            rc = correction_sweeps(ctx, s, zg, pl, run, (float)sor, &e);
             if (rc) return rc;
             const double escale = 1.0 / rscale;   // exact: a power of two
             rscale = r16 ? rhs_scale(rn) : 1.0;
@@ -880,7 +904,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             if (rc) return rc;
             s->cur = 1 - s->cur;
         } else {
-            rc = plain_sweeps(ctx, s, zg, pl, run, prm->sor);
+            rc = plain_sweeps(ctx, s, zg, pl, run, sor);
             if (rc) return rc;
             rc = residual<0>(ctx, s, s->p(s->cur), nullptr, nullptr, &rn, &bn);
             if (rc) return rc;
@@ -892,8 +916,21 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
         if (!(rel == rel)) break;  // NaN
         // per-sweep contraction of this cycle (the first one starts with the
         // fast-decaying components, so only later cycles are reported)
-        if (cycles > 1 && rel < rel_prev && rel > 0)
+        if (cycles > 1 && rel < rel_prev && rel > 0 && resolves_contraction(run, sor)) {
             out->contraction = std::exp(std::log(rel / rel_prev) / run);
+            // adaptive SOR: a cycle below the optimum moves the factor to
+            // the estimate (the next cycle starts from e = 0 anyway)
+            double opt;
+            if (prm->adapt_sor && switches < 2 && rel > prm->tol &&
+                sor_model(sor, out->contraction, &opt) == 1) {
+                const double u = std::ceil(-std::log2(2.0 - opt) * 128.0) / 128.0;
+                const double next = 2.0 - std::exp2(-u);
+                if (next > sor) {
+                    sor = next;
+                    ++switches;
+                }
+            }
+        }
         if (prm->cycle_mode == TSE_CYCLE_ADAPTIVE) {
             // sweeps still needed at the last cycle's rate, clamped
             long next = inner;
@@ -918,6 +955,7 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
     out->cycles = cycles;
     out->rel_residual = rel;
     out->converged = rel <= prm->tol ? 1 : 0;
+    out->sor = sor;
     return 0;
 }
 
@@ -966,6 +1004,13 @@ int tse_fast_p_operator(tse_ctx *ctx, double *Wx, double *Wy, double *Wz, double
             TSE_HIP_TRY(hipMemcpyAsync(dst[a], src[a], sizeof(double) * s->N,
                                        hipMemcpyDeviceToDevice, ctx->stream));
     return 0;
+}
+
+int tse_sor_model(double omega, double contraction, double *omega_opt) {
+    if (!omega_opt || !(contraction > 0.0 && contraction < 1.0) ||
+        !(omega > 1.0 && omega < 2.0))
+        return fail(TSE_ERR_ARG, "tse_sor_model: need 0 < contraction < 1 < omega < 2");
+    return sor_model(omega, contraction, omega_opt) ? 1 : 0;
 }
 
 int tse_fast_p_residual(tse_ctx *ctx, double *rel) {

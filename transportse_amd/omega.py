@@ -77,8 +77,8 @@ class OmegaTuner:
     def model_u(self, lam: float):
         """u of the optimal factor from the contraction per sweep `lam`
         observed at the current factor, if it shows that factor to be below
-        the optimum (u_max if lam is too close to 1 to resolve mu); else
-        None."""
+        the optimum (at most u_max, which is also the answer when lam is too
+        close to 1 to resolve mu); else None."""
         om = self.omega
         if not (0.0 < lam < 1.0) or not (1.0 < om < 2.0) or self.is_above(lam):
             return None
@@ -86,11 +86,22 @@ class OmegaTuner:
         if mu >= 1.0:
             return self.u_max
         om_opt = 2.0 / (1.0 + math.sqrt(1.0 - mu * mu))
-        return -math.log2(2.0 - om_opt)
+        return min(-math.log2(2.0 - om_opt), self.u_max)
 
-    def report(self, sweeps: int, contraction: float = 0.0) -> None:
+    def report(self, sweeps: int, contraction: float = 0.0, omega=None) -> None:
         """Sweeps the last solve at ``omega`` took (and, if known, the
-        residual contraction per sweep it observed); picks the next omega."""
+        residual contraction per sweep it observed); picks the next omega.
+        omega: the factor the solve ended at, when it adapted its own factor
+        (adaptive SOR inside the solve, which moves to the same estimate);
+        the tuner continues from there as after one of its own moves to an
+        estimate."""
+        if omega is not None and 1.0 < omega < 2.0:
+            u = min(max(-math.log2(2.0 - omega), self.u_min), self.u_max)
+            if abs(u - self.u) > 1e-9:
+                self.u = u
+                self.jumps = max(self.jumps, 1)
+                self.dir = 1.0
+                self.step = min(self.step, 1.0 / 16)
         est = self.model_u(contraction) if contraction else None
         # a contraction of about omega - 1: at or above the optimum, so a
         # move that continues goes down; an estimate: below it, so up

@@ -36,6 +36,7 @@ class SolveStats:
     seconds: float
     plan: tuple
     contraction: float = 0.0   # per sweep, last cycle after the first (0: none)
+    sor: float = 0.0           # over-relaxation factor of the last cycle
 
 
 class FastSystem:
@@ -74,10 +75,13 @@ class FastSystem:
         return self.plans[form]
 
     def solve(self, form="mixed", tol=1e-10, sor=1.996, max_sweeps=200000,
-              inner=None, plan=None, adaptive=False) -> SolveStats:
+              inner=None, plan=None, adaptive=False, adapt_sor=False) -> SolveStats:
         """Solve to relative residual `tol`.  adaptive: cycle lengths from
         the observed convergence rate (first cycle `inner`, then up to
-        8*inner), fewer FP64 residual passes; else every cycle is `inner`."""
+        8*inner), fewer FP64 residual passes; else every cycle is `inner`.
+        adapt_sor: `sor` is the starting factor; a cycle whose contraction
+        shows it below the optimum moves it to Young's estimate (at most
+        twice; ``SolveStats.sor`` is the last cycle's)."""
         pl = plan or self.plan(form)
         ns = pl[0]
         inner = inner or 100
@@ -85,7 +89,7 @@ class FastSystem:
         prm = _lib.FastParams(form=FORMS[form], plan=_lib.SweepPlan(*pl),
                               inner=inner, max_sweeps=int(max_sweeps),
                               cycle_mode=_lib.CYCLE_ADAPTIVE if adaptive
-                              else _lib.CYCLE_FIXED,
+                              else _lib.CYCLE_FIXED, adapt_sor=int(bool(adapt_sor)),
                               sor=float(sor), tol=float(tol))
         res = _lib.FastResult()
         t0 = time.perf_counter()
@@ -94,7 +98,7 @@ class FastSystem:
         sec = time.perf_counter() - t0
         return SolveStats(form, res.sweeps, res.cycles, bool(res.converged),
                           res.initial_rel, res.rel_residual, sec, tuple(pl),
-                          res.contraction)
+                          res.contraction, res.sor)
 
     def result_device(self):
         """Write the current iterate into a split-layout device field (kept
