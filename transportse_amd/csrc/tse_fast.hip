@@ -932,9 +932,12 @@ static int sys_solve(tse_ctx *ctx, FastSys *s, const tse_fast_params *prm,
             }
         }
         if (prm->cycle_mode == TSE_CYCLE_ADAPTIVE) {
-            // sweeps still needed at the last cycle's rate, clamped
+            // sweeps still needed at the last cycle's rate, clamped; with
+            // adaptive SOR the second cycle stays `inner` long, so the first
+            // estimate of the factor comes early
             long next = inner;
-            if (rel < rel_prev && rel > prm->tol && rel_prev > 0) {
+            if (rel < rel_prev && rel > prm->tol && rel_prev > 0 &&
+                !(prm->adapt_sor && cycles == 1)) {
                 const double rate = std::log(rel / rel_prev) / run;     // < 0
                 // a variant-5 or -7 cycle cannot reduce the error by much
                 // more than its inner operator's 2^-8 accuracy: aim at most
