@@ -90,3 +90,22 @@ def test_batch_rejects_adaptive_sor():
         assert b"single solves" in lib.tse_last_error()
     finally:
         lib.tse_batch_destroy(b)
+
+
+@pytest.mark.parametrize("form,plan", [(2, (3, 32, 2, 0, 7)), (2, (3, 32, 2, 0, 5)),
+                                       (1, (2, 16, 1, 0, 2))])
+def test_solve_that_runs_no_cycle(form, plan):
+    """Edge cases of the adaptive and FP16-rhs paths: a solve whose initial
+    iterate already meets the tolerance runs no cycle (no contraction, the
+    factor unchanged, the iterate unchanged bit for bit), and so does one on
+    an all-Dirichlet system, whose residual and rhs norms are zero (the FP16
+    rhs scale falls back to 1)."""
+    f = Fields(40, 24, 20, seed=44, dirichlet_p=True)
+    P, r = _solve(f, form, plan, 1.9, True, 1e9)
+    assert r.converged and r.sweeps == 0 and r.cycles == 0
+    assert r.contraction == 0.0 and r.sor == 1.9
+    assert np.array_equal(P, f.P)
+    f.BP = np.ones_like(f.BP)
+    P, r = _solve(f, form, plan, 1.9, True, 1e-10)
+    assert r.converged and r.sweeps == 0 and r.rel_residual == 0.0
+    assert np.array_equal(P, f.P)
