@@ -178,7 +178,7 @@ def test_one_cycle_equals_sor_on_the_rounded_operator(shape, ns, zchunk):
     gap = np.max(np.abs(e16 - e32)) / scale
     gap7 = np.max(np.abs(e7 - e16)) / scale
     print("one cycle", shape, (ns, zchunk), "variant 3 vs FP32 model", d3,
-          "variant 5 vs bf16 model", d5, "variant 7 vs bf16-rhs model", d7,
+          "variant 5 vs bf16 model", d5, "variant 7 vs fp16-rhs model", d7,
           "models apart", gap, gap7, flush=True)
     assert scale > 0
     assert d3 <= 2e-5 and d5 <= 2e-5 and d7 <= 2e-5
