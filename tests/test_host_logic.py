@@ -26,10 +26,10 @@ def test_resume_all_skips_waits_and_reports_ends():
     }
     out = resume_all(gens, [0, 1, 2, 3])
     assert out == {0: (0, "p0"), 1: (1, "t1"), 2: None, 3: None}
-    # round 1 takes one item of every generator before any is resumed again,
-    # so all reductions of a round are enqueued before the first read
-    first_round = [name for name, _ in log[:3]]
-    assert first_round == [0, 1, 2]
+    # the first pass takes one item of every generator before any is resumed
+    # again, so all reductions of a pass are enqueued before the first read
+    first_pass = [name for name, _ in log[:3]]
+    assert first_pass == [0, 1, 2]
     assert log[3:] == [(0, (0, "p0")), (2, (KIND_WAIT, None))]
 
 
